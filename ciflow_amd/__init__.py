@@ -1,0 +1,490 @@
+"""ciflow_amd -- MI355X (gfx950) CI ground-state hot path of CIFlow.
+
+Python front end over the C ABI in include/ciflow_amd.h (libciflow_amd.so,
+hand-written HIP kernels).  The classes mirror the reference's interface:
+
+    Hamiltonian            Hamiltonian.h:80-136   (getTmat/getVmat/getEconst/getL/getnup/getndown)
+    FCI, DOCI              CIMethod.h:135-162     (build_ci, solve, get_ci_energy, get_eigvec,
+                                                   construct_density, get_one_rdm/get_two_rdm,
+                                                   get_dens_energy)
+    SparseMatrix_CRS_Sym   SparseMatrix_CRS.h:120-130 (mvprod, arpackDiagonalize)
+
+There is no CPU fallback: importing works everywhere, but every compute call
+needs the built library and a GPU, and raises if either is missing.
+"""
+from __future__ import annotations
+
+import atexit
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libciflow_amd.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "ciflow_amd.h")
+
+CF_OK, CF_ERR_ARG, CF_ERR_HIP, CF_ERR_NOMEM, CF_ERR_NOCONV, CF_ERR_UNSUPPORTED, CF_ERR_STATE = range(7)
+CF_FCI, CF_DOCI = 0, 1
+CF_F64, CF_F32 = 0, 1
+
+_lib = None
+_exiting = False
+
+
+def _mark_exit():
+    global _exiting
+    _exiting = True
+
+
+atexit.register(_mark_exit)
+
+
+class _Handle:
+    """Owns one native handle; keeps its parent handle alive so native objects
+    are destroyed child-first.  Destroy calls are skipped at interpreter exit
+    (the process tears the device context down)."""
+
+    def __init__(self, destroy: str, parent=None):
+        self.ptr = C.c_void_p()
+        self._destroy = destroy
+        self.parent = parent
+
+    def __del__(self):
+        if self.ptr and not _exiting and _lib is not None:
+            try:
+                getattr(_lib, self._destroy)(self.ptr)
+            except Exception:
+                pass
+        self.ptr = C.c_void_p()
+
+
+dp = C.POINTER(C.c_double)
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+vp = C.c_void_p
+
+
+class CIFlowError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"ciflow_amd error {code}: {msg}")
+        self.code = code
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("ncv", C.c_int), ("maxrestart", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
+                ("tol_f32", C.c_double), ("seed", C.c_uint64)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("eigval", C.c_double), ("energy", C.c_double), ("resid", C.c_double), ("n_matvec", C.c_int),
+                ("n_matvec_f32", C.c_int), ("n_restart", C.c_int), ("status", C.c_int), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def header_symbols() -> list[str]:
+    """Entry points declared in include/ciflow_amd.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(cf_\w+)\s*\(", txt, re.M)))
+
+
+def lib():
+    """Load libciflow_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run python -m ciflow_amd.build")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "cf_version": (C.c_int, []),
+        "cf_last_error": (C.c_char_p, []),
+        "cf_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "cf_ctx_destroy": (C.c_int, [vp]),
+        "cf_ctx_synchronize": (C.c_int, [vp]),
+        "cf_ctx_enable_timing": (C.c_int, [vp, C.c_int]),
+        "cf_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_int64), dp]),
+        "cf_ctx_reset_timing": (C.c_int, [vp]),
+        "cf_gen_integrals": (C.c_int, [C.c_int, C.c_uint64, dp, dp]),
+        "cf_ham_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, C.POINTER(vp)]),
+        "cf_ham_destroy": (C.c_int, [vp]),
+        "cf_space_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+        "cf_space_destroy": (C.c_int, [vp]),
+        "cf_space_dims": (C.c_int, [vp, u64p, u64p, u64p]),
+        "cf_strings_unrank": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, u64p]),
+        "cf_strings_rank": (C.c_int, [vp, C.c_int, C.c_int, u64p, C.c_uint64, u32p]),
+        "cf_crs_build": (C.c_int, [vp, C.c_int, C.c_double, C.POINTER(vp)]),
+        "cf_crs_destroy": (C.c_int, [vp]),
+        "cf_crs_info": (C.c_int, [vp, u64p, u64p, u64p]),
+        "cf_crs_count_upper": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p]),
+        "cf_crs_export_upper": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p, u32p, dp]),
+        "cf_crs_count_full": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p]),
+        "cf_crs_export_full": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p, u32p, dp]),
+        "cf_mvprod": (C.c_int, [vp, dp, dp]),
+        "cf_mvprod_dev": (C.c_int, [vp, C.c_int, vp, vp]),
+        "cf_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
+        "cf_solve": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
+        "cf_rci_create": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_uint64, C.POINTER(vp)]),
+        "cf_rci_step": (C.c_int, [vp, C.POINTER(C.c_int), dp, C.POINTER(C.c_int)]),
+        "cf_rci_result": (C.c_int, [vp, dp, dp, C.POINTER(C.c_int)]),
+        "cf_rci_destroy": (C.c_int, [vp]),
+        "cf_rdm": (C.c_int, [vp, dp, dp, dp]),
+        "cf_rdm_dense": (C.c_int, [vp, dp, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(code: int):
+    if code != CF_OK:
+        msg = lib().cf_last_error()
+        raise CIFlowError(code, msg.decode() if msg else "")
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(dp)
+
+
+def gen_integrals(L: int, seed: int):
+    """Seeded synthetic integrals (h[L,L], physics V[L,L,L,L])."""
+    h = np.zeros((L, L))
+    V = np.zeros((L, L, L, L))
+    _check(lib().cf_gen_integrals(L, seed, _d(h), _d(V)))
+    return h, V
+
+
+class Context:
+    """One GPU + one stream (cf_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.h = _Handle("cf_ctx_destroy")
+        _check(lib().cf_ctx_create(device, C.byref(self.h.ptr)))
+
+    @property
+    def ptr(self):
+        return self.h.ptr
+
+    def synchronize(self):
+        _check(lib().cf_ctx_synchronize(self.ptr))
+
+    def enable_timing(self, on: bool = True):
+        _check(lib().cf_ctx_enable_timing(self.ptr, int(on)))
+
+    def reset_timing(self):
+        _check(lib().cf_ctx_reset_timing(self.ptr))
+
+    def kernel_time(self, family: str):
+        n = C.c_int64()
+        ms = C.c_double()
+        _check(lib().cf_ctx_kernel_time(self.ptr, family.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def unrank(self, L: int, nel: int, first: int, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype=np.uint64)
+        _check(lib().cf_strings_unrank(self.ptr, L, nel, first, count, out.ctypes.data_as(u64p)))
+        return out
+
+    def rank(self, L: int, nel: int, strings) -> np.ndarray:
+        s = np.ascontiguousarray(strings, dtype=np.uint64)
+        out = np.zeros(len(s), dtype=np.uint32)
+        _check(lib().cf_strings_rank(self.ptr, L, nel, s.ctypes.data_as(u64p), len(s), out.ctypes.data_as(u32p)))
+        return out
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Hamiltonian:
+    """Integrals on the GPU (Hamiltonian.h:80-136): h[i,j] = getTmat(i,j),
+    V[i,j,k,l] = getVmat(i,j,k,l) (physics), econst = getEconst()."""
+
+    def __init__(self, h, V, nup: int, ndown: int, econst: float = 0.0, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.h = np.ascontiguousarray(h, dtype=np.float64)
+        self.V = np.ascontiguousarray(V, dtype=np.float64)
+        self.L = self.h.shape[0]
+        assert self.h.shape == (self.L, self.L) and self.V.shape == (self.L,) * 4
+        self.nup, self.ndown, self.econst = int(nup), int(ndown), float(econst)
+        self.handle = _Handle("cf_ham_destroy", self.ctx.h)
+        _check(lib().cf_ham_create(self.ctx.ptr, self.L, self.nup, self.ndown, _d(self.h), _d(self.V), self.econst,
+                                   C.byref(self.handle.ptr)))
+
+    @property
+    def ptr(self):
+        return self.handle.ptr
+
+    @classmethod
+    def synthetic(cls, L: int, nup: int, ndown: int, seed: int, ctx: Context | None = None):
+        h, V = gen_integrals(L, seed)
+        return cls(h, V, nup, ndown, 0.0, ctx)
+
+    @classmethod
+    def from_npz(cls, path: str, ctx: Context | None = None):
+        d = np.load(path)
+        return cls(d["h"], d["V"], int(d["nalpha"]), int(d["nbeta"]), float(d["econst"]), ctx)
+
+    def getL(self):
+        return self.L
+
+    def getnup(self):
+        return self.nup
+
+    def getndown(self):
+        return self.ndown
+
+    def getEconst(self):
+        return self.econst
+
+    def getTmat(self, i, j):
+        return float(self.h[i, j])
+
+    def getVmat(self, i, j, k, l):
+        return float(self.V[i, j, k, l])
+
+
+class SparseMatrix_CRS_Sym:
+    """GPU CRS Hamiltonian (full rows stored; the reference's upper-triangle
+    layout is available through export_upper)."""
+
+    def __init__(self, space: "_Handle", ctx: Context, value_prec: int = CF_F64, zero: float = 1e-14):
+        self.ctx = ctx
+        self.handle = _Handle("cf_crs_destroy", space)
+        _check(lib().cf_crs_build(space.ptr, value_prec, zero, C.byref(self.handle.ptr)))
+        self.value_prec = value_prec
+
+    @property
+    def ptr(self):
+        return self.handle.ptr
+
+    def info(self):
+        n, nf, nu = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().cf_crs_info(self.ptr, C.byref(n), C.byref(nf), C.byref(nu)))
+        return n.value, nf.value, nu.value
+
+    def getn(self):
+        return self.info()[0]
+
+    def datasize(self):
+        """Stored upper-triangle entries, as the reference's data.size()."""
+        return self.info()[2]
+
+    def _export(self, upper: bool, r0: int, r1: int):
+        cnt = C.c_uint64()
+        f = lib().cf_crs_count_upper if upper else lib().cf_crs_count_full
+        _check(f(self.ptr, r0, r1, C.byref(cnt)))
+        row = np.zeros(r1 - r0 + 1, dtype=np.uint64)
+        col = np.zeros(max(cnt.value, 1), dtype=np.uint32)
+        val = np.zeros(max(cnt.value, 1))
+        g = lib().cf_crs_export_upper if upper else lib().cf_crs_export_full
+        _check(g(self.ptr, r0, r1, row.ctypes.data_as(u64p), col.ctypes.data_as(u32p), _d(val)))
+        return row, col[:cnt.value], val[:cnt.value]
+
+    def export_upper(self, r0: int = 0, r1: int | None = None):
+        return self._export(True, r0, self.getn() if r1 is None else r1)
+
+    def export_full(self, r0: int = 0, r1: int | None = None):
+        return self._export(False, r0, self.getn() if r1 is None else r1)
+
+    def mvprod(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib().cf_mvprod(self.ptr, _d(x), _d(y)))
+        return y
+
+    def arpackDiagonalize(self, opts: SolveOpts | None = None, want_vec: bool = True):
+        n = self.getn()
+        if opts is None:
+            opts = SolveOpts()
+            lib().cf_solve_opts_default(C.byref(opts))
+        vec = np.zeros(n) if want_vec else None
+        info = SolveInfo()
+        _check(lib().cf_solve(self.ptr, C.byref(opts), _d(vec) if want_vec else None, C.byref(info)))
+        return info, vec
+
+
+def solve_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    lib().cf_solve_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class CIMethod:
+    """Common part of FCI and DOCI (CIMethod.h:37-132)."""
+
+    kind = CF_FCI
+
+    def __init__(self, ham: Hamiltonian, value_prec: int = CF_F64, build: bool = True, zero: float = 1e-14):
+        self.ham = ham
+        self.ctx = ham.ctx
+        self.space = _Handle("cf_space_destroy", ham.handle)
+        _check(lib().cf_space_create(ham.ptr, self.kind, C.byref(self.space.ptr)))
+        n, da, db = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().cf_space_dims(self.ptr, C.byref(n), C.byref(da), C.byref(db)))
+        self.n, self.da, self.db = n.value, da.value, db.value
+        self._zero = zero
+        self._value_prec = value_prec
+        self._mat = None
+        self._eigval = None
+        self._eigvec = None
+        self._info = None
+        self._one = None
+        self._two = None
+        if build:
+            self.build_ci()
+
+    # CIMethod::build_ci -> build_parallel (CIMethod.cpp:489-561)
+    def build_ci(self):
+        self._mat = None
+        self._mat = SparseMatrix_CRS_Sym(self.space, self.ctx, self._value_prec, self._zero)
+
+    def set_sparsity(self, zero: float):
+        self._zero = zero
+
+    def get_dim(self):
+        return self.n
+
+    def gUpDim(self):
+        return self.da
+
+    def gDownDim(self):
+        return self.db
+
+    def get_mat(self) -> SparseMatrix_CRS_Sym:
+        return self._mat
+
+    # CIMethod::solve (CIMethod.cpp:241-264)
+    def solve(self, precision: int = CF_F64, **kw):
+        if self._mat is None:
+            self.build_ci()
+        opts = solve_opts(precision=precision, **kw)
+        self._info, self._eigvec = self._mat.arpackDiagonalize(opts)
+        self._eigval = self._info.eigval
+        self._one = self._two = None
+        return self._info
+
+    def get_ci_energy(self):
+        return self._eigval + self.ham.econst
+
+    def get_eigvec(self):
+        return self._eigvec
+
+    # CIMethod::construct_density (CIMethod.cpp:161-171)
+    def construct_density(self, trdm: bool = True):
+        L = self.ham.L
+        one = np.zeros(L * (L + 1))
+        two = np.zeros(3 * (L * L) * (L * L + 1) // 2) if trdm else None
+        c = np.ascontiguousarray(self._eigvec)
+        _check(lib().cf_rdm(self.ptr, _d(c), _d(one), _d(two) if trdm else None))
+        self._one, self._two = one, two
+        return one, two
+
+    def density_dense(self, c=None, two: bool = True):
+        L = self.ham.L
+        c = np.ascontiguousarray(self._eigvec if c is None else c, dtype=np.float64)
+        g = np.zeros((2, L, L))
+        G = np.zeros((3, L, L, L, L)) if two else None
+        _check(lib().cf_rdm_dense(self.ptr, _d(c), _d(g), _d(G) if two else None))
+        return g, G
+
+    # CIDens accessors (CIDens.cpp:166-262)
+    def get_one_rdm(self, spin, a, b):
+        L = self.ham.L
+        if a > b:
+            a, b = b, a
+        return float(self._one[spin * (L * (L + 1) // 2) + a * L + b - a * (a + 1) // 2])
+
+    def get_two_rdm(self, spin, i, j, k, l):
+        L = self.ham.L
+        C2 = L * L
+        tsz = C2 * (C2 + 1) // 2
+        idx = lambda r, c: C2 * min(r, c) + max(r, c) - min(r, c) * (min(r, c) + 1) // 2  # noqa: E731
+        if spin == 1:
+            return float(self._two[tsz + idx(i * L + j, k * L + l)])
+        if i == j or k == l:
+            return 0.0
+        sign = 1.0
+        if i > j:
+            i, j, sign = j, i, -sign
+        if k > l:
+            k, l, sign = l, k, -sign
+        P = i * L + j - (i + 1) * (i + 2) // 2
+        Q = k * L + l - (k + 1) * (k + 2) // 2
+        return sign * float(self._two[spin * tsz + idx(P, Q)])
+
+    def get_dens_energy(self):
+        """CIDens::get_dens_energy (CIDens.cpp:898-932) from the dense RDMs."""
+        g, G = self.density_dense()
+        h, V = self.ham.h, self.ham.V
+        e1 = float(np.sum(h * (g[0] + g[1])))
+        e2 = 0.5 * float(np.sum(V * (G[0] + 2 * G[1] + G[2])))
+        return e1 + e2 + self.ham.econst
+
+    @property
+    def ptr(self):
+        return self.space.ptr
+
+
+class FCI(CIMethod):
+    kind = CF_FCI
+
+    def get_name(self):
+        return "FCI"
+
+
+class DOCI(CIMethod):
+    kind = CF_DOCI
+
+    def get_name(self):
+        return "DOCI"
+
+
+class RCILanczos:
+    """dsaupd_-shaped reverse communication (host vectors): call step() until
+    it returns 99; on 1 compute workd[n:2n] = A @ workd[0:n]."""
+
+    def __init__(self, n: int, ncv: int = 42, tol: float = 1e-12, maxrestart: int = 0, seed: int = 20240601):
+        self.n = n
+        self.handle = _Handle("cf_rci_destroy")
+        self.ptr = self.handle.ptr
+        _check(lib().cf_rci_create(n, ncv, tol, maxrestart, seed, C.byref(self.handle.ptr)))
+        self.workd = np.zeros(3 * n)
+        self.ido = C.c_int(0)
+        self.ipntr = (C.c_int * 11)()
+
+    def step(self) -> int:
+        _check(lib().cf_rci_step(self.ptr, C.byref(self.ido), _d(self.workd), self.ipntr))
+        return self.ido.value
+
+    def result(self):
+        ev = C.c_double()
+        vec = np.zeros(self.n)
+        info = C.c_int()
+        _check(lib().cf_rci_result(self.ptr, C.byref(ev), _d(vec), C.byref(info)))
+        return ev.value, vec, info.value
+
+
+def arpack_like(matvec, n: int, ncv: int = 42, tol: float = 1e-12):
+    """The reference's arpackDiagonalize loop (SparseMatrix_CRS.cpp:441-449)
+    over an arbitrary mvprod: returns (eigval, eigvec, info)."""
+    r = RCILanczos(n, ncv, tol)
+    while r.step() != 99:
+        i0 = r.ipntr[0] - 1
+        i1 = r.ipntr[1] - 1
+        r.workd[i1:i1 + n] = matvec(r.workd[i0:i0 + n])
+    return r.result()

@@ -1,0 +1,438 @@
+// Slater-Condon CRS build on the GPU: count pass, exclusive scan, fill pass.
+//
+// Replaces FCI::construct_CI_matrix / DOCI::construct_CI_matrix
+// (CIMethod.cpp:611-660, 749-888) driven by CIMethod::build_parallel
+// (:489-561) and SparseMatrix_CRS::PushToRow/add_from_list
+// (SparseMatrix_CRS.cpp:240-322, 663-696).
+//
+// The reference stores the upper triangle and its SpMV scatters into y[col];
+// the GPU keeps FULL rows so the SpMV is a race-free gather.  Each row is
+// produced by one wavefront directly in ascending column order (no insertion
+// sort), entries with |v| <= zero are dropped exactly as PushToRow drops them,
+// and the surviving entries are compacted with a ballot + mbcnt prefix.
+//
+// Element values follow the reference's formulas AND summation order, so the
+// upper part of every row matches the reference bit for bit:
+//   diagonal      (D1a + D1b) + sum_{k in a} sum_{l in b} V(k,l,k,l)   (:398-407)
+//   single        [h_pq + sum_z (V_pzqz - V_pzzq)] + sum_{d in other} V_pdqd, then sign (:409-427)
+//   ab double     V(a_up1, b_down1, a_up2, b_down2) with signs of both strings (:444-457)
+//   same-spin dbl precomputed per string pair (:429-442)
+// Lower-triangle entries are evaluated in the orientation the reference uses
+// for the mirrored upper entry, so the stored matrix is exactly symmetric.
+//
+// FCI row layout: for row (ia, ib) the columns are ja*db + jb with ja running
+// over the sorted alpha list of ia; an alpha "self" entry expands to the full
+// sorted beta list of ib, an alpha single to the beta self+singles list, an
+// alpha double to the single column ja*db + ib.  segoff[ia] holds the prefix
+// offsets of these segments; a lane finds its segment by binary search.
+#include "kernels.h"
+
+namespace cf {
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__global__ void k_fci_segoff(uint64_t da, int Ea, const Exc* __restrict__ full_a, int Eb, int Sb,
+                             uint32_t* __restrict__ segoff) {
+    uint64_t ia = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ia >= da) return;
+    const Exc* lst = full_a + ia * (uint64_t)Ea;
+    uint32_t* so = segoff + ia * (uint64_t)(Ea + 1);
+    uint32_t acc = 0;
+    for (int e = 0; e < Ea; ++e) {
+        so[e] = acc;
+        int t = lst[e].type;
+        acc += t == 0 ? (uint32_t)Eb : (t == 1 ? (uint32_t)Sb : 1u);
+    }
+    so[Ea] = acc;
+}
+
+void launch_fci_segoff(uint64_t da, int Ea, const Exc* full_a, int Eb, int Sb, uint32_t* segoff,
+                       hipStream_t stream) {
+    if (!da) return;
+    k_fci_segoff<<<(unsigned)((da + 255) / 256), 256, 0, stream>>>(da, Ea, full_a, Eb, Sb, segoff);
+    CF_HIP(hipGetLastError());
+}
+
+// jmix of CIMethod::diagonal (CIMethod.cpp:343-351): sum over alpha k (outer),
+// beta l (inner), accumulated from zero.
+__device__ __forceinline__ double jmix(str_t a, str_t b, const HamDev& H) {
+    double s = 0.0;
+    for (str_t t = a; t; t &= t - 1) {
+        int k = __builtin_ctzll(t);
+        for (str_t u = b; u; u &= u - 1) s += H.Jm[k * H.L + __builtin_ctzll(u)];
+    }
+    return s;
+}
+
+__global__ void k_fci_diag(uint64_t da, uint64_t db, const str_t* __restrict__ sa, const str_t* __restrict__ sb,
+                           const double* __restrict__ d1a, const double* __restrict__ d1b, HamDev H,
+                           double* __restrict__ diag) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= da * db) return;
+    uint64_t ia = r / db, ib = r - ia * db;
+    double s = d1a[ia];
+    s += d1b[ib];
+    s += jmix(sa[ia], sb[ib], H);
+    diag[r] = s;
+}
+
+void launch_fci_diag(uint64_t da, uint64_t db, const str_t* sa, const str_t* sb, const double* d1a,
+                     const double* d1b, HamDev H, double* diag, hipStream_t stream) {
+    uint64_t n = da * db;
+    if (!n) return;
+    k_fci_diag<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(da, db, sa, sb, d1a, d1b, H, diag);
+    CF_HIP(hipGetLastError());
+}
+
+__global__ void k_doci_diag(uint64_t d, const str_t* __restrict__ s, const double* __restrict__ d1, HamDev H,
+                            double* __restrict__ diag) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d) return;
+    double v = d1[r];
+    v += d1[r];
+    v += jmix(s[r], s[r], H);
+    diag[r] = v;
+}
+
+void launch_doci_diag(uint64_t d, const str_t* s, const double* d1, HamDev H, double* diag, hipStream_t stream) {
+    if (!d) return;
+    k_doci_diag<<<(unsigned)((d + 255) / 256), 256, 0, stream>>>(d, s, d1, H, diag);
+    CF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// FCI rows: one wavefront per row.
+// ---------------------------------------------------------------------------
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_fci_rows(FciRowsArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t r = a.row0 + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= a.row0 + a.nrows) return;
+    const uint64_t ia = r / a.db, ib = r - ia * a.db;
+    const str_t Sa = a.sa[ia], Sb = a.sb[ib];
+    const uint32_t* so = a.segoff + ia * (uint64_t)(a.Ea + 1);
+    const uint32_t R = so[a.Ea];
+    const Exc* la = a.fa + ia * (uint64_t)a.Ea;
+    const Exc* lb = a.fb + ib * (uint64_t)a.Eb;
+    const Exc* sbl = a.subb + ib * (uint64_t)a.Sb;
+    const int L = a.H.L;
+    int64_t base = 0;
+    if (FILL) base = a.rowptr[r];
+    uint32_t cnt = 0;
+    for (uint32_t t0 = 0; t0 < R; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        bool keep = false;
+        uint32_t col = 0;
+        double val = 0.0;
+        if (t < R) {
+            int lo = 0, hi = a.Ea;  // largest e with so[e] <= t
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (so[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            const Exc ea = la[lo];
+            const uint32_t u = t - so[lo];
+            if (ea.type == 0) {
+                const Exc eb = lb[u];
+                col = (uint32_t)(ia * a.db + eb.J);
+                if (eb.type == 0) {
+                    val = a.diag[r];
+                } else if (eb.type == 1) {
+                    int p = eb.f < eb.t ? eb.f : eb.t, q = eb.f < eb.t ? eb.t : eb.f;
+                    const double* w = a.H.W + ((size_t)p * L + q) * L;
+                    val = eb.x;
+                    for (str_t s = Sa; s; s &= s - 1) val += w[__builtin_ctzll(s)];
+                    if (eb.sign < 0) val *= -1;
+                } else {
+                    val = eb.x;
+                }
+            } else if (ea.type == 1) {
+                const Exc eb = sbl[u];
+                col = (uint32_t)((uint64_t)ea.J * a.db + eb.J);
+                if (eb.type == 0) {
+                    int p = ea.f < ea.t ? ea.f : ea.t, q = ea.f < ea.t ? ea.t : ea.f;
+                    const double* w = a.H.W + ((size_t)p * L + q) * L;
+                    val = ea.x;
+                    for (str_t s = Sb; s; s &= s - 1) val += w[__builtin_ctzll(s)];
+                    if (ea.sign < 0) val *= -1;
+                } else {
+                    size_t i0, i1, i2, i3;
+                    if (ea.J > ia) { i0 = ea.f; i1 = eb.f; i2 = ea.t; i3 = eb.t; }
+                    else { i0 = ea.t; i1 = eb.t; i2 = ea.f; i3 = eb.f; }
+                    val = a.H.V[((i0 * L + i1) * L + i2) * L + i3];
+                    if ((ea.sign < 0) != (eb.sign < 0)) val *= -1;
+                }
+            } else {
+                col = (uint32_t)((uint64_t)ea.J * a.db + ib);
+                val = ea.x;
+            }
+            keep = fabs(val) > a.zero;
+        }
+        const uint64_t mask = __ballot(keep);
+        if (FILL && keep) {
+            const int64_t pos = base + cnt + lanes_below(mask);
+            a.col[pos] = col;
+            a.val[pos] = val;
+            if (a.val32) a.val32[pos] = (float)val;
+        }
+        cnt += (uint32_t)__popcll(mask);
+    }
+    if (!FILL && lane == 0) a.rowcnt[r - a.row0] = cnt;
+}
+
+void launch_fci_rows(const FciRowsArgs& a, bool fill, hipStream_t stream) {
+    if (!a.nrows) return;
+    unsigned grid = (unsigned)((a.nrows + 3) / 4);
+    if (fill) k_fci_rows<true><<<grid, 256, 0, stream>>>(a);
+    else k_fci_rows<false><<<grid, 256, 0, stream>>>(a);
+    CF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// DOCI rows (CIMethod.cpp:611-660): pair hops j -> k of one string.  Columns in
+// ascending order without sorting: hops that lower the string come first
+// (occupied j descending, then virtual k < j ascending), then the diagonal,
+// then hops that raise it (virtual k ascending, then occupied j < k
+// descending).  Element: V(a,a,b,b) with a the orbital occupied in the
+// smaller of the two strings, i.e. two_diff_orbitals(up1,up2,up1,up2) of the
+// reference's upper triangle; the phase is always +1.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int select_bit(str_t v, int m) {  // position of the m-th (0-based) set bit
+    for (int i = 0; i < m; ++i) v &= v - 1;
+    return __builtin_ctzll(v);
+}
+
+__device__ __forceinline__ uint32_t rank_str(str_t s, const uint64_t* binom) {
+    uint32_t r = 0;
+    int m = 1;
+    while (s) {
+        r += (uint32_t)binom[__builtin_ctzll(s) * kBinomCols + m];
+        ++m;
+        s &= s - 1;
+    }
+    return r;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_doci_rows(DociRowsArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t r = a.row0 + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= a.row0 + a.nrows) return;
+    const int L = a.L;
+    const str_t full = (L == 64) ? ~0ull : (((str_t)1 << L) - 1);
+    const str_t s = a.s[r];
+    const str_t vir = ~s & full;
+    uint32_t ndown = 0;
+    for (str_t t = s; t; t &= t - 1) {
+        int j = __builtin_ctzll(t);
+        ndown += (uint32_t)__popcll(vir & (((str_t)1 << j) - 1));
+    }
+    const uint32_t R = 1u + (uint32_t)a.np * (uint32_t)(L - a.np);
+    int64_t base = 0;
+    if (FILL) base = a.rowptr[r];
+    uint32_t cnt = 0;
+    for (uint32_t t0 = 0; t0 < R; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        bool keep = false;
+        uint32_t col = 0;
+        double val = 0.0;
+        if (t < R) {
+            if (t == ndown) {
+                col = (uint32_t)r;
+                val = a.diag[r];
+            } else {
+                int j = 0, k = 0;
+                if (t < ndown) {
+                    uint32_t rem = t;
+                    for (int jj = L - 1; jj >= 0; --jj) {
+                        if (!((s >> jj) & 1)) continue;
+                        str_t below = vir & (((str_t)1 << jj) - 1);
+                        uint32_t c = (uint32_t)__popcll(below);
+                        if (rem < c) { j = jj; k = select_bit(below, (int)rem); break; }
+                        rem -= c;
+                    }
+                } else {
+                    uint32_t rem = t - ndown - 1;
+                    for (int kk = 0; kk < L; ++kk) {
+                        if ((s >> kk) & 1) continue;
+                        str_t below = s & (((str_t)1 << kk) - 1);
+                        uint32_t c = (uint32_t)__popcll(below);
+                        if (rem < c) { k = kk; j = select_bit(below, (int)(c - 1 - rem)); break; }
+                        rem -= c;
+                    }
+                }
+                const str_t tgt = s ^ ((str_t)1 << j) ^ ((str_t)1 << k);
+                col = rank_str(tgt, a.binom);
+                const size_t lo = j < k ? j : k, hi = j < k ? k : j;
+                val = a.H.V[((lo * L + lo) * L + hi) * L + hi];
+            }
+            keep = fabs(val) > a.zero;
+        }
+        const uint64_t mask = __ballot(keep);
+        if (FILL && keep) {
+            const int64_t pos = base + cnt + lanes_below(mask);
+            a.col[pos] = col;
+            a.val[pos] = val;
+            if (a.val32) a.val32[pos] = (float)val;
+        }
+        cnt += (uint32_t)__popcll(mask);
+    }
+    if (!FILL && lane == 0) a.rowcnt[r - a.row0] = cnt;
+}
+
+void launch_doci_rows(const DociRowsArgs& a, bool fill, hipStream_t stream) {
+    if (!a.nrows) return;
+    unsigned grid = (unsigned)((a.nrows + 3) / 4);
+    if (fill) k_doci_rows<true><<<grid, 256, 0, stream>>>(a);
+    else k_doci_rows<false><<<grid, 256, 0, stream>>>(a);
+    CF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan u32 -> i64 (three phases: per-block scan, scan of block
+// sums, add).  4096 items per 1024-thread block.
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+    const int lane = threadIdx.x & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        int64_t o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+__device__ int64_t block_excl_scan(int64_t v, int64_t* total) {
+    __shared__ int64_t wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int64_t inc = wave_incl_scan(v);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        int64_t x = lane < kScanThreads / 64 ? wsum[lane] : 0;
+        int64_t xi = wave_incl_scan(x);
+        if (lane < kScanThreads / 64) wsum[lane] = xi - x;
+        if (lane == kScanThreads / 64 - 1) *total = xi;
+    }
+    __syncthreads();
+    int64_t r = inc - v + wsum[wid];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const uint32_t* __restrict__ cnt, int64_t* __restrict__ out,
+                                                          uint64_t n, int64_t* __restrict__ bsum) {
+    __shared__ int64_t tot;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    int64_t v[kScanItems];
+    int64_t s = 0;
+    for (int i = 0; i < kScanItems; ++i) {
+        v[i] = (base + i < n) ? (int64_t)cnt[base + i] : 0;
+        s += v[i];
+    }
+    int64_t pre = block_excl_scan(s, &tot);
+    for (int i = 0; i < kScanItems; ++i) {
+        if (base + i < n) out[base + i] = pre;
+        pre += v[i];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_bsum(int64_t* __restrict__ bsum, uint64_t nb,
+                                                         int64_t* __restrict__ grand) {
+    __shared__ int64_t tot;
+    int64_t carry = 0;
+    for (uint64_t b0 = 0; b0 < nb; b0 += kScanThreads) {
+        uint64_t i = b0 + threadIdx.x;
+        int64_t v = i < nb ? bsum[i] : 0;
+        int64_t pre = block_excl_scan(v, &tot);
+        if (i < nb) bsum[i] = pre + carry;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *grand = carry;
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out, uint64_t n, const int64_t* __restrict__ bsum) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += bsum[i / kScanTile];
+}
+
+void exclusive_scan_u32_to_i64(const uint32_t* cnt, int64_t* out, uint64_t n, hipStream_t stream) {
+    uint64_t nb = (n + kScanTile - 1) / kScanTile;
+    if (nb == 0) nb = 1;
+    DBuf<int64_t> bsum(nb);
+    k_scan_tiles<<<(unsigned)nb, kScanThreads, 0, stream>>>(cnt, out, n, bsum.p);
+    CF_HIP(hipGetLastError());
+    k_scan_bsum<<<1, kScanThreads, 0, stream>>>(bsum.p, nb, out + n);
+    CF_HIP(hipGetLastError());
+    if (n) {
+        k_scan_add<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(out, n, bsum.p);
+        CF_HIP(hipGetLastError());
+    }
+    CF_HIP(hipStreamSynchronize(stream));  // bsum is freed on return
+}
+
+// ---------------------------------------------------------------------------
+// Export helpers.
+// ---------------------------------------------------------------------------
+__global__ void k_count_upper(const int64_t* __restrict__ rowptr, const uint32_t* __restrict__ col, uint64_t r0,
+                              uint64_t r1, uint64_t* __restrict__ cnt) {
+    uint64_t r = r0 + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= r1) return;
+    int64_t lo = rowptr[r], hi = rowptr[r + 1];
+    const int64_t end = hi;
+    while (lo < hi) {  // first col >= r
+        int64_t mid = (lo + hi) >> 1;
+        if (col[mid] < r) lo = mid + 1;
+        else hi = mid;
+    }
+    cnt[r - r0] = (uint64_t)(end - lo);
+}
+
+void launch_count_upper(const int64_t* rowptr, const uint32_t* col, uint64_t r0, uint64_t r1, uint64_t* cnt,
+                        hipStream_t stream) {
+    if (r1 <= r0) return;
+    k_count_upper<<<(unsigned)((r1 - r0 + 255) / 256), 256, 0, stream>>>(rowptr, col, r0, r1, cnt);
+    CF_HIP(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256) k_export_rows(const int64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                                     const double* __restrict__ val, uint64_t r0, uint64_t r1,
+                                                     bool upper_only, const uint64_t* __restrict__ off,
+                                                     uint32_t* __restrict__ ocol, double* __restrict__ oval) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t r = r0 + (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= r1) return;
+    int64_t b = rowptr[r], e = rowptr[r + 1];
+    if (upper_only) {
+        int64_t lo = b, hi = e;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (col[mid] < r) lo = mid + 1;
+            else hi = mid;
+        }
+        b = lo;
+    }
+    const uint64_t o = off[r - r0];
+    for (int64_t k = b + lane; k < e; k += 64) {
+        ocol[o + (uint64_t)(k - b)] = col[k];
+        oval[o + (uint64_t)(k - b)] = val[k];
+    }
+}
+
+void launch_export_rows(const int64_t* rowptr, const uint32_t* col, const double* val, uint64_t r0, uint64_t r1,
+                        bool upper_only, const uint64_t* out_off, uint32_t* ocol, double* oval, hipStream_t stream) {
+    if (r1 <= r0) return;
+    k_export_rows<<<(unsigned)((r1 - r0 + 3) / 4), 256, 0, stream>>>(rowptr, col, val, r0, r1, upper_only, out_off,
+                                                                       ocol, oval);
+    CF_HIP(hipGetLastError());
+}
+
+}  // namespace cf

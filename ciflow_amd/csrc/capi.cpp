@@ -1,0 +1,951 @@
+// C ABI of the ciflow_amd library (declarations and reference citations in
+// include/ciflow_amd.h).
+#include "../../include/ciflow_amd.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.h"
+#include "lanczos.h"
+#include "rdm.h"
+
+namespace cf {
+void gen_integrals(int L, uint64_t seed, double* h, double* V);
+}
+
+using namespace cf;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace
+
+#define API_BEGIN try {
+#define API_END                                              \
+    }                                                        \
+    catch (const cf::Error& e) {                             \
+        return fail(e.code, e.what());                       \
+    }                                                        \
+    catch (const std::bad_alloc&) {                          \
+        return fail(CF_ERR_NOMEM, "host allocation failed"); \
+    }                                                        \
+    catch (const std::exception& e) {                        \
+        return fail(CF_ERR_HIP, e.what());                   \
+    }                                                        \
+    return CF_OK;
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct TimerFamily {
+    int64_t launches = 0;
+    double ms = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct cf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DBuf<uint64_t> binom;  // C(k, m): (kMaxL+1) x kBinomCols
+    bool timing = false;
+    std::map<std::string, TimerFamily> timers;
+    std::vector<hipEvent_t> pool;
+    DBuf<double> hx, hy;  // staging for host-pointer products
+
+    void use() { CF_HIP(hipSetDevice(device)); }
+    hipEvent_t event() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        CF_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void flush(TimerFamily& f) {
+        for (auto& pr : f.pending) {
+            CF_HIP(hipEventSynchronize(pr.second));
+            float ms = 0.f;
+            CF_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+            f.ms += ms;
+            f.launches++;
+            pool.push_back(pr.first);
+            pool.push_back(pr.second);
+        }
+        f.pending.clear();
+    }
+    ~cf_ctx() {
+        for (auto& kv : timers)
+            for (auto& pr : kv.second.pending) {
+                (void)hipEventDestroy(pr.first);
+                (void)hipEventDestroy(pr.second);
+            }
+        for (auto e : pool) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+// Records HIP events around one kernel launch on the context stream.
+struct KTimer {
+    cf_ctx* c;
+    const char* fam;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(cf_ctx* ctx, const char* family) : c(ctx), fam(family) {
+        if (c->timing) {
+            a = c->event();
+            b = c->event();
+            CF_HIP(hipEventRecord(a, c->stream));
+        }
+    }
+    ~KTimer() {
+        if (a) {
+            (void)hipEventRecord(b, c->stream);
+            c->timers[fam].pending.emplace_back(a, b);
+        }
+    }
+};
+}  // namespace
+
+struct cf_ham {
+    cf_ctx* ctx = nullptr;
+    int L = 0, nup = 0, ndown = 0;
+    double econst = 0.0;
+    std::vector<double> h, V;
+    DBuf<double> dh, dV, dW, dJm;
+    HamDev dev() const { return HamDev{L, dh.p, dV.p, dW.p, dJm.p}; }
+};
+
+struct StrTab {
+    int L = 0, nel = 0;
+    uint64_t d = 0;
+    DBuf<str_t> s;
+    DBuf<double> d1;
+    int E = 0, S = 0;
+    DBuf<Exc> full, sub;
+};
+
+struct cf_space {
+    cf_ham* ham = nullptr;
+    int kind = CF_FCI;
+    uint64_t da = 0, db = 0, n = 0;
+    std::unique_ptr<StrTab> ta, tb_own;
+    StrTab* tb = nullptr;
+    DBuf<uint32_t> segoff;
+    DBuf<double> diag;
+    DBuf<double> vec;  // resident eigenvector after cf_solve
+    bool vec_valid = false;
+};
+
+struct cf_crs {
+    cf_space* sp = nullptr;
+    uint64_t n = 0, nnz = 0;
+    int64_t nnz_upper = -1;
+    DBuf<int64_t> rowptr;
+    DBuf<uint32_t> col;
+    DBuf<double> val;
+    DBuf<float> val32;
+    int lpr = 64;
+};
+
+namespace {
+StrTab* make_strtab(cf_ctx* ctx, const cf_ham* H, int nel, bool lists) {
+    std::unique_ptr<StrTab> t(new StrTab());
+    t->L = H->L;
+    t->nel = nel;
+    t->d = binom_host(H->L, nel);
+    CF_REQUIRE(t->d > 0 && t->d < (1ull << 32), "string space dimension must be in [1, 2^32)");
+    t->s.alloc(t->d);
+    t->d1.alloc(t->d);
+    launch_unrank(H->L, nel, 0, t->d, ctx->binom.p, t->s.p, ctx->stream);
+    launch_diag1(nel, t->s.p, t->d, H->dev(), t->d1.p, ctx->stream);
+    if (lists) {
+        const int nv = H->L - nel;
+        const uint64_t ns = (uint64_t)nel * nv;
+        const uint64_t nd = binom_host(nel, 2) * binom_host(nv, 2);
+        CF_REQUIRE(1 + ns + nd <= 2048, "excitation list longer than 2048 entries");
+        t->E = (int)(1 + ns + nd);
+        t->S = (int)(1 + ns);
+        t->full.alloc(t->d * t->E);
+        t->sub.alloc(t->d * t->S);
+        launch_exc_lists(H->L, nel, t->s.p, t->d, ctx->binom.p, H->dev(), t->d1.p, t->E, t->S, t->full.p, t->sub.p,
+                         ctx->stream);
+    }
+    return t.release();
+}
+
+void ensure_staging(cf_ctx* c, uint64_t n) {
+    if (c->hx.n < n) c->hx.alloc(n);
+    if (c->hy.n < n) c->hy.alloc(n);
+}
+
+void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
+    cf_ctx* c = m->sp->ham->ctx;
+    if (prec == CF_F32) {
+        CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
+        KTimer t(c, "spmv_f32");
+        launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->lpr, c->stream);
+    } else {
+        KTimer t(c, "spmv_f64");
+        launch_spmv_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->lpr, c->stream);
+    }
+}
+
+// Average row length decides how many lanes share a row.
+int pick_lanes_per_row(uint64_t n, uint64_t nnz) {
+    double avg = n ? (double)nnz / (double)n : 0.0;
+    if (avg >= 256) return 64;
+    if (avg >= 96) return 32;
+    if (avg >= 32) return 16;
+    return 8;
+}
+
+// ---------------------------------------------------------------------------
+// Krylov backends
+// ---------------------------------------------------------------------------
+struct DeviceBackend : KrylovBackend {
+    cf_ctx* c;
+    uint64_t n;
+    int M;
+    DBuf<double> Vb, w, partial, cvec, c2, Yd, scal;
+    int nb;
+    std::function<void(const double*, double*)> op;
+    std::vector<double> hbuf;
+
+    DeviceBackend(cf_ctx* ctx, uint64_t n_, int ncv) : c(ctx), n(n_) {
+        M = (uint64_t)ncv > n ? (int)n : ncv;
+        Vb.alloc((size_t)(M + 1) * n);
+        w.alloc(n);
+        nb = multi_dot_blocks(n);
+        partial.alloc((size_t)64 * nb);
+        cvec.alloc(64);
+        c2.alloc(64);
+        Yd.alloc(64 * 64);
+        scal.alloc(4);
+        hbuf.resize(8);
+    }
+    double* v(int i) { return Vb.p + (size_t)i * n; }
+    void dots(int k, const double* base, const double* x, double* out) {
+        launch_multi_dot(n, k, base, x, partial.p, nb, c->stream);
+        launch_reduce_partials(k, nb, partial.p, out, c->stream);
+    }
+    double fetch(const double* d) {
+        double h;
+        CF_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipStreamSynchronize(c->stream));
+        return h;
+    }
+    void start_random(uint64_t seed) override {
+        launch_fill_start(n, seed, v(0), c->stream);
+        normalize0();
+    }
+    void start_keep() override { normalize0(); }
+    void apply(int j) override { op(v(j), w.p); }
+    void orth(int j, double* alpha, double* beta) override {
+        dots(j + 1, Vb.p, w.p, cvec.p);
+        launch_multi_axpy(n, j + 1, Vb.p, cvec.p, w.p, c->stream);
+        dots(j + 1, Vb.p, w.p, c2.p);
+        launch_multi_axpy(n, j + 1, Vb.p, c2.p, w.p, c->stream);
+        dots(1, w.p, w.p, scal.p);
+        CF_HIP(hipMemcpyAsync(&hbuf[0], cvec.p + j, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipMemcpyAsync(&hbuf[1], c2.p + j, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipMemcpyAsync(&hbuf[2], scal.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipStreamSynchronize(c->stream));
+        *alpha = hbuf[0] + hbuf[1];
+        *beta = std::sqrt(std::max(0.0, hbuf[2]));
+    }
+    void set_next(int j1, double beta) override { launch_scale_copy(n, w.p, 1.0 / beta, v(j1), c->stream); }
+    void rotate(int m, int k, const double* Y) override {
+        CF_HIP(hipMemcpyAsync(Yd.p, Y, sizeof(double) * m * m, hipMemcpyHostToDevice, c->stream));
+        launch_rotate_basis(n, m, k, Vb.p, Yd.p, c->stream);
+        CF_HIP(hipStreamSynchronize(c->stream));  // Y is a host temporary
+    }
+    void move(int from, int to) override { launch_copy(n, v(from), v(to), c->stream); }
+    void normalize0() override {
+        dots(1, v(0), v(0), scal.p);
+        double s = fetch(scal.p);
+        launch_scale_copy(n, v(0), 1.0 / std::sqrt(s), v(0), c->stream);
+    }
+    void rayleigh(double* theta, double* resid) override {
+        dots(1, v(0), w.p, scal.p);
+        double th = fetch(scal.p);
+        CF_HIP(hipMemcpyAsync(cvec.p, &th, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        launch_multi_axpy(n, 1, v(0), cvec.p, w.p, c->stream);  // w -= theta v0
+        dots(1, w.p, w.p, scal.p);
+        double r2 = fetch(scal.p);
+        *theta = th;
+        *resid = std::sqrt(std::max(0.0, r2));
+    }
+};
+
+struct HostBackend : KrylovBackend {
+    uint64_t n;
+    int M;
+    std::vector<double> Vb, w;
+    HostBackend(uint64_t n_, int ncv) : n(n_) {
+        M = (uint64_t)ncv > n ? (int)n : ncv;
+        Vb.assign((size_t)(M + 1) * n, 0.0);
+        w.assign(n, 0.0);
+    }
+    double* v(int i) { return Vb.data() + (size_t)i * n; }
+    double dot(const double* a, const double* b) {
+        double s = 0.0;
+        for (uint64_t q = 0; q < n; ++q) s += a[q] * b[q];
+        return s;
+    }
+    void start_random(uint64_t seed) override {
+        double* v0 = v(0);
+        for (uint64_t q = 0; q < n; ++q) {
+            uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (q + 1);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+            z ^= z >> 31;
+            v0[q] = 1.0 + 0.5 * ((double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+        }
+        normalize0();
+    }
+    void start_keep() override { normalize0(); }
+    void apply(int) override { throw Error(ERR_STATE, "host backend is driven by reverse communication"); }
+    void orth(int j, double* alpha, double* beta) override {
+        double a = 0.0;
+        std::vector<double> c(j + 1);
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int i = 0; i <= j; ++i) c[i] = dot(v(i), w.data());
+            a += c[j];
+            for (int i = 0; i <= j; ++i) {
+                const double* vi = v(i);
+                for (uint64_t q = 0; q < n; ++q) w[q] -= c[i] * vi[q];
+            }
+        }
+        *alpha = a;
+        *beta = std::sqrt(dot(w.data(), w.data()));
+    }
+    void set_next(int j1, double beta) override {
+        double* d = v(j1);
+        for (uint64_t q = 0; q < n; ++q) d[q] = w[q] / beta;
+    }
+    void rotate(int m, int k, const double* Y) override {
+        std::vector<double> row(m);
+        for (uint64_t q = 0; q < n; ++q) {
+            for (int i = 0; i < m; ++i) row[i] = Vb[(size_t)i * n + q];
+            for (int c = 0; c < k; ++c) {
+                double s = 0.0;
+                for (int i = 0; i < m; ++i) s += row[i] * Y[(size_t)i * m + c];
+                Vb[(size_t)c * n + q] = s;
+            }
+        }
+    }
+    void move(int from, int to) override { std::memcpy(v(to), v(from), n * sizeof(double)); }
+    void normalize0() override {
+        double s = 1.0 / std::sqrt(dot(v(0), v(0)));
+        double* v0 = v(0);
+        for (uint64_t q = 0; q < n; ++q) v0[q] *= s;
+    }
+    void rayleigh(double* theta, double* resid) override {
+        double th = dot(v(0), w.data());
+        double r2 = 0.0;
+        const double* v0 = v(0);
+        for (uint64_t q = 0; q < n; ++q) {
+            double r = w[q] - th * v0[q];
+            r2 += r * r;
+        }
+        *theta = th;
+        *resid = std::sqrt(r2);
+    }
+};
+
+void sign_fix(std::vector<double>& x) {
+    size_t im = 0;
+    for (size_t q = 1; q < x.size(); ++q)
+        if (std::fabs(x[q]) > std::fabs(x[im])) im = q;
+    if (!x.empty() && x[im] < 0)
+        for (auto& v : x) v = -v;
+}
+}  // namespace
+
+struct cf_rci {
+    std::unique_ptr<HostBackend> b;
+    std::unique_ptr<ThickRestartLanczos> sm;
+    uint64_t n = 0;
+    bool started = false, done = false;
+};
+
+// ---------------------------------------------------------------------------
+// API
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int cf_version(void) { return 100; }
+
+const char* cf_last_error(void) { return g_err.c_str(); }
+
+int cf_ctx_create(int device, cf_ctx** out) {
+    API_BEGIN
+    CF_REQUIRE(out != nullptr, "out is NULL");
+    int ndev = 0;
+    CF_HIP(hipGetDeviceCount(&ndev));
+    CF_REQUIRE(device >= 0 && device < ndev, "device index out of range");
+    std::unique_ptr<cf_ctx> c(new cf_ctx());
+    c->device = device;
+    c->use();
+    CF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    std::vector<uint64_t> hb((kMaxL + 1) * kBinomCols, 0);
+    for (int k = 0; k <= kMaxL; ++k)
+        for (int m = 0; m < kBinomCols; ++m) hb[k * kBinomCols + m] = binom_host(k, m);
+    c->binom.alloc(hb.size());
+    CF_HIP(hipMemcpy(c->binom.p, hb.data(), hb.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    *out = c.release();
+    API_END
+}
+
+int cf_ctx_destroy(cf_ctx* ctx) {
+    API_BEGIN
+    if (ctx) {
+        ctx->use();
+        (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    }
+    API_END
+}
+
+int cf_ctx_synchronize(cf_ctx* ctx) {
+    API_BEGIN
+    CF_REQUIRE(ctx, "ctx is NULL");
+    ctx->use();
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int cf_ctx_enable_timing(cf_ctx* ctx, int enable) {
+    API_BEGIN
+    CF_REQUIRE(ctx, "ctx is NULL");
+    ctx->timing = enable != 0;
+    API_END
+}
+
+int cf_ctx_kernel_time(cf_ctx* ctx, const char* family, int64_t* launches, double* total_ms) {
+    API_BEGIN
+    CF_REQUIRE(ctx && family, "NULL argument");
+    ctx->use();
+    auto it = ctx->timers.find(family);
+    if (it == ctx->timers.end()) {
+        if (launches) *launches = 0;
+        if (total_ms) *total_ms = 0.0;
+    } else {
+        ctx->flush(it->second);
+        if (launches) *launches = it->second.launches;
+        if (total_ms) *total_ms = it->second.ms;
+    }
+    API_END
+}
+
+int cf_ctx_reset_timing(cf_ctx* ctx) {
+    API_BEGIN
+    CF_REQUIRE(ctx, "ctx is NULL");
+    ctx->use();
+    for (auto& kv : ctx->timers) {
+        ctx->flush(kv.second);
+        kv.second.launches = 0;
+        kv.second.ms = 0.0;
+    }
+    API_END
+}
+
+int cf_gen_integrals(int L, uint64_t seed, double* h, double* V) {
+    API_BEGIN
+    CF_REQUIRE(L >= 1 && L <= kMaxL && h && V, "bad arguments");
+    gen_integrals(L, seed, h, V);
+    API_END
+}
+
+int cf_ham_create(cf_ctx* ctx, int L, int nup, int ndown, const double* h, const double* V, double econst,
+                  cf_ham** out) {
+    API_BEGIN
+    CF_REQUIRE(ctx && h && V && out, "NULL argument");
+    CF_REQUIRE(L >= 1 && L <= kMaxL, "L must be in [1, 64]");
+    CF_REQUIRE(nup >= 0 && nup <= L && ndown >= 0 && ndown <= L, "electron counts out of range");
+    CF_REQUIRE(nup < kBinomCols && ndown < kBinomCols, "at most 32 electrons per spin");
+    ctx->use();
+    std::unique_ptr<cf_ham> H(new cf_ham());
+    H->ctx = ctx;
+    H->L = L;
+    H->nup = nup;
+    H->ndown = ndown;
+    H->econst = econst;
+    const size_t L2 = (size_t)L * L, L3 = L2 * L, L4 = L3 * L;
+    H->h.assign(h, h + L2);
+    H->V.assign(V, V + L4);
+    std::vector<double> W(L3), Jm(L2);
+    for (int p = 0; p < L; ++p)
+        for (int q = 0; q < L; ++q)
+            for (int d = 0; d < L; ++d) W[((size_t)p * L + q) * L + d] = V[(((size_t)p * L + d) * L + q) * L + d];
+    for (int k = 0; k < L; ++k)
+        for (int l = 0; l < L; ++l) Jm[(size_t)k * L + l] = V[(((size_t)k * L + l) * L + k) * L + l];
+    H->dh.alloc(L2);
+    H->dV.alloc(L4);
+    H->dW.alloc(L3);
+    H->dJm.alloc(L2);
+    CF_HIP(hipMemcpy(H->dh.p, h, L2 * sizeof(double), hipMemcpyHostToDevice));
+    CF_HIP(hipMemcpy(H->dV.p, V, L4 * sizeof(double), hipMemcpyHostToDevice));
+    CF_HIP(hipMemcpy(H->dW.p, W.data(), L3 * sizeof(double), hipMemcpyHostToDevice));
+    CF_HIP(hipMemcpy(H->dJm.p, Jm.data(), L2 * sizeof(double), hipMemcpyHostToDevice));
+    *out = H.release();
+    API_END
+}
+
+int cf_ham_destroy(cf_ham* ham) {
+    API_BEGIN
+    if (ham) {
+        ham->ctx->use();
+        delete ham;
+    }
+    API_END
+}
+
+int cf_space_create(cf_ham* ham, int kind, cf_space** out) {
+    API_BEGIN
+    CF_REQUIRE(ham && out, "NULL argument");
+    CF_REQUIRE(kind == CF_FCI || kind == CF_DOCI, "kind must be CF_FCI or CF_DOCI");
+    cf_ctx* c = ham->ctx;
+    c->use();
+    std::unique_ptr<cf_space> sp(new cf_space());
+    sp->ham = ham;
+    sp->kind = kind;
+    if (kind == CF_DOCI) {
+        CF_REQUIRE(ham->nup == ham->ndown, "DOCI needs nup == ndown (pairs)");
+        sp->ta.reset(make_strtab(c, ham, ham->nup, false));
+        sp->tb = sp->ta.get();
+        sp->da = sp->db = sp->ta->d;
+        sp->n = sp->da;
+        sp->diag.alloc(sp->n);
+        launch_doci_diag(sp->n, sp->ta->s.p, sp->ta->d1.p, ham->dev(), sp->diag.p, c->stream);
+    } else {
+        sp->ta.reset(make_strtab(c, ham, ham->nup, true));
+        if (ham->ndown == ham->nup) {
+            sp->tb = sp->ta.get();
+        } else {
+            sp->tb_own.reset(make_strtab(c, ham, ham->ndown, true));
+            sp->tb = sp->tb_own.get();
+        }
+        sp->da = sp->ta->d;
+        sp->db = sp->tb->d;
+        CF_REQUIRE(sp->da * sp->db < (1ull << 32), "FCI dimension must be < 2^32");
+        sp->n = sp->da * sp->db;
+        sp->segoff.alloc(sp->da * (uint64_t)(sp->ta->E + 1));
+        launch_fci_segoff(sp->da, sp->ta->E, sp->ta->full.p, sp->tb->E, sp->tb->S, sp->segoff.p, c->stream);
+        sp->diag.alloc(sp->n);
+        launch_fci_diag(sp->da, sp->db, sp->ta->s.p, sp->tb->s.p, sp->ta->d1.p, sp->tb->d1.p, ham->dev(),
+                        sp->diag.p, c->stream);
+    }
+    CF_HIP(hipStreamSynchronize(c->stream));
+    *out = sp.release();
+    API_END
+}
+
+int cf_space_destroy(cf_space* sp) {
+    API_BEGIN
+    if (sp) {
+        sp->ham->ctx->use();
+        delete sp;
+    }
+    API_END
+}
+
+int cf_space_dims(const cf_space* sp, uint64_t* n, uint64_t* dim_up, uint64_t* dim_down) {
+    API_BEGIN
+    CF_REQUIRE(sp, "space is NULL");
+    if (n) *n = sp->n;
+    if (dim_up) *dim_up = sp->da;
+    if (dim_down) *dim_down = sp->db;
+    API_END
+}
+
+int cf_strings_unrank(cf_ctx* ctx, int L, int nel, uint64_t first, uint64_t count, uint64_t* out) {
+    API_BEGIN
+    CF_REQUIRE(ctx && out, "NULL argument");
+    CF_REQUIRE(L >= 1 && L <= kMaxL && nel >= 0 && nel <= L && nel < kBinomCols, "bad L / nel");
+    CF_REQUIRE(first + count <= binom_host(L, nel), "rank range beyond the space");
+    ctx->use();
+    DBuf<str_t> d(count);
+    launch_unrank(L, nel, first, count, ctx->binom.p, d.p, ctx->stream);
+    CF_HIP(hipMemcpyAsync(out, d.p, count * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int cf_strings_rank(cf_ctx* ctx, int L, int nel, const uint64_t* strings, uint64_t count, uint32_t* ranks) {
+    API_BEGIN
+    CF_REQUIRE(ctx && strings && ranks, "NULL argument");
+    CF_REQUIRE(L >= 1 && L <= kMaxL && nel >= 0 && nel < kBinomCols, "bad L / nel");
+    const uint64_t full = L == 64 ? ~0ull : ((1ull << L) - 1);
+    for (uint64_t i = 0; i < count; ++i)
+        CF_REQUIRE((strings[i] & ~full) == 0 && __builtin_popcountll(strings[i]) == nel,
+                   "string outside the n-of-L space");
+    ctx->use();
+    DBuf<str_t> d(count);
+    DBuf<uint32_t> r(count);
+    CF_HIP(hipMemcpyAsync(d.p, strings, count * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    launch_rank(L, nel, d.p, count, ctx->binom.p, r.p, ctx->stream);
+    CF_HIP(hipMemcpyAsync(ranks, r.p, count * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
+    API_BEGIN
+    CF_REQUIRE(sp && out, "NULL argument");
+    CF_REQUIRE(value_prec == CF_F64 || value_prec == CF_F32, "value_prec must be CF_F64 or CF_F32");
+    CF_REQUIRE(zero >= 0.0, "zero threshold must be >= 0");
+    cf_ctx* c = sp->ham->ctx;
+    c->use();
+    std::unique_ptr<cf_crs> m(new cf_crs());
+    m->sp = sp;
+    m->n = sp->n;
+    CF_REQUIRE(m->n < (1ull << 32), "matrix dimension must be < 2^32 (u32 columns)");
+    DBuf<uint32_t> cnt(m->n);
+    m->rowptr.alloc(m->n + 1);
+    const cf_ham* H = sp->ham;
+    FciRowsArgs fa{};
+    DociRowsArgs dargs{};
+    if (sp->kind == CF_FCI) {
+        fa.da = sp->da;
+        fa.db = sp->db;
+        fa.Ea = sp->ta->E;
+        fa.Sa = sp->ta->S;
+        fa.Eb = sp->tb->E;
+        fa.Sb = sp->tb->S;
+        fa.sa = sp->ta->s.p;
+        fa.sb = sp->tb->s.p;
+        fa.fa = sp->ta->full.p;
+        fa.fb = sp->tb->full.p;
+        fa.subb = sp->tb->sub.p;
+        fa.segoff = sp->segoff.p;
+        fa.diag = sp->diag.p;
+        fa.H = H->dev();
+        fa.zero = zero;
+        fa.row0 = 0;
+        fa.nrows = m->n;
+        fa.rowcnt = cnt.p;
+        KTimer t(c, "crs_count");
+        launch_fci_rows(fa, false, c->stream);
+    } else {
+        dargs.d = sp->n;
+        dargs.L = H->L;
+        dargs.np = H->nup;
+        dargs.s = sp->ta->s.p;
+        dargs.binom = c->binom.p;
+        dargs.diag = sp->diag.p;
+        dargs.H = H->dev();
+        dargs.zero = zero;
+        dargs.row0 = 0;
+        dargs.nrows = m->n;
+        dargs.rowcnt = cnt.p;
+        KTimer t(c, "crs_count");
+        launch_doci_rows(dargs, false, c->stream);
+    }
+    exclusive_scan_u32_to_i64(cnt.p, m->rowptr.p, m->n, c->stream);
+    int64_t nnz = 0;
+    CF_HIP(hipMemcpy(&nnz, m->rowptr.p + m->n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    m->nnz = (uint64_t)nnz;
+    m->col.alloc(m->nnz);
+    m->val.alloc(m->nnz);
+    if (value_prec == CF_F32) m->val32.alloc(m->nnz);
+    if (sp->kind == CF_FCI) {
+        fa.rowptr = m->rowptr.p;
+        fa.col = m->col.p;
+        fa.val = m->val.p;
+        fa.val32 = m->val32.p;
+        KTimer t(c, "crs_fill");
+        launch_fci_rows(fa, true, c->stream);
+    } else {
+        dargs.rowptr = m->rowptr.p;
+        dargs.col = m->col.p;
+        dargs.val = m->val.p;
+        dargs.val32 = m->val32.p;
+        KTimer t(c, "crs_fill");
+        launch_doci_rows(dargs, true, c->stream);
+    }
+    CF_HIP(hipStreamSynchronize(c->stream));
+    m->lpr = pick_lanes_per_row(m->n, m->nnz);
+    *out = m.release();
+    API_END
+}
+
+int cf_crs_destroy(cf_crs* m) {
+    API_BEGIN
+    if (m) {
+        m->sp->ham->ctx->use();
+        delete m;
+    }
+    API_END
+}
+
+static uint64_t count_rows(cf_crs* m, uint64_t r0, uint64_t r1, bool upper, std::vector<uint64_t>* per_row) {
+    cf_ctx* c = m->sp->ham->ctx;
+    const uint64_t nr = r1 - r0;
+    std::vector<uint64_t> cnt(nr);
+    if (upper) {
+        DBuf<uint64_t> d(nr);
+        launch_count_upper(m->rowptr.p, m->col.p, r0, r1, d.p, c->stream);
+        CF_HIP(hipMemcpyAsync(cnt.data(), d.p, nr * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<int64_t> rp(nr + 1);
+        CF_HIP(hipMemcpy(rp.data(), m->rowptr.p + r0, (nr + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < nr; ++i) cnt[i] = (uint64_t)(rp[i + 1] - rp[i]);
+    }
+    uint64_t tot = 0;
+    for (auto v : cnt) tot += v;
+    if (per_row) per_row->swap(cnt);
+    return tot;
+}
+
+static int export_rows(cf_crs* m, uint64_t r0, uint64_t r1, bool upper, uint64_t* row, uint32_t* col, double* val) {
+    API_BEGIN
+    CF_REQUIRE(m && row && col && val, "NULL argument");
+    CF_REQUIRE(r0 <= r1 && r1 <= m->n, "row range out of bounds");
+    cf_ctx* c = m->sp->ham->ctx;
+    c->use();
+    std::vector<uint64_t> per;
+    uint64_t tot = count_rows(m, r0, r1, upper, &per);
+    const uint64_t nr = r1 - r0;
+    row[0] = 0;
+    for (uint64_t i = 0; i < nr; ++i) row[i + 1] = row[i] + per[i];
+    if (!nr) return CF_OK;
+    DBuf<uint64_t> off(nr);
+    CF_HIP(hipMemcpy(off.p, row, nr * sizeof(uint64_t), hipMemcpyHostToDevice));
+    DBuf<uint32_t> oc(tot ? tot : 1);
+    DBuf<double> ov(tot ? tot : 1);
+    launch_export_rows(m->rowptr.p, m->col.p, m->val.p, r0, r1, upper, off.p, oc.p, ov.p, c->stream);
+    CF_HIP(hipMemcpyAsync(col, oc.p, tot * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipMemcpyAsync(val, ov.p, tot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int cf_crs_info(const cf_crs* m, uint64_t* n, uint64_t* nnz_full, uint64_t* nnz_upper) {
+    API_BEGIN
+    CF_REQUIRE(m, "matrix is NULL");
+    if (n) *n = m->n;
+    if (nnz_full) *nnz_full = m->nnz;
+    if (nnz_upper) {
+        cf_crs* mm = const_cast<cf_crs*>(m);
+        if (mm->nnz_upper < 0) {
+            mm->sp->ham->ctx->use();
+            mm->nnz_upper = (int64_t)count_rows(mm, 0, mm->n, true, nullptr);
+        }
+        *nnz_upper = (uint64_t)mm->nnz_upper;
+    }
+    API_END
+}
+
+int cf_crs_count_upper(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* count) {
+    API_BEGIN
+    CF_REQUIRE(m && count, "NULL argument");
+    CF_REQUIRE(row_begin <= row_end && row_end <= m->n, "row range out of bounds");
+    m->sp->ham->ctx->use();
+    *count = count_rows(m, row_begin, row_end, true, nullptr);
+    API_END
+}
+
+int cf_crs_count_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* count) {
+    API_BEGIN
+    CF_REQUIRE(m && count, "NULL argument");
+    CF_REQUIRE(row_begin <= row_end && row_end <= m->n, "row range out of bounds");
+    m->sp->ham->ctx->use();
+    *count = count_rows(m, row_begin, row_end, false, nullptr);
+    API_END
+}
+
+int cf_crs_export_upper(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* row, uint32_t* col,
+                        double* val) {
+    return export_rows(m, row_begin, row_end, true, row, col, val);
+}
+
+int cf_crs_export_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* row, uint32_t* col,
+                       double* val) {
+    return export_rows(m, row_begin, row_end, false, row, col, val);
+}
+
+int cf_mvprod(cf_crs* m, const double* x, double* y) {
+    API_BEGIN
+    CF_REQUIRE(m && x && y, "NULL argument");
+    cf_ctx* c = m->sp->ham->ctx;
+    c->use();
+    ensure_staging(c, m->n);
+    CF_HIP(hipMemcpyAsync(c->hx.p, x, m->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    spmv(m, CF_F64, c->hx.p, c->hy.p);
+    CF_HIP(hipMemcpyAsync(y, c->hy.p, m->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int cf_mvprod_dev(cf_crs* m, int value_prec, const double* dx, double* dy) {
+    API_BEGIN
+    CF_REQUIRE(m && dx && dy, "NULL argument");
+    m->sp->ham->ctx->use();
+    spmv(m, value_prec, dx, dy);
+    API_END
+}
+
+void cf_solve_opts_default(cf_solve_opts* o) {
+    if (!o) return;
+    o->ncv = 42;
+    o->maxrestart = 0;
+    o->tol = 1e-12;
+    o->precision = CF_F64;
+    o->tol_f32 = 1e-8;
+    o->seed = 20240601ULL;
+}
+
+int cf_solve(cf_crs* m, const cf_solve_opts* opts, double* eigvec, cf_solve_info* info) {
+    API_BEGIN
+    CF_REQUIRE(m && info, "NULL argument");
+    cf_solve_opts o;
+    if (opts) o = *opts;
+    else cf_solve_opts_default(&o);
+    CF_REQUIRE(o.ncv >= 2 && o.ncv <= 63, "ncv must be in [2, 63]");
+    CF_REQUIRE(o.tol > 0.0, "tol must be > 0");
+    CF_REQUIRE(o.precision == CF_F64 || o.precision == CF_F32, "precision must be CF_F64 or CF_F32");
+    cf_ctx* c = m->sp->ham->ctx;
+    c->use();
+    const auto t0 = std::chrono::steady_clock::now();
+    const uint64_t n = m->n;
+    const int maxr = o.maxrestart > 0 ? o.maxrestart : (int)std::min<uint64_t>(3 * n, 1u << 30);
+    DeviceBackend b(c, n, o.ncv);
+    int nmv32 = 0, nrs = 0;
+    bool keep = false;
+    if (o.precision == CF_F32) {
+        CF_REQUIRE(m->val32.p != nullptr, "FP32 solve needs a matrix built with CF_F32");
+        b.op = [m](const double* x, double* y) { spmv(m, CF_F32, x, y); };
+        ThickRestartLanczos sm(&b, n, o.ncv, o.tol_f32, maxr, o.seed, false);
+        while (sm.step() == ThickRestartLanczos::NEED_APPLY) b.apply(sm.pending());
+        nmv32 = sm.n_apply();
+        nrs += sm.n_restart();
+        keep = true;  // v0 holds the FP32-phase Ritz vector: polish from it
+    }
+    b.op = [m](const double* x, double* y) { spmv(m, CF_F64, x, y); };
+    ThickRestartLanczos sm(&b, n, o.ncv, o.tol, maxr, o.seed, keep);
+    while (sm.step() == ThickRestartLanczos::NEED_APPLY) b.apply(sm.pending());
+    info->eigval = sm.eigval();
+    info->energy = sm.eigval() + m->sp->ham->econst;
+    info->resid = sm.resid();
+    info->n_matvec = sm.n_apply();
+    info->n_matvec_f32 = nmv32;
+    info->n_restart = nrs + sm.n_restart();
+    info->status = sm.status();
+    // resident copy of the eigenvector (sign convention applied on export)
+    cf_space* sp = m->sp;
+    if (sp->vec.n != n) sp->vec.alloc(n);
+    launch_copy(n, b.v(0), sp->vec.p, c->stream);
+    std::vector<double> hv(n);
+    CF_HIP(hipMemcpyAsync(hv.data(), b.v(0), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipStreamSynchronize(c->stream));
+    sign_fix(hv);
+    CF_HIP(hipMemcpy(sp->vec.p, hv.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    sp->vec_valid = true;
+    if (eigvec) std::memcpy(eigvec, hv.data(), n * sizeof(double));
+    info->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    API_END
+}
+
+int cf_rci_create(uint64_t n, int ncv, double tol, int maxrestart, uint64_t seed, cf_rci** out) {
+    API_BEGIN
+    CF_REQUIRE(out && n > 0, "bad arguments");
+    CF_REQUIRE(ncv >= 2 && ncv <= 63 && tol > 0.0, "bad ncv / tol");
+    std::unique_ptr<cf_rci> r(new cf_rci());
+    r->n = n;
+    r->b.reset(new HostBackend(n, ncv));
+    int maxr = maxrestart > 0 ? maxrestart : (int)std::min<uint64_t>(3 * n, 1u << 30);
+    r->sm.reset(new ThickRestartLanczos(r->b.get(), n, ncv, tol, maxr, seed, false));
+    *out = r.release();
+    API_END
+}
+
+int cf_rci_step(cf_rci* r, int* ido, double* workd, int* ipntr) {
+    API_BEGIN
+    CF_REQUIRE(r && ido && workd && ipntr, "NULL argument");
+    CF_REQUIRE(!r->done, "reverse communication already finished");
+    const uint64_t n = r->n;
+    if (*ido == 0) {
+        CF_REQUIRE(!r->started, "ido == 0 after the iteration started");
+        r->started = true;
+    } else if (*ido == 1) {
+        CF_REQUIRE(r->started, "ido == 1 before the first call");
+        std::memcpy(r->b->w.data(), workd + n, n * sizeof(double));
+    } else {
+        throw Error(ERR_ARG, "ido must be 0 or 1");
+    }
+    if (r->sm->step() == ThickRestartLanczos::NEED_APPLY) {
+        std::memcpy(workd, r->b->v(r->sm->pending()), n * sizeof(double));
+        ipntr[0] = 1;
+        ipntr[1] = (int)n + 1;
+        *ido = 1;
+    } else {
+        r->done = true;
+        *ido = 99;
+    }
+    API_END
+}
+
+int cf_rci_result(cf_rci* r, double* eigval, double* eigvec, int* info) {
+    API_BEGIN
+    CF_REQUIRE(r && r->done, "reverse communication not finished");
+    if (eigval) *eigval = r->sm->eigval();
+    if (info) *info = r->sm->status();
+    if (eigvec) {
+        std::vector<double> x(r->b->v(0), r->b->v(0) + r->n);
+        sign_fix(x);
+        std::memcpy(eigvec, x.data(), r->n * sizeof(double));
+    }
+    API_END
+}
+
+int cf_rci_destroy(cf_rci* r) {
+    API_BEGIN
+    delete r;
+    API_END
+}
+
+int cf_rdm(cf_space* sp, const double* cvec, double* one, double* two) {
+    API_BEGIN
+    CF_REQUIRE(sp && cvec && one, "NULL argument");
+    cf_ctx* c = sp->ham->ctx;
+    c->use();
+    const int L = sp->ham->L;
+    std::vector<double> g(2 * L * L), G;
+    if (two) G.resize((size_t)3 * L * L * L * L);
+    {
+        KTimer t(c, "rdm");
+        rdm_dense(c->stream, sp->kind, L, sp->ham->nup, sp->ham->ndown, sp->da, sp->db, sp->ta->s.p, sp->tb->s.p,
+                  sp->ta->sub.p, sp->tb->sub.p, sp->ta->S, sp->tb->S, c->binom.p, cvec, sp->n, g.data(),
+                  two ? G.data() : nullptr);
+    }
+    pack_rdm(L, g.data(), two ? G.data() : nullptr, one, two);
+    API_END
+}
+
+int cf_rdm_dense(cf_space* sp, const double* cvec, double* gamma, double* Gamma) {
+    API_BEGIN
+    CF_REQUIRE(sp && cvec && gamma, "NULL argument");
+    cf_ctx* c = sp->ham->ctx;
+    c->use();
+    const int L = sp->ham->L;
+    KTimer t(c, "rdm");
+    rdm_dense(c->stream, sp->kind, L, sp->ham->nup, sp->ham->ndown, sp->da, sp->db, sp->ta->s.p, sp->tb->s.p,
+              sp->ta->sub.p, sp->tb->sub.p, sp->ta->S, sp->tb->S, c->binom.p, cvec, sp->n, gamma, Gamma);
+    API_END
+}
+
+}  // extern "C"

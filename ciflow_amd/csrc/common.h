@@ -1,0 +1,84 @@
+// Shared definitions for the ciflow_amd library (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace cf {
+
+typedef uint64_t str_t;  // one spin string: bit k = spatial orbital k occupied
+
+// Error carrying a status code across the C ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status {
+    OK = 0,
+    ERR_ARG = 1,
+    ERR_HIP = 2,
+    ERR_NOMEM = 3,
+    ERR_NOCONV = 4,
+    ERR_UNSUPPORTED = 5,
+    ERR_STATE = 6,
+};
+
+#define CF_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::cf::Error(e_ == hipErrorOutOfMemory ? ::cf::ERR_NOMEM : ::cf::ERR_HIP,    \
+                              std::string(#call) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+#define CF_REQUIRE(cond, msg)                                  \
+    do {                                                       \
+        if (!(cond)) throw ::cf::Error(::cf::ERR_ARG, (msg));  \
+    } while (0)
+
+// Owning device buffer.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count) CF_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Binomial coefficients C(k, m) for 0 <= k <= 64, 0 <= m <= 64 (exact while < 2^64).
+inline uint64_t binom_host(int n, int k) {
+    if (k < 0 || k > n) return 0;
+    unsigned __int128 r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (unsigned)(n - k + i) / (unsigned)i;
+    return (uint64_t)r;
+}
+
+constexpr int kMaxL = 64;
+constexpr int kBinomCols = 33;  // m <= 32 electrons per spin string
+
+}  // namespace cf

@@ -1,0 +1,113 @@
+// Device data structures and kernel launchers of the ciflow_amd library.
+#pragma once
+
+#include "common.h"
+
+namespace cf {
+
+// One entry of a string's excitation list, sorted by target rank J.
+//  type 0: the string itself          x = one-string diagonal part
+//  type 1: single, orbital f -> t     x = h_pq + sum_z (V_pzqz - V_pzzq), sign = (-1)^(stacked)
+//  type 2: same-spin double           x = signed matrix element (canonical orientation)
+struct Exc {
+    uint32_t J;
+    uint8_t type;
+    int8_t sign;
+    uint8_t f;
+    uint8_t t;
+    double x;
+};
+static_assert(sizeof(Exc) == 16, "Exc must stay 16 bytes");
+
+// Device-side view of a Hamiltonian.
+struct HamDev {
+    int L;
+    const double* h;   // L*L
+    const double* V;   // L^4 physics, V[((i*L+j)*L+k)*L+l]
+    const double* W;   // L^3, W[(p*L+q)*L+d] = V(p,d,q,d)
+    const double* Jm;  // L^2, Jm[k*L+l] = V(k,l,k,l)
+};
+
+// launchers (all asynchronous on `stream`)
+void launch_unrank(int L, int nel, uint64_t first, uint64_t count, const uint64_t* binom, str_t* out,
+                   hipStream_t stream);
+void launch_rank(int L, int nel, const str_t* strings, uint64_t count, const uint64_t* binom, uint32_t* out,
+                 hipStream_t stream);
+void launch_diag1(int nel, const str_t* strs, uint64_t d, HamDev H, double* out, hipStream_t stream);
+// Excitation lists of all strings: full (E per string) and self+singles (S per string).
+void launch_exc_lists(int L, int nel, const str_t* strs, uint64_t d, const uint64_t* binom, HamDev H,
+                      const double* diag1, int E, int S, Exc* full, Exc* sub, hipStream_t stream);
+// Row-layout offsets of the FCI rows of each alpha string: segoff[ia*(Ea+1)+e].
+void launch_fci_segoff(uint64_t da, int Ea, const Exc* full_a, int Eb, int Sb, uint32_t* segoff,
+                       hipStream_t stream);
+void launch_fci_diag(uint64_t da, uint64_t db, const str_t* sa, const str_t* sb, const double* d1a,
+                     const double* d1b, HamDev H, double* diag, hipStream_t stream);
+void launch_doci_diag(uint64_t d, const str_t* s, const double* d1, HamDev H, double* diag, hipStream_t stream);
+
+struct FciRowsArgs {
+    uint64_t da, db;
+    int Ea, Sa, Eb, Sb;
+    const str_t* sa;
+    const str_t* sb;
+    const Exc* fa;   // full alpha lists
+    const Exc* fb;   // full beta lists
+    const Exc* subb; // beta self+singles lists
+    const uint32_t* segoff;
+    const double* diag;
+    HamDev H;
+    double zero;
+    uint64_t row0, nrows;  // row range handled by this launch
+    // count pass
+    uint32_t* rowcnt;
+    // fill pass
+    const int64_t* rowptr;
+    uint32_t* col;
+    double* val;
+    float* val32;
+};
+void launch_fci_rows(const FciRowsArgs& a, bool fill, hipStream_t stream);
+
+struct DociRowsArgs {
+    uint64_t d;
+    int L, np;
+    const str_t* s;
+    const uint64_t* binom;
+    const double* diag;
+    HamDev H;
+    double zero;
+    uint64_t row0, nrows;
+    uint32_t* rowcnt;
+    const int64_t* rowptr;
+    uint32_t* col;
+    double* val;
+    float* val32;
+};
+void launch_doci_rows(const DociRowsArgs& a, bool fill, hipStream_t stream);
+
+// Exclusive scan of 32-bit counts into 64-bit offsets (n+1 entries).
+void exclusive_scan_u32_to_i64(const uint32_t* cnt, int64_t* out, uint64_t n, hipStream_t stream);
+
+// Upper-triangle / full export of a row range.
+void launch_count_upper(const int64_t* rowptr, const uint32_t* col, uint64_t r0, uint64_t r1, uint64_t* cnt,
+                        hipStream_t stream);
+void launch_export_rows(const int64_t* rowptr, const uint32_t* col, const double* val, uint64_t r0, uint64_t r1,
+                        bool upper_only, const uint64_t* out_off, uint32_t* ocol, double* oval, hipStream_t stream);
+
+// SpMV y = A x on full rows.
+void launch_spmv_f64(uint64_t n, const int64_t* rowptr, const uint32_t* col, const double* val, const double* x,
+                     double* y, int lanes_per_row, hipStream_t stream);
+void launch_spmv_f32(uint64_t n, const int64_t* rowptr, const uint32_t* col, const float* val, const double* x,
+                     double* y, int lanes_per_row, hipStream_t stream);
+
+// Krylov vector kernels (vectors are columns of a column-major n x k block).
+void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
+                      hipStream_t stream);
+void launch_reduce_partials(int k, int nblocks, const double* partial, double* out, hipStream_t stream);
+int multi_dot_blocks(uint64_t n);
+void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream);
+void launch_scale_copy(uint64_t n, const double* src, double s, double* dst, hipStream_t stream);
+void launch_rotate_basis(uint64_t n, int m, int k, double* Vb, const double* Y, hipStream_t stream);
+void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream);
+void launch_copy(uint64_t n, const double* src, double* dst, hipStream_t stream);
+
+}  // namespace cf

@@ -1,0 +1,173 @@
+// Krylov-vector kernels for the GPU-resident Lanczos (the dense work ARPACK's
+// dsaitr/dsapps do on the host in the reference, SparseMatrix_CRS.cpp:441-458).
+// All HBM-bound: a basis of k vectors of length n is stored column-major
+// (vector i at Vb + i*n).
+#include "kernels.h"
+
+namespace cf {
+
+constexpr int kDotChunk = 8;  // vectors per block row
+constexpr int kDotThreads = 256;
+
+int multi_dot_blocks(uint64_t n) {
+    uint64_t nb = (n + 4095) / 4096;
+    if (nb < 1) nb = 1;
+    if (nb > 512) nb = 512;
+    return (int)nb;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// partial[i * nblocks + bx] = sum over this block's elements of Vb[i] . w
+__global__ void __launch_bounds__(kDotThreads) k_multi_dot(uint64_t n, int k, const double* __restrict__ Vb,
+                                                           const double* __restrict__ w, double* __restrict__ partial) {
+    __shared__ double red[kDotChunk][kDotThreads / 64];
+    const int i0 = blockIdx.y * kDotChunk;
+    double acc[kDotChunk];
+#pragma unroll
+    for (int j = 0; j < kDotChunk; ++j) acc[j] = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * kDotThreads;
+    for (uint64_t q = (uint64_t)blockIdx.x * kDotThreads + threadIdx.x; q < n; q += stride) {
+        const double wq = w[q];
+#pragma unroll
+        for (int j = 0; j < kDotChunk; ++j)
+            if (i0 + j < k) acc[j] += Vb[(uint64_t)(i0 + j) * n + q] * wq;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kDotChunk; ++j) {
+        double s = wave_sum(acc[j]);
+        if (lane == 0) red[j][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kDotChunk && i0 + (int)threadIdx.x < k) {
+        double s = 0.0;
+        for (int q = 0; q < kDotThreads / 64; ++q) s += red[threadIdx.x][q];
+        partial[(uint64_t)(i0 + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
+                      hipStream_t stream) {
+    if (k <= 0) return;
+    dim3 grid((unsigned)nblocks, (unsigned)((k + kDotChunk - 1) / kDotChunk));
+    k_multi_dot<<<grid, kDotThreads, 0, stream>>>(n, k, Vb, w, partial);
+    CF_HIP(hipGetLastError());
+}
+
+// out[i] = sum_b partial[i*nblocks + b], fixed order (deterministic).
+__global__ void k_reduce_partials(int k, int nblocks, const double* __restrict__ partial, double* __restrict__ out) {
+    const int i = blockIdx.x;
+    if (i >= k) return;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 64) s += partial[(uint64_t)i * nblocks + b];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) out[i] = s;
+}
+
+void launch_reduce_partials(int k, int nblocks, const double* partial, double* out, hipStream_t stream) {
+    if (k <= 0) return;
+    k_reduce_partials<<<k, 64, 0, stream>>>(k, nblocks, partial, out);
+    CF_HIP(hipGetLastError());
+}
+
+// w -= Vb[:, 0:k] c
+__global__ void __launch_bounds__(256) k_multi_axpy(uint64_t n, int k, const double* __restrict__ Vb,
+                                                    const double* __restrict__ c, double* __restrict__ w) {
+    __shared__ double cs[64];
+    if (threadIdx.x < 64) cs[threadIdx.x] = (int)threadIdx.x < k ? c[threadIdx.x] : 0.0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        double s = 0.0;
+        for (int i = 0; i < k; ++i) s += cs[i] * Vb[(uint64_t)i * n + q];
+        w[q] -= s;
+    }
+}
+
+void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream) {
+    if (k <= 0 || !n) return;
+    if (k > 64) throw Error(ERR_ARG, "multi_axpy: k > 64");
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    k_multi_axpy<<<(unsigned)grid, 256, 0, stream>>>(n, k, Vb, c, w);
+    CF_HIP(hipGetLastError());
+}
+
+// dst = src * s
+__global__ void k_scale_copy(uint64_t n, const double* __restrict__ src, double s, double* __restrict__ dst) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = src[q] * s;
+}
+
+void launch_scale_copy(uint64_t n, const double* src, double s, double* dst, hipStream_t stream) {
+    if (!n) return;
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    k_scale_copy<<<(unsigned)grid, 256, 0, stream>>>(n, src, s, dst);
+    CF_HIP(hipGetLastError());
+}
+
+// In place: Vb[:, 0:k] = Vb[:, 0:m] * Y[:, 0:k]   (Y is m x m row-major).
+// Each thread owns element q of every vector, so reading all m inputs before
+// writing the k outputs is race-free.
+__global__ void __launch_bounds__(256) k_rotate_basis(uint64_t n, int m, int k, double* __restrict__ Vb,
+                                                      const double* __restrict__ Y) {
+    __shared__ double ys[64 * 64];
+    for (int i = threadIdx.x; i < m * m; i += blockDim.x) ys[i] = Y[i];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        double v[64];
+#pragma unroll
+        for (int i = 0; i < 64; ++i) v[i] = (i < m) ? Vb[(uint64_t)i * n + q] : 0.0;
+        for (int c = 0; c < k; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < 64; ++i)
+                if (i < m) s += v[i] * ys[i * m + c];
+            Vb[(uint64_t)c * n + q] = s;
+        }
+    }
+}
+
+void launch_rotate_basis(uint64_t n, int m, int k, double* Vb, const double* Y, hipStream_t stream) {
+    if (m > 64 || k > m) throw Error(ERR_ARG, "rotate_basis: m > 64 or k > m");
+    if (!n) return;
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    k_rotate_basis<<<(unsigned)grid, 256, 0, stream>>>(n, m, k, Vb, Y);
+    CF_HIP(hipGetLastError());
+}
+
+// Deterministic start vector: v[q] = 1 + 0.5*(u(seed, q) - 0.5), u in [0,1).
+__global__ void k_fill_start(uint64_t n, uint64_t seed, double* __restrict__ v) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (q + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        v[q] = 1.0 + 0.5 * (u - 0.5);
+    }
+}
+
+void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream) {
+    if (!n) return;
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    k_fill_start<<<(unsigned)grid, 256, 0, stream>>>(n, seed, v);
+    CF_HIP(hipGetLastError());
+}
+
+void launch_copy(uint64_t n, const double* src, double* dst, hipStream_t stream) {
+    if (!n) return;
+    CF_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+}
+
+}  // namespace cf

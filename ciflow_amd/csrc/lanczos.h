@@ -1,0 +1,77 @@
+// Thick-restart Lanczos for the lowest eigenpair, written as a
+// reverse-communication state machine so the same control logic drives
+//   * the GPU-resident solver (Krylov vectors in HBM, operator = GPU SpMV),
+//   * the dsaupd_-shaped host RCI entry point (cf_rci_*), which the
+//     reference's arpackDiagonalize loop shape expects
+//     (SparseMatrix_CRS.cpp:441-449: call, get ido == 1, apply mvprod, call).
+//
+// ARPACK (arpack-ng 3.1.5-3.3.0, an external Fortran dependency of the
+// reference) is not available; this is the published thick-restart Lanczos
+// of Wu & Simon (SIAM J. Matrix Anal. Appl. 22 (2000) 602) with full
+// classical Gram-Schmidt re-orthogonalisation done twice, keeping ARPACK's
+// parameters (which="SA", nev=1, ncv=min(42,n), convergence
+// ||r|| <= tol*|theta|).  The returned eigenvalue is the Rayleigh quotient of
+// the normalised Ritz vector, evaluated with one extra operator application.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace cf {
+
+// Vector storage + operations of one Lanczos run.  Indices refer to basis
+// columns 0..ncv (ncv+1 columns); "w" is the work vector.
+struct KrylovBackend {
+    virtual ~KrylovBackend() {}
+    virtual void start_random(uint64_t seed) = 0;  // v0 = normalised deterministic random vector
+    virtual void start_keep() = 0;                 // v0 already holds the start vector: normalise it
+    virtual void apply(int j) = 0;                 // w = A v_j (not used by the RCI backend)
+    // Two passes of classical Gram-Schmidt of w against v_0..v_j; alpha = total
+    // coefficient on v_j, beta = ||w|| afterwards.
+    virtual void orth(int j, double* alpha, double* beta) = 0;
+    virtual void set_next(int j1, double beta) = 0;  // v_j1 = w / beta
+    virtual void rotate(int m, int k, const double* Y) = 0;  // v_0..v_k-1 = V_m Y[:, 0:k] (Y m x m row-major)
+    virtual void move(int from, int to) = 0;
+    virtual void normalize0() = 0;
+    // with w = A v_0: theta = v_0.w, resid = ||w - theta v_0||
+    virtual void rayleigh(double* theta, double* resid) = 0;
+};
+
+// Symmetric eigen-decomposition (cyclic Jacobi), ascending; Z columns = vectors.
+void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& Z);
+
+class ThickRestartLanczos {
+public:
+    enum Event { NEED_APPLY = 1, DONE = 99 };
+    ThickRestartLanczos(KrylovBackend* b, uint64_t n, int ncv, double tol, int maxrestart, uint64_t seed,
+                        bool start_given);
+    // Advance until the operator must be applied to v_{pending()} (NEED_APPLY,
+    // the result is expected in the backend's w) or the run ended (DONE).
+    Event step();
+    int pending() const { return pending_; }
+    double eigval() const { return theta_; }
+    double resid() const { return resid_; }
+    int n_apply() const { return napply_; }
+    int n_restart() const { return nrestart_; }
+    int status() const { return status_; }  // 0 converged, 1 restart limit
+
+private:
+    enum Phase { INIT, LOOP_APPLY, AFTER_APPLY, FINAL_APPLY, AFTER_FINAL, FINISHED };
+    void eig_and_restart();
+    KrylovBackend* b_;
+    uint64_t n_;
+    int M_, m_ = 0, j_ = 0, k_ = 0;
+    double tol_;
+    int maxrestart_;
+    uint64_t seed_;
+    bool start_given_;
+    Phase phase_ = INIT;
+    int pending_ = 0;
+    std::vector<double> T_;
+    double beta_last_ = 0.0, anorm_ = 0.0;
+    bool breakdown_ = false;
+    double theta_ = 0.0, resid_ = 0.0;
+    int napply_ = 0, nrestart_ = 0, status_ = 1;
+};
+
+}  // namespace cf

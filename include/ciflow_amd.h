@@ -1,0 +1,169 @@
+/*
+ * ciflow_amd.h -- C ABI of the MI355X (gfx950) CI ground-state library.
+ *
+ * Every entry point takes plain C types, opaque handles and host or device
+ * pointers, and returns an int status (CF_OK on success); cf_last_error()
+ * returns the message of the last failure on the calling thread.  No
+ * exceptions cross the ABI.  A handle is used from one host thread at a time;
+ * different handles may be used concurrently.
+ *
+ * Each declaration names the reference CIFlow interface it replaces
+ * (file:line in the CIFlow sources).
+ */
+#ifndef CIFLOW_AMD_H
+#define CIFLOW_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CF_OK = 0,
+    CF_ERR_ARG = 1,
+    CF_ERR_HIP = 2,
+    CF_ERR_NOMEM = 3,
+    CF_ERR_NOCONV = 4,
+    CF_ERR_UNSUPPORTED = 5,
+    CF_ERR_STATE = 6
+};
+
+/* CI space kinds: CIMethod subclasses FCI and DOCI (CIMethod.h:135-162). */
+enum { CF_FCI = 0, CF_DOCI = 1 };
+
+/* Value storage of the CRS matrix and arithmetic of a product. */
+enum {
+    CF_F64 = 0,     /* FP64 values (the reference's SparseMatrix_CRS data) */
+    CF_F32 = 1      /* FP32-stored values, FP64 accumulation */
+};
+
+typedef struct cf_ctx cf_ctx;
+typedef struct cf_ham cf_ham;
+typedef struct cf_space cf_space;
+typedef struct cf_crs cf_crs;
+typedef struct cf_rci cf_rci;
+
+/* ---- library / device context ------------------------------------------ */
+int cf_version(void);
+const char* cf_last_error(void);
+/* One context = one GPU + one stream.  No reference counterpart (the
+ * reference is a single-process CPU library). */
+int cf_ctx_create(int device, cf_ctx** out);
+int cf_ctx_destroy(cf_ctx* ctx);
+int cf_ctx_synchronize(cf_ctx* ctx);
+/* Per-kernel event timing on the context stream: enable, then read the
+ * number of timed launches and their summed duration (ms) for a kernel
+ * family: "spmv_f64", "spmv_f32", "crs_count", "crs_fill", "sigma", "rdm". */
+int cf_ctx_enable_timing(cf_ctx* ctx, int enable);
+int cf_ctx_kernel_time(cf_ctx* ctx, const char* family, int64_t* launches, double* total_ms);
+int cf_ctx_reset_timing(cf_ctx* ctx);
+
+/* ---- integrals: replaces Hamiltonian::getTmat/getVmat/getEconst/getL/
+ *      getnup/getndown (Hamiltonian.h:80-136, Hamiltonian.cpp:267-320).
+ *      h: L*L row-major, h[i*L+j] = getTmat(i,j).
+ *      V: L^4 row-major, V[((i*L+j)*L+k)*L+l] = getVmat(i,j,k,l) (physics).
+ *      For a model Hamiltonian with _modham set, pass getVmat's values
+ *      (zero unless i == j), as Hamiltonian.cpp:311-318 returns them. */
+int cf_ham_create(cf_ctx* ctx, int L, int nup, int ndown, const double* h, const double* V, double econst,
+                  cf_ham** out);
+int cf_ham_destroy(cf_ham* ham);
+/* Seeded synthetic integrals (no reference counterpart: the reference reads
+ * psi4 dumps).  h_ii = -2+3i/(L-1), h_ij ~ N(0,0.05^2); (ij|kl) = sum_P
+ * B^P_ij B^P_kl over 2L symmetric B^P ~ N(0,0.1^2) with B^P_PP += 0.6 (P<L). */
+int cf_gen_integrals(int L, uint64_t seed, double* h, double* V);
+
+/* ---- determinant space: Permutator_Bit::get_start_int/permutate_bit
+ *      (Permutator.cpp:113-153) and vertex-weight ranking determine_weight
+ *      (CIMethod.cpp:663-747).  Determinant order: alpha rank slow, beta
+ *      rank fast (CIMethod.cpp:771). */
+int cf_space_create(cf_ham* ham, int kind, cf_space** out);
+int cf_space_destroy(cf_space* sp);
+int cf_space_dims(const cf_space* sp, uint64_t* n, uint64_t* dim_up, uint64_t* dim_down);
+/* Unrank strings [first, first+count) of the n-of-L space on the GPU (host out). */
+int cf_strings_unrank(cf_ctx* ctx, int L, int nel, uint64_t first, uint64_t count, uint64_t* out);
+/* Rank arbitrary n-of-L strings on the GPU (host in/out). */
+int cf_strings_rank(cf_ctx* ctx, int L, int nel, const uint64_t* strings, uint64_t count, uint32_t* ranks);
+
+/* ---- Slater-Condon CRS build: FCI/DOCI::construct_CI_matrix via
+ *      CIMethod::build_parallel (CIMethod.cpp:489-561, 611-660, 749-888).
+ *      The GPU keeps full rows (both triangles); entries with |v| <= zero
+ *      are not stored, as SparseMatrix_CRS::PushToRow does (:240-272).
+ *      value_prec CF_F32 also stores an FP32 copy of the values. */
+int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out);
+int cf_crs_destroy(cf_crs* m);
+int cf_crs_info(const cf_crs* m, uint64_t* n, uint64_t* nnz_full, uint64_t* nnz_upper);
+/* Export rows [row_begin, row_end) of the upper triangle in the reference
+ * layout (SparseMatrix_CRS.h:109-114: row pointer, column, value; columns
+ * ascending, col >= row).  row has row_end-row_begin+1 entries, relative to
+ * the first exported row; col/val must hold the exported count. */
+int cf_crs_count_upper(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* count);
+int cf_crs_export_upper(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* row, uint32_t* col,
+                        double* val);
+/* Full-row export (both triangles), same conventions. */
+int cf_crs_count_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* count);
+int cf_crs_export_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* row, uint32_t* col,
+                       double* val);
+
+/* ---- matrix-vector product: SparseMatrix_CRS_Sym::mvprod
+ *      (SparseMatrix_CRS.cpp:700-723), the virtual seam that
+ *      arpackDiagonalize calls (:446).  y is overwritten. */
+int cf_mvprod(cf_crs* m, const double* x, double* y);                       /* host pointers, synchronous */
+int cf_mvprod_dev(cf_crs* m, int value_prec, const double* dx, double* dy);  /* device pointers, ctx stream */
+
+/* ---- ground state: CIMethod::solve -> SparseMatrix_CRS::arpackDiagonalize
+ *      (CIMethod.cpp:241-268, SparseMatrix_CRS.cpp:360-481), ARPACK's
+ *      parameters: which=SA, nev=1, ncv=min(42,n), maxiter=3n.
+ *      Krylov vectors stay in HBM; the host runs the restart logic. */
+typedef struct {
+    int ncv;            /* Krylov basis size (42 as the reference) */
+    int maxrestart;     /* <= 0: 3n as the reference */
+    double tol;         /* ||r|| <= tol*|theta| (1e-12) */
+    int precision;      /* CF_F64, or CF_F32: FP32-value Lanczos + FP64 polish */
+    double tol_f32;     /* convergence of the FP32 phase (1e-8) */
+    uint64_t seed;      /* start-vector seed */
+} cf_solve_opts;
+
+typedef struct {
+    double eigval;       /* lowest eigenvalue of the CI matrix (no Econst) */
+    double energy;       /* eigval + Econst: CIMethod::get_ci_energy */
+    double resid;        /* ||H x - theta x|| of the returned vector (FP64 operator) */
+    int n_matvec;        /* FP64 products */
+    int n_matvec_f32;    /* FP32-value products */
+    int n_restart;
+    int status;          /* 0 converged, 1 restart limit reached (ARPACK info=1) */
+    double ms;           /* wall time of the solve */
+} cf_solve_info;
+
+void cf_solve_opts_default(cf_solve_opts* o);
+/* eigvec: host array of n doubles or NULL.  The vector also stays on the GPU
+ * for cf_rdm_resident. */
+int cf_solve(cf_crs* m, const cf_solve_opts* opts, double* eigvec, cf_solve_info* info);
+
+/* ---- reverse-communication Lanczos with the dsaupd_ contract
+ *      (matrix.h:42-52; loop at SparseMatrix_CRS.cpp:441-449): on ido == 1
+ *      the caller computes workd[ipntr[1]-1 ...] = A * workd[ipntr[0]-1 ...]
+ *      (length n) and calls again; ido == 99 ends.  Host vectors. */
+int cf_rci_create(uint64_t n, int ncv, double tol, int maxrestart, uint64_t seed, cf_rci** out);
+int cf_rci_step(cf_rci* r, int* ido, double* workd, int* ipntr);
+/* dseupd_ counterpart: eigenvalue, eigenvector (n), info (0 / 1). */
+int cf_rci_result(cf_rci* r, double* eigval, double* eigvec, int* info);
+int cf_rci_destroy(cf_rci* r);
+
+/* ---- RDMs: CIMethod::construct_density -> CIDens::construct_density
+ *      (CIMethod.cpp:161-171, CIDens.cpp:323-334, DensFCI :1530-1631,
+ *      DensDOCI :1073-1224).  Output layout = the reference storage:
+ *      one[2][L(L+1)/2]      packed upper (CIDens.cpp:255-288),
+ *      two[3][L^2(L^2+1)/2]  Vector2d(L^2,L^2) blocks aa, ab, bb
+ *                            (CIDens.cpp:166-252, matrix.cpp:674-689).
+ *      two may be NULL (1-RDM only).  c: host CI vector (n). */
+int cf_rdm(cf_space* sp, const double* c, double* one, double* two);
+/* Dense form: gamma[2][L][L], Gamma[3][L][L][L][L] with
+ * Gamma[b][i][j][k][l] = <a+_is a+_jt a_lt a_ks>, b: 0 aa, 1 ab, 2 bb. */
+int cf_rdm_dense(cf_space* sp, const double* c, double* gamma, double* Gamma);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CIFLOW_AMD_H */
