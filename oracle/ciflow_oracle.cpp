@@ -1097,3 +1097,24 @@ extern "C" int or_num_threads(void) {
     return 1;
 #endif
 }
+
+// Whole ground-state pipeline in one call (for large golden values without
+// Python-side copies): build_parallel + add_from_list, expansion to full rows,
+// Lanczos.  Returns the Lanczos status; E and the product count via pointers.
+extern "C" int or_ground_state(int kind, int L, int nup, int ndown, const double* h, const double* V,
+                               double zero, int num_t, int ncv, double tol, double* E, int* n_matvec,
+                               uint64_t* nnz_upper, double* eigvec /* may be null */) {
+    const int lines = (int)binom(L, nup);
+    OrCRS* up = (OrCRS*)or_build_crs(kind, L, nup, ndown, h, V, zero, 0, lines, num_t);
+    *nnz_upper = up->data.size();
+    OrCRS* full = (OrCRS*)or_crs_full_from_upper(up->nrows, up->row.data(), up->col.data(), up->data.data());
+    const uint64_t n = up->nrows;
+    delete up;
+    std::vector<double> vec(n);
+    int nrs = 0;
+    int st = or_lanczos(n, full->row.data(), full->col.data(), full->data.data(), ncv, tol, (int)std::min<uint64_t>(3 * n, 1u << 30),
+                        num_t, E, vec.data(), n_matvec, &nrs);
+    if (eigvec) std::memcpy(eigvec, vec.data(), n * sizeof(double));
+    delete full;
+    return st;
+}
