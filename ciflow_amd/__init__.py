@@ -132,6 +132,12 @@ def lib():
         "cf_rci_result": (C.c_int, [vp, dp, dp, C.POINTER(C.c_int)]),
         "cf_rci_destroy": (C.c_int, [vp]),
         "cf_rdm": (C.c_int, [vp, dp, dp, dp]),
+        "cf_crs_set_spmv_kernel": (C.c_int, [vp, C.c_int, C.c_int]),
+        "cf_rotate_integrals": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp, dp]),
+        "cf_ham_rotate": (C.c_int, [vp, dp, C.POINTER(vp)]),
+        "cf_ham_get": (C.c_int, [vp, dp, dp]),
+        "cf_expm_skew": (C.c_int, [C.c_int, dp, dp]),
+        "cf_gen_rotation": (C.c_int, [C.c_int, C.c_uint64, C.c_double, dp]),
         "cf_rdm_dense": (C.c_int, [vp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
@@ -158,6 +164,35 @@ def gen_integrals(L: int, seed: int):
     V = np.zeros((L, L, L, L))
     _check(lib().cf_gen_integrals(L, seed, _d(h), _d(V)))
     return h, V
+
+
+def expm_skew(K) -> np.ndarray:
+    """U = exp(K) for antisymmetric K (UnitaryMatrix::updateUnitary)."""
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    U = np.zeros_like(K)
+    _check(lib().cf_expm_skew(K.shape[0], _d(K), _d(U)))
+    return U
+
+
+def gen_rotation(L: int, seed: int, sigma: float = 0.05) -> np.ndarray:
+    K = np.zeros((L, L))
+    _check(lib().cf_gen_rotation(L, seed, sigma, _d(K)))
+    return K
+
+
+def rotate_integrals(h, V, U, ctx: "Context | None" = None):
+    """Batched OrbitalTransform::fillHamCI on the GPU: h[b,L,L], V[b,L,L,L,L], U[b,L,L]
+    (a single Hamiltonian without the batch axis is accepted too)."""
+    ctx = ctx or default_context()
+    single = np.ndim(h) == 2
+    h = np.ascontiguousarray(h[None] if single else h, dtype=np.float64)
+    V = np.ascontiguousarray(V[None] if single else V, dtype=np.float64)
+    U = np.ascontiguousarray(U[None] if single else U, dtype=np.float64)
+    B, L = h.shape[0], h.shape[1]
+    ho = np.zeros_like(h)
+    Vo = np.zeros_like(V)
+    _check(lib().cf_rotate_integrals(ctx.ptr, L, B, _d(h), _d(V), _d(U), _d(ho), _d(Vo)))
+    return (ho[0], Vo[0]) if single else (ho, Vo)
 
 
 class Context:
@@ -226,6 +261,25 @@ class Hamiltonian:
     @property
     def ptr(self):
         return self.handle.ptr
+
+    @classmethod
+    def _adopt(cls, handle, L, nup, ndown, econst, ctx):
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.handle = handle
+        self.L, self.nup, self.ndown, self.econst = L, nup, ndown, econst
+        self.h = np.zeros((L, L))
+        self.V = np.zeros((L, L, L, L))
+        _check(lib().cf_ham_get(handle.ptr, _d(self.h), _d(self.V)))
+        return self
+
+    def rotate(self, U) -> "Hamiltonian":
+        """OrbitalTransform::fillHamCI on the GPU: a new Hamiltonian with
+        h' = U h U^T, V'_abcd = sum U_ai U_bj U_ck U_dl V_ijkl."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        hd = _Handle("cf_ham_destroy", self.ctx.h)
+        _check(lib().cf_ham_rotate(self.ptr, _d(U), C.byref(hd.ptr)))
+        return Hamiltonian._adopt(hd, self.L, self.nup, self.ndown, self.econst, self.ctx)
 
     @classmethod
     def synthetic(cls, L: int, nup: int, ndown: int, seed: int, ctx: Context | None = None):
@@ -298,6 +352,10 @@ class SparseMatrix_CRS_Sym:
 
     def export_full(self, r0: int = 0, r1: int | None = None):
         return self._export(False, r0, self.getn() if r1 is None else r1)
+
+    def set_spmv_kernel(self, kind: int, tile_rows: int = 0):
+        """0 automatic, 1 plain row gather, 2 FCI alpha-block staged."""
+        _check(lib().cf_crs_set_spmv_kernel(self.ptr, kind, tile_rows))
 
     def mvprod(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64)

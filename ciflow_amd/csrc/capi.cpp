@@ -14,6 +14,8 @@
 #include "kernels.h"
 #include "lanczos.h"
 #include "rdm.h"
+#include "spmv_fci.h"
+#include "transform.h"
 
 namespace cf {
 void gen_integrals(int L, uint64_t seed, double* h, double* V);
@@ -157,6 +159,8 @@ struct cf_crs {
     DBuf<double> val;
     DBuf<float> val32;
     int lpr = 64;
+    int kernel = 0;  // 0 auto, 1 row gather, 2 FCI staged
+    std::unique_ptr<FciSpmvPlan> plan;
 };
 
 namespace {
@@ -192,22 +196,40 @@ void ensure_staging(cf_ctx* c, uint64_t n) {
 
 void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
     cf_ctx* c = m->sp->ham->ctx;
+    const bool staged = m->kernel == 2 && m->plan;
     if (prec == CF_F32) {
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
-        launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->lpr, c->stream);
+        if (staged) launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream);
+        else launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->lpr, c->stream);
     } else {
         KTimer t(c, "spmv_f64");
-        launch_spmv_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->lpr, c->stream);
+        if (staged) launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream);
+        else launch_spmv_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->lpr, c->stream);
     }
 }
 
-// Average row length decides how many lanes share a row.
+// Staged FCI SpMV plan (alpha-block structure of the rows).
+void make_fci_plan(cf_crs* m, int W) {
+    cf_space* sp = m->sp;
+    const int Ea = sp->ta->E, Sb = sp->tb->S, Eb = sp->tb->E;
+    const int nsa = sp->ta->S - 1, nda = Ea - 1 - nsa;
+    const int R = Eb + nsa * Sb + nda;
+    std::unique_ptr<FciSpmvPlan> p(new FciSpmvPlan());
+    p->init(sp->da, sp->db, Ea, sp->ta->full.p, sp->segoff.p, R, nda, Sb, W, sp->ham->ctx->stream);
+    p->a.rowptr = m->rowptr.p;
+    p->a.col = m->col.p;
+    m->plan = std::move(p);
+}
+
+// Average row length decides how many lanes share a row (measured on MI355X:
+// long FCI rows run best with 4 rows per wavefront, whose alpha-double
+// gathers then coalesce; DOCI rows of ~150 entries with 2 rows per wavefront).
 int pick_lanes_per_row(uint64_t n, uint64_t nnz) {
     double avg = n ? (double)nnz / (double)n : 0.0;
-    if (avg >= 256) return 64;
-    if (avg >= 96) return 32;
-    if (avg >= 32) return 16;
+    if (avg >= 512) return 16;
+    if (avg >= 128) return 32;
+    if (avg >= 24) return 16;
     return 8;
 }
 
@@ -791,6 +813,21 @@ int cf_mvprod(cf_crs* m, const double* x, double* y) {
     API_END
 }
 
+int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
+    API_BEGIN
+    CF_REQUIRE(m, "matrix is NULL");
+    CF_REQUIRE(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    m->sp->ham->ctx->use();
+    if (kind == 2) {
+        CF_REQUIRE(m->sp->kind == CF_FCI, "the staged SpMV needs an FCI matrix");
+        const int W = tile_rows > 0 ? tile_rows : 16;
+        make_fci_plan(m, W);
+        CF_HIP(hipStreamSynchronize(m->sp->ham->ctx->stream));
+    }
+    m->kernel = kind;
+    API_END
+}
+
 int cf_mvprod_dev(cf_crs* m, int value_prec, const double* dx, double* dy) {
     API_BEGIN
     CF_REQUIRE(m && dx && dy, "NULL argument");
@@ -915,6 +952,83 @@ int cf_rci_result(cf_rci* r, double* eigval, double* eigvec, int* info) {
 int cf_rci_destroy(cf_rci* r) {
     API_BEGIN
     delete r;
+    API_END
+}
+
+int cf_expm_skew(int L, const double* K, double* U) {
+    API_BEGIN
+    CF_REQUIRE(L >= 1 && L <= kMaxL && K && U, "bad arguments");
+    expm_skew(L, K, U);
+    API_END
+}
+
+int cf_gen_rotation(int L, uint64_t seed, double sigma, double* K) {
+    API_BEGIN
+    CF_REQUIRE(L >= 1 && L <= kMaxL && K, "bad arguments");
+    gen_rotation(L, seed, sigma, K);
+    API_END
+}
+
+int cf_rotate_integrals(cf_ctx* ctx, int L, int batch, const double* h, const double* V, const double* U,
+                        double* h_out, double* V_out) {
+    API_BEGIN
+    CF_REQUIRE(ctx && h && V && U && h_out && V_out, "NULL argument");
+    CF_REQUIRE(L >= 1 && L <= kMaxL && batch >= 1, "bad L / batch");
+    ctx->use();
+    const size_t L2 = (size_t)L * L, L4 = L2 * L2;
+    DBuf<double> dh(batch * L2), dV(batch * L4), dU(batch * L2), oh(batch * L2), oV(batch * L4), tmp(batch * L4);
+    CF_HIP(hipMemcpyAsync(dh.p, h, dh.bytes(), hipMemcpyHostToDevice, ctx->stream));
+    CF_HIP(hipMemcpyAsync(dV.p, V, dV.bytes(), hipMemcpyHostToDevice, ctx->stream));
+    CF_HIP(hipMemcpyAsync(dU.p, U, dU.bytes(), hipMemcpyHostToDevice, ctx->stream));
+    {
+        KTimer t(ctx, "rotate");
+        rotate_integrals_dev(L, batch, dh.p, dV.p, dU.p, oh.p, oV.p, tmp.p, ctx->stream);
+    }
+    CF_HIP(hipMemcpyAsync(h_out, oh.p, oh.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    CF_HIP(hipMemcpyAsync(V_out, oV.p, oV.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int cf_ham_rotate(const cf_ham* src, const double* U, cf_ham** out) {
+    API_BEGIN
+    CF_REQUIRE(src && U && out, "NULL argument");
+    cf_ctx* ctx = src->ctx;
+    ctx->use();
+    const int L = src->L;
+    const size_t L2 = (size_t)L * L, L3 = L2 * L, L4 = L3 * L;
+    std::unique_ptr<cf_ham> H(new cf_ham());
+    H->ctx = ctx;
+    H->L = L;
+    H->nup = src->nup;
+    H->ndown = src->ndown;
+    H->econst = src->econst;
+    H->dh.alloc(L2);
+    H->dV.alloc(L4);
+    H->dW.alloc(L3);
+    H->dJm.alloc(L2);
+    DBuf<double> dU(L2), tmp(L4);
+    CF_HIP(hipMemcpyAsync(dU.p, U, L2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    {
+        KTimer t(ctx, "rotate");
+        rotate_integrals_dev(L, 1, src->dh.p, src->dV.p, dU.p, H->dh.p, H->dV.p, tmp.p, ctx->stream);
+    }
+    launch_derived(L, H->dV.p, H->dW.p, H->dJm.p, ctx->stream);
+    H->h.resize(L2);
+    H->V.resize(L4);
+    CF_HIP(hipMemcpyAsync(H->h.data(), H->dh.p, L2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CF_HIP(hipMemcpyAsync(H->V.data(), H->dV.p, L4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    *out = H.release();
+    API_END
+}
+
+int cf_ham_get(const cf_ham* ham, double* h, double* V) {
+    API_BEGIN
+    CF_REQUIRE(ham, "ham is NULL");
+    const size_t L2 = (size_t)ham->L * ham->L;
+    if (h) std::memcpy(h, ham->h.data(), L2 * sizeof(double));
+    if (V) std::memcpy(V, ham->V.data(), L2 * L2 * sizeof(double));
     API_END
 }
 

@@ -82,3 +82,62 @@ void gen_integrals(int L, uint64_t seed, double* h, double* V) {
 }
 
 }  // namespace cf
+
+namespace cf {
+
+// Antisymmetric generator of an orbital rotation: K_ij ~ N(0, sigma^2) for
+// i < j drawn row-major from the splitmix64/Box-Muller stream seeded with
+// seed ^ 0x6A09E667F3BCC909, K_ji = -K_ij, K_ii = 0.
+void gen_rotation(int L, uint64_t seed, double sigma, double* K) {
+    SplitMix g(seed ^ 0x6A09E667F3BCC909ULL);
+    for (int i = 0; i < L * L; ++i) K[i] = 0.0;
+    for (int i = 0; i < L; ++i)
+        for (int j = i + 1; j < L; ++j) {
+            const double x = sigma * g.normal();
+            K[i * L + j] = x;
+            K[j * L + i] = -x;
+        }
+}
+
+// U = exp(K) by scaling and squaring with a degree-18 Taylor polynomial
+// (Moler & Van Loan, "Nineteen dubious ways", method 3): ||K/2^s||_1 <= 1/4,
+// truncation error < (1/4)^19/19! ~ 1e-28.  The reference builds exp(X) from
+// the eigenvectors of X^2 and 2x2 rotation blocks (UnitaryMatrix.cpp:229-340);
+// both give the same orthogonal matrix to rounding.
+void expm_skew(int L, const double* K, double* U) {
+    double nrm = 0.0;
+    for (int j = 0; j < L; ++j) {
+        double c = 0.0;
+        for (int i = 0; i < L; ++i) c += std::fabs(K[i * L + j]);
+        nrm = nrm > c ? nrm : c;
+    }
+    int s = 0;
+    while (nrm > 0.25) {
+        nrm *= 0.5;
+        ++s;
+    }
+    const double scale = std::ldexp(1.0, -s);
+    std::vector<double> A((size_t)L * L), T((size_t)L * L), R((size_t)L * L), tmp((size_t)L * L);
+    for (int i = 0; i < L * L; ++i) A[i] = K[i] * scale;
+    for (int i = 0; i < L * L; ++i) R[i] = T[i] = (i % (L + 1) == 0) ? 1.0 : 0.0;
+    auto mul = [L](const std::vector<double>& X, const std::vector<double>& Y, std::vector<double>& Z) {
+        for (int i = 0; i < L; ++i)
+            for (int j = 0; j < L; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < L; ++k) acc += X[i * L + k] * Y[k * L + j];
+                Z[i * L + j] = acc;
+            }
+    };
+    for (int k = 1; k <= 18; ++k) {
+        mul(T, A, tmp);
+        for (int i = 0; i < L * L; ++i) T[i] = tmp[i] / (double)k;
+        for (int i = 0; i < L * L; ++i) R[i] += T[i];
+    }
+    for (int q = 0; q < s; ++q) {
+        mul(R, R, tmp);
+        R.swap(tmp);
+    }
+    for (int i = 0; i < L * L; ++i) U[i] = R[i];
+}
+
+}  // namespace cf

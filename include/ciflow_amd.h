@@ -73,6 +73,25 @@ int cf_ham_destroy(cf_ham* ham);
  * B^P_ij B^P_kl over 2L symmetric B^P ~ N(0,0.1^2) with B^P_PP += 0.6 (P<L). */
 int cf_gen_integrals(int L, uint64_t seed, double* h, double* V);
 
+/* ---- orbital rotation of the integrals: OrbitalTransform::fillHamCI
+ *      (OrbitalTransform.cpp:64-238), h' = U h U^T and
+ *      V'_abcd = sum U_ai U_bj U_ck U_dl V_ijkl, with U[a*L+i] = U_ai (new
+ *      orbital a in old orbital i; the reference's column-major Umx(a,i)),
+ *      four cyclic quarter transforms on FP64 MFMA. */
+/* Batched, host arrays: h[batch][L*L], V[batch][L^4], U[batch][L*L]. */
+int cf_rotate_integrals(cf_ctx* ctx, int L, int batch, const double* h, const double* V, const double* U,
+                        double* h_out, double* V_out);
+/* Rotated copy of a Hamiltonian kept on the GPU (Econst unchanged). */
+int cf_ham_rotate(const cf_ham* src, const double* U, cf_ham** out);
+/* Read a Hamiltonian's integrals back (h: L*L, V: L^4; either may be NULL). */
+int cf_ham_get(const cf_ham* ham, double* h, double* V);
+/* U = exp(K) for antisymmetric K (UnitaryMatrix::updateUnitary,
+ * UnitaryMatrix.cpp:229-340); host arrays L*L. */
+int cf_expm_skew(int L, const double* K, double* U);
+/* Seeded antisymmetric K with K_ij ~ N(0, sigma^2) (i < j), host array L*L.
+ * No reference counterpart: stands in for the optimisers' rotation steps. */
+int cf_gen_rotation(int L, uint64_t seed, double sigma, double* K);
+
 /* ---- determinant space: Permutator_Bit::get_start_int/permutate_bit
  *      (Permutator.cpp:113-153) and vertex-weight ranking determine_weight
  *      (CIMethod.cpp:663-747).  Determinant order: alpha rank slow, beta
@@ -110,6 +129,9 @@ int cf_crs_export_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t
  *      arpackDiagonalize calls (:446).  y is overwritten. */
 int cf_mvprod(cf_crs* m, const double* x, double* y);                       /* host pointers, synchronous */
 int cf_mvprod_dev(cf_crs* m, int value_prec, const double* dx, double* dy);  /* device pointers, ctx stream */
+/* SpMV kernel selection (no reference counterpart): 0 = automatic, 1 = plain
+ * row gather, 2 = FCI alpha-block staged (FCI matrices only). */
+int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
 
 /* ---- ground state: CIMethod::solve -> SparseMatrix_CRS::arpackDiagonalize
  *      (CIMethod.cpp:241-268, SparseMatrix_CRS.cpp:360-481), ARPACK's

@@ -1118,3 +1118,62 @@ extern "C" int or_ground_state(int kind, int L, int nup, int ndown, const double
     delete full;
     return st;
 }
+
+// OrbitalTransform::fillHamCI / rotate_old_to_new (OrbitalTransform.cpp:64-136,
+// 182-200) for one C1 block: h' = U h U^T, V' by four quarter transforms in the
+// reference's order (first index, last index, third, second), U[a*L+i] = U(a,i).
+extern "C" void or_rotate_integrals(int L, const double* U, const double* h, const double* V, double* ho,
+                                    double* Vo) {
+    const size_t L2 = (size_t)L * L, L3 = L2 * L, L4 = L3 * L;
+    std::vector<double> t(L2, 0.0);
+    for (int a = 0; a < L; ++a)
+        for (int j = 0; j < L; ++j) {
+            double s = 0;
+            for (int i = 0; i < L; ++i) s += U[a * L + i] * h[i * L + j];
+            t[a * L + j] = s;
+        }
+    for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b) {
+            double s = 0;
+            for (int j = 0; j < L; ++j) s += t[a * L + j] * U[b * L + j];
+            ho[a * L + b] = s;
+        }
+    std::vector<double> A(V, V + L4), B(L4, 0.0);
+    auto idx = [&](int i, int j, int k, int l) { return (((size_t)i * L + j) * L + k) * L + l; };
+    // first index: (ijkl) -> (ajkl)
+    for (int a = 0; a < L; ++a)
+        for (int j = 0; j < L; ++j)
+            for (int k = 0; k < L; ++k)
+                for (int l = 0; l < L; ++l) {
+                    double s = 0;
+                    for (int i = 0; i < L; ++i) s += U[a * L + i] * A[idx(i, j, k, l)];
+                    B[idx(a, j, k, l)] = s;
+                }
+    // last index: (ajkl) -> (ajkd)
+    for (int a = 0; a < L; ++a)
+        for (int j = 0; j < L; ++j)
+            for (int k = 0; k < L; ++k)
+                for (int d = 0; d < L; ++d) {
+                    double s = 0;
+                    for (int l = 0; l < L; ++l) s += B[idx(a, j, k, l)] * U[d * L + l];
+                    A[idx(a, j, k, d)] = s;
+                }
+    // third index: (ajkd) -> (ajcd)
+    for (int a = 0; a < L; ++a)
+        for (int j = 0; j < L; ++j)
+            for (int c = 0; c < L; ++c)
+                for (int d = 0; d < L; ++d) {
+                    double s = 0;
+                    for (int k = 0; k < L; ++k) s += A[idx(a, j, k, d)] * U[c * L + k];
+                    B[idx(a, j, c, d)] = s;
+                }
+    // second index: (ajcd) -> (abcd)
+    for (int a = 0; a < L; ++a)
+        for (int b = 0; b < L; ++b)
+            for (int c = 0; c < L; ++c)
+                for (int d = 0; d < L; ++d) {
+                    double s = 0;
+                    for (int j = 0; j < L; ++j) s += B[idx(a, j, c, d)] * U[b * L + j];
+                    Vo[idx(a, b, c, d)] = s;
+                }
+}

@@ -75,6 +75,7 @@ def lib():
         L.or_dens_energy.argtypes = [C.c_int, dp, dp, C.c_double, dp, dp]
         L.or_dens_energy.restype = C.c_double
         L.or_num_threads.restype = C.c_int
+        L.or_rotate_integrals.argtypes = [C.c_int, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -241,3 +242,15 @@ def dens_energy(h, V, econst, one, two):
 
 def num_threads() -> int:
     return int(lib().or_num_threads())
+
+
+def rotate_integrals(U, h, V):
+    """OrbitalTransform::fillHamCI restated (quarter transforms, C1)."""
+    L = h.shape[0]
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    ho = np.zeros_like(h)
+    Vo = np.zeros_like(V)
+    lib().or_rotate_integrals(L, _d(U), _d(h), _d(V), _d(ho), _d(Vo))
+    return ho, Vo

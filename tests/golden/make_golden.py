@@ -9,8 +9,12 @@ Contents:
               sign bits, sum |H_ij|, and the ground-state energy from a dense
               numpy eigh of the oracle's CRS.
   beh2        FCI and DOCI energies of the reference fixture (eigh).
+  config2     (--slow, ~10 min on 8 cores) FCI L=12 6a6b seed 1, 853,776
+              determinants: the oracle's full build_parallel + Lanczos energy.
+  config3     (--slow) DOCI L=24 12 pairs seed 2, 2,704,156 determinants.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--slow]
+(without --slow the slow entries already in golden.json are kept)
 """
 from __future__ import annotations
 
@@ -23,6 +27,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import ctypes as C  # noqa: E402
 
 from oracle import oracle as O  # noqa: E402
 
@@ -47,7 +53,21 @@ def crs_digest(m):
     }
 
 
+def ground_state(kind: int, L: int, nel: int, seed: int):
+    h, V = O.gen_integrals(L, seed)
+    lib = O.lib()
+    lib.or_ground_state.argtypes = [C.c_int] * 4 + [O.dp, O.dp, C.c_double, C.c_int, C.c_int, C.c_double, O.dp,
+                                                    C.POINTER(C.c_int), O.u64p, O.dp]
+    E, nmv, nnz = C.c_double(), C.c_int(), C.c_uint64()
+    st = lib.or_ground_state(kind, L, nel, nel, h.ctypes.data_as(O.dp), V.ctypes.data_as(O.dp), 1e-14,
+                             O.num_threads(), 42, 1e-12, C.byref(E), C.byref(nmv), C.byref(nnz), None)
+    assert st == 0
+    return {"energy": E.value, "n_matvec": nmv.value, "nnz_upper": int(nnz.value)}
+
+
 def main():
+    path = os.path.join(HERE, "golden.json")
+    old = json.load(open(path)) if os.path.exists(path) else {}
     out = {"integrals": {}}
     for L, seed in INTEGRALS:
         h, V = O.gen_integrals(L, seed)
@@ -69,7 +89,14 @@ def main():
         "fci_crs": crs_digest(mf),
         "doci_crs": crs_digest(md),
     }
-    with open(os.path.join(HERE, "golden.json"), "w") as f:
+    if "--slow" in sys.argv:
+        out["config2"] = ground_state(0, 12, 6, 1)
+        out["config3"] = ground_state(1, 24, 12, 2)
+    else:
+        for k in ("config2", "config3"):
+            if k in old:
+                out[k] = old[k]
+    with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(json.dumps(out, indent=1))
 

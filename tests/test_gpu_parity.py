@@ -254,3 +254,80 @@ def test_config3_doci_L24_rows_bit_exact():
         ref = O.build_crs("DOCI", h, V, 12, 12, start=r0, end=r0 + 300)
         row, col, val = cim.get_mat().export_upper(r0, r0 + 300)
         assert np.array_equal(row, ref.row) and np.array_equal(col, ref.col) and np.array_equal(val, ref.data)
+
+
+# ---- (e) integral rotation on FP64 MFMA ---------------------------------------
+@pytest.mark.parametrize("L", [4, 7, 12, 20])
+def test_rotate_integrals_match_oracle(L):
+    say("rotate", L)
+    B = 3
+    hs, Vs, Us = [], [], []
+    for b in range(B):
+        h, V = CA.gen_integrals(L, 100 + b)
+        hs.append(h)
+        Vs.append(V)
+        Us.append(CA.expm_skew(CA.gen_rotation(L, 1000 + b, 0.05 if b else 0.8)))
+    ho, Vo = CA.rotate_integrals(np.array(hs), np.array(Vs), np.array(Us))
+    for b in range(B):
+        rh, rV = O.rotate_integrals(Us[b], hs[b], Vs[b])
+        assert np.max(np.abs(ho[b] - rh)) < 1e-13
+        assert np.max(np.abs(Vo[b] - rV)) < 1e-13 * max(1.0, np.max(np.abs(rV)))
+
+
+def test_rotation_round_trip_and_fci_invariance():
+    """TestTransformations.cpp:120 (U then U^-1 restores the integrals) and
+    E_FCI(U . ints) = E_FCI(ints) through the GPU build and solve."""
+    g = golden()["config1"]
+    h, V = CA.gen_integrals(8, 0)
+    ham = CA.Hamiltonian(h, V, 4, 4)
+    U = CA.expm_skew(CA.gen_rotation(8, 7, 0.3))
+    hr = ham.rotate(U)
+    back = hr.rotate(U.T)
+    assert np.max(np.abs(back.h - h)) < 1e-13 and np.max(np.abs(back.V - V)) < 1e-13
+    cim = CA.FCI(hr)
+    cim.solve()
+    assert abs(cim.get_ci_energy() - g["energy"]) < 1e-10
+
+
+# ---- (c) staged FCI SpMV ------------------------------------------------------
+@pytest.mark.parametrize("L,n,W", [(8, 4, 16), (10, 5, 8), (11, 5, 16), (12, 6, 16), (12, 6, 32)])
+def test_staged_fci_spmv_matches_row_gather(L, n, W):
+    say("staged spmv", L, n, W)
+    h, V = CA.gen_integrals(L, 1)
+    cim = CA.FCI(CA.Hamiltonian(h, V, n, n))
+    m = cim.get_mat()
+    x = np.random.default_rng(2).standard_normal(cim.get_dim())
+    m.set_spmv_kernel(1)
+    y1 = m.mvprod(x)
+    m.set_spmv_kernel(2, W)
+    y2 = m.mvprod(x)
+    assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1))
+
+
+def test_staged_spmv_falls_back_on_dropped_entries():
+    """D2h fixture: symmetry zeros are dropped, rows are shorter than the
+    template, the staged kernel must take its plain-gather path."""
+    h, V, na, nb, ec = fixture("beh2_sto3g.npz")
+    cim = CA.FCI(CA.Hamiltonian(h, V, na, nb, ec))
+    m = cim.get_mat()
+    x = np.random.default_rng(3).standard_normal(cim.get_dim())
+    m.set_spmv_kernel(1)
+    y1 = m.mvprod(x)
+    m.set_spmv_kernel(2, 8)
+    y2 = m.mvprod(x)
+    assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1))
+    cim.solve()
+    assert abs(cim.get_ci_energy() - golden()["beh2"]["fci_energy"]) < 1e-10
+
+
+def test_config2_energy_matches_oracle():
+    """FCI L=12 6a6b seed 1: GPU solve vs the oracle's full CPU solve (golden)."""
+    g = golden()["config2"]
+    h, V = CA.gen_integrals(12, 1)
+    cim = CA.FCI(CA.Hamiltonian(h, V, 6, 6), value_prec=CA.CF_F32)
+    info = cim.solve()
+    say("config2 f64", info.as_dict())
+    assert abs(cim.get_ci_energy() - g["energy"]) < 1e-10
+    info = cim.solve(precision=CA.CF_F32)
+    say("config2 f32+polish", info.as_dict())
+    assert abs(cim.get_ci_energy() - g["energy"]) < 1e-8

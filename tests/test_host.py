@@ -79,3 +79,23 @@ def test_arguments_rejected_with_status_codes():
     assert lib.cf_rci_create(0, 10, 1e-12, 0, 1, CA.C.byref(p)) == CA.CF_ERR_ARG
     assert lib.cf_gen_integrals(0, 1, None, None) == CA.CF_ERR_ARG
     assert lib.cf_ctx_create(0, None) != CA.CF_OK
+
+
+# --- orbital rotation helpers (host side) ---------------------------------
+def test_expm_skew_matches_scipy_and_is_orthogonal():
+    import scipy.linalg as sl
+    for L, seed in [(2, 1), (8, 2), (20, 3)]:
+        K = CA.gen_rotation(L, seed, 0.05)
+        assert np.array_equal(K, -K.T) and np.all(np.diag(K) == 0)
+        U = CA.expm_skew(K)
+        assert np.max(np.abs(U - sl.expm(K))) < 1e-14
+        assert np.max(np.abs(U @ U.T - np.eye(L))) < 1e-14
+        assert abs(np.linalg.det(U) - 1.0) < 1e-13
+        assert np.max(np.abs(CA.expm_skew(-K) @ U - np.eye(L))) < 1e-14
+    # 2x2 closed form: exp([[0, t], [-t, 0]]) = [[cos t, sin t], [-sin t, cos t]]
+    t = 0.7
+    U = CA.expm_skew(np.array([[0.0, t], [-t, 0.0]]))
+    assert np.max(np.abs(U - np.array([[np.cos(t), np.sin(t)], [-np.sin(t), np.cos(t)]]))) < 1e-15
+    # a large rotation exercises the squaring phase
+    K = CA.gen_rotation(6, 9, 1.5)
+    assert np.max(np.abs(CA.expm_skew(K) - sl.expm(K))) < 1e-12

@@ -224,3 +224,25 @@ def test_lanczos_matches_eigh(L, na, seed):
     assert info["status"] == 0
     assert abs(E - w[0]) < 1e-11
     assert abs(abs(x @ U[:, 0]) - 1) < 1e-9
+
+
+# --- a15: integral rotation ---------------------------------------------------
+def test_rotation_oracle_matches_einsum_and_round_trips():
+    import scipy.linalg as sl
+    L = 6
+    h, V = O.gen_integrals(L, 7)
+    rng = np.random.default_rng(1)
+    K = rng.normal(0, 0.3, (L, L))
+    K = K - K.T
+    U = sl.expm(K)
+    ho, Vo = O.rotate_integrals(U, h, V)
+    assert np.max(np.abs(ho - U @ h @ U.T)) < 1e-13
+    ref = np.einsum("ai,bj,ck,dl,ijkl->abcd", U, U, U, U, V, optimize=True)
+    assert np.max(np.abs(Vo - ref)) < 1e-13
+    # TestTransformations.cpp:120: U then U^-1 gives back the integrals
+    h2, V2 = O.rotate_integrals(U.T, ho, Vo)
+    assert np.max(np.abs(h2 - h)) < 1e-13 and np.max(np.abs(V2 - V)) < 1e-13
+    # FCI energy is invariant under orbital rotation
+    e0 = np.linalg.eigvalsh(O.dense_from_upper(O.build_crs("FCI", h, V, 3, 3)))[0]
+    e1 = np.linalg.eigvalsh(O.dense_from_upper(O.build_crs("FCI", ho, Vo, 3, 3)))[0]
+    assert abs(e0 - e1) < 1e-11
