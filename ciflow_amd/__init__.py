@@ -138,6 +138,13 @@ def lib():
         "cf_ham_get": (C.c_int, [vp, dp, dp]),
         "cf_expm_skew": (C.c_int, [C.c_int, dp, dp]),
         "cf_gen_rotation": (C.c_int, [C.c_int, C.c_uint64, C.c_double, dp]),
+        "cf_sigma_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+        "cf_sigma_destroy": (C.c_int, [vp]),
+        "cf_sigma_apply_dev": (C.c_int, [vp, C.c_int, vp, vp]),
+        "cf_sigma_apply": (C.c_int, [vp, C.c_int, dp, dp]),
+        "cf_solve_direct": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
+        "cf_rdm1_direct": (C.c_int, [vp, dp, dp]),
+        "cf_space_eigvec": (C.c_int, [vp, dp]),
         "cf_rdm_dense": (C.c_int, [vp, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
@@ -510,6 +517,53 @@ class DOCI(CIMethod):
 
     def get_name(self):
         return "DOCI"
+
+
+class FCI_Direct(CIMethod):
+    """FCI with the on-the-fly sigma instead of a stored matrix -- the role of
+    the reference's CI_Big (CIMethod.h:176-189, CIMethod.cpp:943-1116), for
+    spaces too large to store."""
+
+    kind = CF_FCI
+
+    def __init__(self, ham: Hamiltonian, value_prec: int = CF_F64):
+        super().__init__(ham, value_prec=value_prec, build=False)
+        self.sig = _Handle("cf_sigma_destroy", self.space)
+        _check(lib().cf_sigma_create(self.ptr, value_prec, C.byref(self.sig.ptr)))
+
+    def get_name(self):
+        return "FCI_Direct"
+
+    def build_ci(self):
+        raise CIFlowError(CF_ERR_STATE, "FCI_Direct stores no matrix")
+
+    def mvprod(self, x, precision: int = CF_F64):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib().cf_sigma_apply(self.sig.ptr, precision, _d(x), _d(y)))
+        return y
+
+    def solve(self, precision: int = CF_F64, want_vec: bool = True, **kw):
+        opts = solve_opts(precision=precision, **kw)
+        info = SolveInfo()
+        vec = np.zeros(self.n) if want_vec else None
+        _check(lib().cf_solve_direct(self.sig.ptr, C.byref(opts), _d(vec) if want_vec else None, C.byref(info)))
+        self._info, self._eigvec, self._eigval = info, vec, info.eigval
+        self._one = self._two = None
+        return info
+
+    def construct_density(self, trdm: bool = False):
+        if trdm:
+            raise CIFlowError(CF_ERR_UNSUPPORTED, "FCI_Direct computes the 1-RDM only")
+        L = self.ham.L
+        one = np.zeros(L * (L + 1))
+        c = self._eigvec
+        if c is None:
+            c = np.zeros(self.n)
+            _check(lib().cf_space_eigvec(self.ptr, _d(c)))
+        _check(lib().cf_rdm1_direct(self.ptr, _d(np.ascontiguousarray(c)), _d(one)))
+        self._one = one
+        return one, None
 
 
 class RCILanczos:

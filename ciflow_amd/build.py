@@ -25,7 +25,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-u
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-SOURCES = ["strings.hip", "build.hip", "spmv.hip", "spmv_fci.hip", "krylov.hip", "rdm.hip", "transform.hip",
+SOURCES = ["strings.hip", "build.hip", "spmv.hip", "spmv_fci.hip", "krylov.hip", "rdm.hip", "transform.hip", "sigma.hip",
            "lanczos.cpp", "integrals.cpp", "capi.cpp"]
 
 
@@ -51,7 +51,8 @@ def build(verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lrocblas",
+                                                                                     "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

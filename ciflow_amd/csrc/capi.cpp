@@ -14,7 +14,10 @@
 #include "kernels.h"
 #include "lanczos.h"
 #include "rdm.h"
+#include "sigma.h"
 #include "spmv_fci.h"
+
+#include <rocblas/rocblas.h>
 #include "transform.h"
 
 namespace cf {
@@ -846,46 +849,46 @@ void cf_solve_opts_default(cf_solve_opts* o) {
     o->seed = 20240601ULL;
 }
 
-int cf_solve(cf_crs* m, const cf_solve_opts* opts, double* eigvec, cf_solve_info* info) {
-    API_BEGIN
-    CF_REQUIRE(m && info, "NULL argument");
+}  // extern "C"
+
+namespace {
+// Shared driver of cf_solve / cf_solve_direct: FP64 thick-restart Lanczos, or
+// an FP32-operator phase followed by the FP64 phase started from its Ritz
+// vector.  The eigenvector stays resident in sp->vec.
+void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::function<void(const double*, double*)> op64,
+               std::function<void(const double*, double*)> op32, double* eigvec, cf_solve_info* info) {
     cf_solve_opts o;
     if (opts) o = *opts;
     else cf_solve_opts_default(&o);
     CF_REQUIRE(o.ncv >= 2 && o.ncv <= 63, "ncv must be in [2, 63]");
     CF_REQUIRE(o.tol > 0.0, "tol must be > 0");
     CF_REQUIRE(o.precision == CF_F64 || o.precision == CF_F32, "precision must be CF_F64 or CF_F32");
-    cf_ctx* c = m->sp->ham->ctx;
-    c->use();
+    cf_ctx* c = sp->ham->ctx;
     const auto t0 = std::chrono::steady_clock::now();
-    const uint64_t n = m->n;
     const int maxr = o.maxrestart > 0 ? o.maxrestart : (int)std::min<uint64_t>(3 * n, 1u << 30);
     DeviceBackend b(c, n, o.ncv);
     int nmv32 = 0, nrs = 0;
     bool keep = false;
     if (o.precision == CF_F32) {
-        CF_REQUIRE(m->val32.p != nullptr, "FP32 solve needs a matrix built with CF_F32");
-        b.op = [m](const double* x, double* y) { spmv(m, CF_F32, x, y); };
+        CF_REQUIRE((bool)op32, "the FP32 operator is not available (build with CF_F32)");
+        b.op = op32;
         ThickRestartLanczos sm(&b, n, o.ncv, o.tol_f32, maxr, o.seed, false);
         while (sm.step() == ThickRestartLanczos::NEED_APPLY) b.apply(sm.pending());
         nmv32 = sm.n_apply();
         nrs += sm.n_restart();
         keep = true;  // v0 holds the FP32-phase Ritz vector: polish from it
     }
-    b.op = [m](const double* x, double* y) { spmv(m, CF_F64, x, y); };
+    b.op = op64;
     ThickRestartLanczos sm(&b, n, o.ncv, o.tol, maxr, o.seed, keep);
     while (sm.step() == ThickRestartLanczos::NEED_APPLY) b.apply(sm.pending());
     info->eigval = sm.eigval();
-    info->energy = sm.eigval() + m->sp->ham->econst;
+    info->energy = sm.eigval() + sp->ham->econst;
     info->resid = sm.resid();
     info->n_matvec = sm.n_apply();
     info->n_matvec_f32 = nmv32;
     info->n_restart = nrs + sm.n_restart();
     info->status = sm.status();
-    // resident copy of the eigenvector (sign convention applied on export)
-    cf_space* sp = m->sp;
     if (sp->vec.n != n) sp->vec.alloc(n);
-    launch_copy(n, b.v(0), sp->vec.p, c->stream);
     std::vector<double> hv(n);
     CF_HIP(hipMemcpyAsync(hv.data(), b.v(0), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CF_HIP(hipStreamSynchronize(c->stream));
@@ -894,6 +897,234 @@ int cf_solve(cf_crs* m, const cf_solve_opts* opts, double* eigvec, cf_solve_info
     sp->vec_valid = true;
     if (eigvec) std::memcpy(eigvec, hv.data(), n * sizeof(double));
     info->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+}  // namespace
+
+struct cf_sigma {
+    cf_space* sp = nullptr;
+    bool f32 = false;
+    DBuf<double> Ha, Hb_own;
+    DBuf<float> Ha32, Hb32_own;
+    const double* Hb = nullptr;
+    const float* Hb32 = nullptr;
+    DBuf<float> x32, y32;
+    rocblas_handle blas = nullptr;
+    SigmaArgs args;
+    ~cf_sigma() {
+        if (blas) rocblas_destroy_handle(blas);
+    }
+};
+
+namespace {
+#define CF_BLAS(call)                                                                       \
+    do {                                                                                    \
+        rocblas_status st_ = (call);                                                        \
+        if (st_ != rocblas_status_success)                                                  \
+            throw ::cf::Error(::cf::ERR_HIP, std::string(#call) + ": rocblas status " +     \
+                                                 std::to_string((int)st_));                 \
+    } while (0)
+
+// y = Ha C + C Hb^T + sigma_ab.  Row-major C (da x db) is the column-major
+// db x da matrix Cc, so y_c = Cc Ha + Hb Cc (Ha, Hb symmetric).
+void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
+    cf_space* sp = S->sp;
+    cf_ctx* c = sp->ham->ctx;
+    const int m = (int)sp->db, nn = (int)sp->da;
+    KTimer t(c, prec == CF_F32 ? "sigma_f32" : "sigma_f64");
+    if (prec == CF_F32) {
+        CF_REQUIRE(S->f32, "sigma plan was created without FP32 operands");
+        const float one = 1.0f, zero = 0.0f;
+        launch_to_f32(sp->n, x, S->x32.p, c->stream);
+        CF_BLAS(rocblas_sgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, S->x32.p, m,
+                              S->Ha32.p, nn, &zero, S->y32.p, m));
+        CF_BLAS(rocblas_sgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb32, m,
+                              S->x32.p, m, &one, S->y32.p, m));
+        launch_to_f64(sp->n, S->y32.p, y, c->stream);
+        launch_sigma_ab<float>(S->args, S->x32.p, y, c->stream);
+    } else {
+        const double one = 1.0, zero = 0.0;
+        CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, x, m,
+                              S->Ha.p, nn, &zero, y, m));
+        CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb, m, x,
+                              m, &one, y, m));
+        launch_sigma_ab<double>(S->args, x, y, c->stream);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int cf_solve(cf_crs* m, const cf_solve_opts* opts, double* eigvec, cf_solve_info* info) {
+    API_BEGIN
+    CF_REQUIRE(m && info, "NULL argument");
+    m->sp->ham->ctx->use();
+    std::function<void(const double*, double*)> op32;
+    if (m->val32.p) op32 = [m](const double* x, double* y) { spmv(m, CF_F32, x, y); };
+    run_solve(m->sp, m->n, opts, [m](const double* x, double* y) { spmv(m, CF_F64, x, y); }, op32, eigvec, info);
+    API_END
+}
+
+int cf_space_eigvec(cf_space* sp, double* eigvec) {
+    API_BEGIN
+    CF_REQUIRE(sp && eigvec, "NULL argument");
+    CF_REQUIRE(sp->vec_valid, "no solve has run on this space");
+    sp->ham->ctx->use();
+    CF_HIP(hipMemcpy(eigvec, sp->vec.p, sp->n * sizeof(double), hipMemcpyDeviceToHost));
+    API_END
+}
+
+int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
+    API_BEGIN
+    CF_REQUIRE(sp && out, "NULL argument");
+    CF_REQUIRE(sp->kind == CF_FCI, "the on-the-fly sigma is for FCI spaces");
+    CF_REQUIRE(prec == CF_F64 || prec == CF_F32, "prec must be CF_F64 or CF_F32");
+    CF_REQUIRE(sp->ham->L <= 32 && sp->db < (1ull << 21) && sp->tb->S - 1 <= 64,
+               "on-the-fly sigma supports L <= 32, beta dimension < 2^21, <= 64 beta singles");
+    CF_REQUIRE(sp->da <= 2147483647ull && sp->db <= 2147483647ull, "dimension too large for the GEMMs");
+    cf_ctx* c = sp->ham->ctx;
+    c->use();
+    std::unique_ptr<cf_sigma> S(new cf_sigma());
+    S->sp = sp;
+    S->f32 = prec == CF_F32;
+    S->Ha.alloc(sp->da * sp->da);
+    launch_dense_string_ham<double>(sp->da, sp->ta->E, sp->ta->full.p, S->Ha.p, c->stream);
+    if (sp->tb != sp->ta.get()) {
+        S->Hb_own.alloc(sp->db * sp->db);
+        launch_dense_string_ham<double>(sp->db, sp->tb->E, sp->tb->full.p, S->Hb_own.p, c->stream);
+        S->Hb = S->Hb_own.p;
+    } else {
+        S->Hb = S->Ha.p;
+    }
+    if (S->f32) {
+        S->Ha32.alloc(sp->da * sp->da);
+        launch_dense_string_ham<float>(sp->da, sp->ta->E, sp->ta->full.p, S->Ha32.p, c->stream);
+        if (sp->tb != sp->ta.get()) {
+            S->Hb32_own.alloc(sp->db * sp->db);
+            launch_dense_string_ham<float>(sp->db, sp->tb->E, sp->tb->full.p, S->Hb32_own.p, c->stream);
+            S->Hb32 = S->Hb32_own.p;
+        } else {
+            S->Hb32 = S->Ha32.p;
+        }
+        S->x32.alloc(sp->n);
+        S->y32.alloc(sp->n);
+    }
+    CF_BLAS(rocblas_create_handle(&S->blas));
+    CF_BLAS(rocblas_set_stream(S->blas, c->stream));
+    SigmaArgs& a = S->args;
+    a.da = sp->da;
+    a.db = sp->db;
+    a.L = sp->ham->L;
+    a.Sa = sp->ta->S;
+    a.Sb = sp->tb->S;
+    a.nchunk = (uint32_t)((sp->db + 1023) / 1024);
+    const uint64_t nb = sp->da * a.nchunk;
+    CF_REQUIRE(nb < (1ull << 31), "too many sigma workgroups");
+    a.nblocks = (uint32_t)nb;
+    a.sa = sp->ta->s.p;
+    a.sb = sp->tb->s.p;
+    a.suba = sp->ta->sub.p;
+    a.subb = sp->tb->sub.p;
+    a.V = sp->ham->dV.p;
+    CF_REQUIRE(sigma_ab_lds_bytes(a, false) <= 160 * 1024, "beta string space too large for the LDS row");
+    CF_HIP(hipStreamSynchronize(c->stream));
+    *out = S.release();
+    API_END
+}
+
+int cf_sigma_destroy(cf_sigma* s) {
+    API_BEGIN
+    if (s) {
+        s->sp->ham->ctx->use();
+        delete s;
+    }
+    API_END
+}
+
+int cf_sigma_apply_dev(cf_sigma* s, int value_prec, const double* dx, double* dy) {
+    API_BEGIN
+    CF_REQUIRE(s && dx && dy, "NULL argument");
+    s->sp->ham->ctx->use();
+    sigma_apply(s, value_prec, dx, dy);
+    API_END
+}
+
+int cf_sigma_apply(cf_sigma* s, int value_prec, const double* x, double* y) {
+    API_BEGIN
+    CF_REQUIRE(s && x && y, "NULL argument");
+    cf_ctx* c = s->sp->ham->ctx;
+    c->use();
+    const uint64_t n = s->sp->n;
+    ensure_staging(c, n);
+    CF_HIP(hipMemcpyAsync(c->hx.p, x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    sigma_apply(s, value_prec, c->hx.p, c->hy.p);
+    CF_HIP(hipMemcpyAsync(y, c->hy.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int cf_solve_direct(cf_sigma* s, const cf_solve_opts* opts, double* eigvec, cf_solve_info* info) {
+    API_BEGIN
+    CF_REQUIRE(s && info, "NULL argument");
+    s->sp->ham->ctx->use();
+    std::function<void(const double*, double*)> op32;
+    if (s->f32) op32 = [s](const double* x, double* y) { sigma_apply(s, CF_F32, x, y); };
+    run_solve(s->sp, s->sp->n, opts, [s](const double* x, double* y) { sigma_apply(s, CF_F64, x, y); }, op32, eigvec,
+              info);
+    API_END
+}
+
+int cf_rdm1_direct(cf_space* sp, const double* cvec, double* one) {
+    API_BEGIN
+    CF_REQUIRE(sp && cvec && one, "NULL argument");
+    CF_REQUIRE(sp->kind == CF_FCI, "cf_rdm1_direct is for FCI spaces");
+    cf_ctx* c = sp->ham->ctx;
+    c->use();
+    const int L = sp->ham->L;
+    const uint64_t n = sp->n, da = sp->da, db = sp->db;
+    DBuf<double> C(n), Ct(n), da_out(da * (uint64_t)sp->ta->S), db_out(db * (uint64_t)sp->tb->S);
+    CF_HIP(hipMemcpyAsync(C.p, cvec, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    {
+        KTimer t(c, "rdm");
+        launch_row_dots(da, db, sp->ta->S, sp->ta->sub.p, C.p, da_out.p, c->stream);
+        launch_transpose(da, db, C.p, Ct.p, c->stream);
+        launch_row_dots(db, da, sp->tb->S, sp->tb->sub.p, Ct.p, db_out.p, c->stream);
+    }
+    std::vector<double> ha(da * (uint64_t)sp->ta->S), hb(db * (uint64_t)sp->tb->S);
+    std::vector<Exc> la(da * (uint64_t)sp->ta->S), lb(db * (uint64_t)sp->tb->S);
+    CF_HIP(hipMemcpyAsync(ha.data(), da_out.p, ha.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipMemcpyAsync(hb.data(), db_out.p, hb.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipMemcpyAsync(la.data(), sp->ta->sub.p, la.size() * sizeof(Exc), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipMemcpyAsync(lb.data(), sp->tb->sub.p, lb.size() * sizeof(Exc), hipMemcpyDeviceToHost, c->stream));
+    std::vector<str_t> sa(da), sb(db);
+    CF_HIP(hipMemcpyAsync(sa.data(), sp->ta->s.p, da * 8, hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipMemcpyAsync(sb.data(), sp->tb->s.p, db * 8, hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipStreamSynchronize(c->stream));
+    // gamma^s_pr = sum over strings K, entries e of K's list: self -> n_p(K) |C_K|^2 on the
+    // diagonal; single p -> r: sign * C_K . C_J at (p, r) (= <a+_p a_r>, symmetric)
+    std::vector<double> g(2 * (size_t)L * L, 0.0);
+    for (int s = 0; s < 2; ++s) {
+        const uint64_t d = s == 0 ? da : db;
+        const int S = s == 0 ? sp->ta->S : sp->tb->S;
+        const std::vector<double>& dots = s == 0 ? ha : hb;
+        const std::vector<Exc>& lst = s == 0 ? la : lb;
+        const std::vector<str_t>& strs = s == 0 ? sa : sb;
+        double* gs = g.data() + (size_t)s * L * L;
+        for (uint64_t K = 0; K < d; ++K)
+            for (int e = 0; e < S; ++e) {
+                const Exc& x = lst[K * S + e];
+                const double v = dots[K * S + e];
+                if (x.type == 0) {
+                    for (str_t t = strs[K]; t; t &= t - 1) gs[__builtin_ctzll(t) * (L + 1)] += v;
+                } else {
+                    gs[x.f * L + x.t] += x.sign < 0 ? -v : v;
+                }
+            }
+    }
+    const size_t osz = (size_t)L * (L + 1) / 2;
+    for (int s = 0; s < 2; ++s)
+        for (int p = 0; p < L; ++p)
+            for (int q = p; q < L; ++q)
+                one[s * osz + (size_t)p * L + q - (size_t)p * (p + 1) / 2] = g[(size_t)s * L * L + p * L + q];
     API_END
 }
 

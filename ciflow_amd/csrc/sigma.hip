@@ -1,0 +1,224 @@
+// On-the-fly FCI sigma = H c without a stored matrix (replaces the O(dim^2)
+// on-the-fly product of CI_Big::mvprod, CIMethod.cpp:943-984, which is
+// infeasible at 1.66e8 determinants).
+//
+// With C viewed as the d_a x d_b matrix C[Ia][Ib] (alpha rank slow, beta rank
+// fast, CIMethod.cpp:771), H splits exactly into
+//   sigma = Ha C + C Hb^T + sigma_ab
+// Ha / Hb: the same-spin string Hamiltonians (one-string diagonal, single
+//   excitations h_pq + sum_z (V_pzqz - V_pzzq) with their phase, same-spin
+//   doubles) -- the entries of the sorted excitation lists, scattered into a
+//   dense d x d matrix once per Hamiltonian.  The two products are plain dense
+//   GEMMs (rocBLAS, FP64 or FP32 MFMA): 2 d^3 flop each.
+// sigma_ab(Ia,Ib) = sum_{a in A(Ia)} sum_{b in A(Ib)} s_a s_b V[p_a][q_b][r_a][s_b] C(J_a, J_b),
+//   A(I) = the single excitations of I (p in I -> r, J = I - p + r, phase
+//   s) plus the number operators (p = r in I, J = I, s = +1): this is
+//   sum_pqrs V_pqrs E^a_pr E^b_qs and carries the alpha-beta Coulomb parts of
+//   the reference's diagonal (jmix) and single elements
+//   (CIMethod.cpp:343-351, 409-427) and its ab doubles (:444-457).
+//   Kernel: a workgroup owns one alpha string Ia and 1024 beta strings (one
+//   per thread, whose singles sit in registers); for each alpha action it
+//   stages the C row J_a and the V slice V[p_a][.][r_a][.] in LDS and every
+//   thread gathers its |A(Ib)| terms from LDS.
+// Algorithmic HBM stream per sigma (L=16 8a8b): the C rows staged per alpha
+// action, d_a * |A_a| * d_b * 8 B = 12,870 * 65 * 12,870 * 8 B = 86 GB (from
+// L2 when the 13 beta chunks of one Ia run together), plus the GEMM operands.
+#include "sigma.h"
+
+namespace cf {
+
+// Dense same-spin string Hamiltonian from the excitation lists.
+template <typename T>
+__global__ void k_dense_string_ham(uint64_t d, int E, const Exc* __restrict__ full, T* __restrict__ H) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d * (uint64_t)E) return;
+    const uint64_t I = t / E;
+    const Exc x = full[t];
+    double v = x.x;
+    if (x.type == 1 && x.sign < 0) v = -v;
+    H[I * d + x.J] = (T)v;
+}
+
+template <typename T>
+void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream_t s) {
+    CF_HIP(hipMemsetAsync(H, 0, d * d * sizeof(T), s));
+    const uint64_t tot = d * (uint64_t)E;
+    k_dense_string_ham<T><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(d, E, full, H);
+    CF_HIP(hipGetLastError());
+}
+template void launch_dense_string_ham<double>(uint64_t, int, const Exc*, double*, hipStream_t);
+template void launch_dense_string_ham<float>(uint64_t, int, const Exc*, float*, hipStream_t);
+
+constexpr int kSigThreads = 1024;
+constexpr int kMaxSingles = 64;  // n (L - n) <= 64
+
+// sigma_ab += ..., CT = double or float (staged C precision); accumulation FP64.
+template <typename CT>
+__global__ void __launch_bounds__(kSigThreads) k_sigma_ab(SigmaArgs a, const CT* __restrict__ C,
+                                                         double* __restrict__ sig) {
+    extern __shared__ unsigned char smem[];
+    const uint64_t db = a.db;
+    const uint32_t nchunk = a.nchunk;
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (lb >= a.nblocks) return;
+    const uint64_t Ia = lb / nchunk;
+    const uint64_t Ib = (uint64_t)(lb - Ia * nchunk) * kSigThreads + threadIdx.x;
+    const bool act = Ib < db;
+    const int L = a.L;
+    double* Vs = (double*)smem;             // L*L
+    CT* row = (CT*)(Vs + L * L);            // db
+    // this thread's beta actions: singles (packed) + number operators (bits of Ib)
+    uint32_t sb[kMaxSingles];
+    const int nsb = a.Sb - 1;
+    str_t sbits = 0;
+    if (act) {
+        sbits = a.sb[Ib];
+        const Exc* lb_ = a.subb + Ib * (uint64_t)a.Sb;
+        int w = 0;
+#pragma unroll
+        for (int e = 0; e < kMaxSingles + 1; ++e) {
+            if (e < a.Sb) {
+                const Exc x = lb_[e];
+                if (x.type == 1 && w < kMaxSingles) {
+                    sb[w] = (x.J << 11) | ((uint32_t)x.f << 6) | ((uint32_t)x.t << 1) | (x.sign < 0 ? 1u : 0u);
+                    ++w;
+                }
+            }
+        }
+    }
+    const Exc* la = a.suba + Ia * (uint64_t)a.Sa;
+    const str_t Sa = a.sa[Ia];
+    double acc = 0.0;
+    for (int e = 0; e < a.Sa; ++e) {
+        const Exc x = la[e];
+        const uint64_t Ja = x.J;
+        __syncthreads();
+        // V slice: self -> sum over p in Ia of V[p][q][p][s]; single p->r -> s_a V[p][q][r][s]
+        for (int i = threadIdx.x; i < L * L; i += kSigThreads) {
+            const int q = i / L, s = i % L;
+            double v = 0.0;
+            if (x.type == 0) {
+                for (str_t t = Sa; t; t &= t - 1) {
+                    const int p = __builtin_ctzll(t);
+                    v += a.V[(((size_t)p * L + q) * L + p) * L + s];
+                }
+            } else {
+                v = a.V[(((size_t)x.f * L + q) * L + x.t) * L + s];
+                if (x.sign < 0) v = -v;
+            }
+            Vs[i] = v;
+        }
+        const CT* src = C + Ja * db;
+        for (uint64_t i = threadIdx.x; i < db; i += kSigThreads) row[i] = src[i];
+        __syncthreads();
+        if (act) {
+            double t = 0.0;
+            // beta number operators: J_b = Ib
+            double dsum = 0.0;
+            for (str_t u = sbits; u; u &= u - 1) {
+                const int q = __builtin_ctzll(u);
+                dsum += Vs[q * L + q];
+            }
+            t += dsum * (double)row[Ib];
+#pragma unroll
+            for (int j = 0; j < kMaxSingles; ++j) {
+                if (j < nsb) {
+                    const uint32_t c = sb[j];
+                    const int q = (c >> 6) & 31, s = (c >> 1) & 31;
+                    double v = Vs[q * L + s] * (double)row[c >> 11];
+                    t += (c & 1) ? -v : v;
+                }
+            }
+            acc += t;
+        }
+    }
+    if (act) sig[Ia * db + Ib] += acc;
+}
+
+size_t sigma_ab_lds_bytes(const SigmaArgs& a, bool f32) {
+    return (size_t)a.L * a.L * sizeof(double) + a.db * (f32 ? sizeof(float) : sizeof(double));
+}
+
+template <typename CT>
+void launch_sigma_ab(const SigmaArgs& a, const CT* C, double* sig, hipStream_t s) {
+    const uint64_t grid = (a.nblocks + 7) / 8 * 8;
+    const size_t lds = sigma_ab_lds_bytes(a, sizeof(CT) == 4);
+    k_sigma_ab<CT><<<(unsigned)grid, kSigThreads, lds, s>>>(a, C, sig);
+    CF_HIP(hipGetLastError());
+}
+template void launch_sigma_ab<double>(const SigmaArgs&, const double*, double*, hipStream_t);
+template void launch_sigma_ab<float>(const SigmaArgs&, const float*, double*, hipStream_t);
+
+__global__ void k_to_f32(uint64_t n, const double* __restrict__ x, float* __restrict__ y) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = (float)x[i];
+}
+__global__ void k_to_f64(uint64_t n, const float* __restrict__ x, double* __restrict__ y) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = (double)x[i];
+}
+void launch_to_f32(uint64_t n, const double* x, float* y, hipStream_t s) {
+    k_to_f32<<<8192, 256, 0, s>>>(n, x, y);
+    CF_HIP(hipGetLastError());
+}
+void launch_to_f64(uint64_t n, const float* x, double* y, hipStream_t s) {
+    k_to_f64<<<8192, 256, 0, s>>>(n, x, y);
+    CF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// 1-RDM without a stored matrix (large FCI spaces): for every alpha string K
+// and single K -> J (p in K, r in J, phase s): gamma^a_pr += s * C(K,:).C(J,:);
+// the diagonal gamma^a_pp = sum_K n_p(K) |C(K,:)|^2.  Beta from the transposed
+// vector.  One workgroup per K, fixed-order reductions (deterministic).
+// out[K][e] = row dot products for the S entries of K's self+singles list.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_row_dots(uint64_t da, uint64_t db, int S, const Exc* __restrict__ sub,
+                                                  const double* __restrict__ C, double* __restrict__ out) {
+    __shared__ double red[4];
+    const uint64_t K = blockIdx.x;
+    const double* ck = C + K * db;
+    const Exc* lst = sub + K * (uint64_t)S;
+    for (int e = 0; e < S; ++e) {
+        const double* cj = C + (uint64_t)lst[e].J * db;
+        double acc = 0.0;
+        for (uint64_t i = threadIdx.x; i < db; i += 256) acc += ck[i] * cj[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) out[K * (uint64_t)S + e] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
+}
+
+void launch_row_dots(uint64_t da, uint64_t db, int S, const Exc* sub, const double* C, double* out, hipStream_t s) {
+    if (!da) return;
+    k_row_dots<<<(unsigned)da, 256, 0, s>>>(da, db, S, sub, C, out);
+    CF_HIP(hipGetLastError());
+}
+
+// out = in^T for a rows x cols row-major matrix (tiled through LDS)
+__global__ void k_transpose(uint64_t rows, uint64_t cols, const double* __restrict__ in, double* __restrict__ out) {
+    __shared__ double tile[32][33];
+    const uint64_t c0 = (uint64_t)blockIdx.x * 32, r0 = (uint64_t)blockIdx.y * 32;
+    for (int i = threadIdx.y; i < 32; i += 8) {
+        const uint64_t r = r0 + i, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) tile[i][threadIdx.x] = in[r * cols + c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.y; i < 32; i += 8) {
+        const uint64_t c = c0 + i, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[c * rows + r] = tile[threadIdx.x][i];
+    }
+}
+
+void launch_transpose(uint64_t rows, uint64_t cols, const double* in, double* out, hipStream_t s) {
+    dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+    if (grid.y > 65535) throw Error(ERR_UNSUPPORTED, "transpose too tall");
+    k_transpose<<<grid, dim3(32, 8), 0, s>>>(rows, cols, in, out);
+    CF_HIP(hipGetLastError());
+}
+
+}  // namespace cf

@@ -161,6 +161,29 @@ void cf_solve_opts_default(cf_solve_opts* o);
 /* eigvec: host array of n doubles or NULL.  The vector also stays on the GPU
  * for cf_rdm_resident. */
 int cf_solve(cf_crs* m, const cf_solve_opts* opts, double* eigvec, cf_solve_info* info);
+/* The resident eigenvector of the last solve on this space (host out, n). */
+int cf_space_eigvec(cf_space* sp, double* eigvec);
+
+/* ---- on-the-fly sigma for FCI spaces too large to store (replaces the
+ *      O(dim^2) on-the-fly product of CI_Big::mvprod, CIMethod.cpp:943-984):
+ *      sigma = Ha C + C Hb^T (dense same-spin string Hamiltonians, library
+ *      GEMMs) + the alpha-beta part from the string single lists.
+ *      prec: CF_F64, or CF_F32 to also prepare the FP32 product. */
+typedef struct cf_sigma cf_sigma;
+int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out);
+int cf_sigma_destroy(cf_sigma* s);
+/* y = H x, device pointers on the context stream; value_prec CF_F32 uses
+ * FP32 operands with FP64 accumulation of the alpha-beta part. */
+int cf_sigma_apply_dev(cf_sigma* s, int value_prec, const double* dx, double* dy);
+/* y = H x, host pointers, synchronous. */
+int cf_sigma_apply(cf_sigma* s, int value_prec, const double* x, double* y);
+/* Ground state with the on-the-fly sigma as the operator (same options and
+ * result as cf_solve). */
+int cf_solve_direct(cf_sigma* s, const cf_solve_opts* opts, double* eigvec, cf_solve_info* info);
+/* 1-RDM of a large FCI space from row dot products of the CI vector (no
+ * transition-density matrices); c: host vector (n), one[2][L(L+1)/2] in the
+ * reference layout (CIDens.cpp:255-288). */
+int cf_rdm1_direct(cf_space* sp, const double* c, double* one);
 
 /* ---- reverse-communication Lanczos with the dsaupd_ contract
  *      (matrix.h:42-52; loop at SparseMatrix_CRS.cpp:441-449): on ido == 1
