@@ -85,6 +85,24 @@ class SolveInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Request(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("kind", C.c_int), ("L", C.c_int), ("nup", C.c_int), ("ndown", C.c_int),
+                ("h", dp), ("V", dp), ("U", dp), ("econst", C.c_double), ("precision", C.c_int), ("ncv", C.c_int),
+                ("tol", C.c_double)]
+
+
+class Result(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int), ("solve_status", C.c_int), ("n", C.c_uint64),
+                ("energy", C.c_double), ("eigval", C.c_double), ("resid", C.c_double), ("n_matvec", C.c_int),
+                ("n_matvec_f32", C.c_int), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_end", C.c_double), ("error", C.c_char * 160)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["error"] = self.error.decode(errors="replace")
+        return d
+
+
 def header_symbols() -> list[str]:
     """Entry points declared in include/ciflow_amd.h."""
     txt = open(HEADER).read()
@@ -146,6 +164,13 @@ def lib():
         "cf_rdm1_direct": (C.c_int, [vp, dp, dp]),
         "cf_space_eigvec": (C.c_int, [vp, dp]),
         "cf_rdm_dense": (C.c_int, [vp, dp, dp, dp]),
+        "cf_server_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.POINTER(vp)]),
+        "cf_server_submit": (C.c_int, [vp, C.POINTER(Request)]),
+        "cf_server_poll": (C.c_int, [vp, C.POINTER(Result), C.c_int, C.POINTER(C.c_int)]),
+        "cf_server_drain": (C.c_int, [vp, C.c_double]),
+        "cf_server_stats": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
+        "cf_server_destroy": (C.c_int, [vp]),
+        "cf_request_footprint": (C.c_int, [C.POINTER(Request), u64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -564,6 +589,68 @@ class FCI_Direct(CIMethod):
         _check(lib().cf_rdm1_direct(self.ptr, _d(np.ascontiguousarray(c)), _d(one)))
         self._one = one
         return one, None
+
+
+class SolveServer(_Handle):
+    """Continuously batched stream of rotated-integral ground-state solves
+    (the set_ham -> build_ci -> solve loop of OrbitalOptimization::update_cim,
+    OrbitalOptimization.cpp, driven by SimulatedAnnealing.cpp:83-87 and
+    LocalMinimizer.cpp:73-235), served by ``slots`` worker streams on one GPU
+    with HBM-footprint admission."""
+
+    def __init__(self, slots: int = 4, hbm_budget: float = 0.0, device: int = 0):
+        super().__init__("cf_server_destroy")
+        _check(lib().cf_server_create(device, slots, float(hbm_budget), C.byref(self.ptr)))
+
+    def close(self):
+        """Stop the workers (each finishes the request it is running)."""
+        if self.ptr:
+            _check(lib().cf_server_destroy(self.ptr))
+            self.ptr = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def _request(tag, kind, h, V, nup, ndown, U=None, econst=0.0, precision=CF_F64, ncv=0, tol=0.0):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        U = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        L = h.shape[0]
+        assert h.shape == (L, L) and V.shape == (L, L, L, L) and (U is None or U.shape == (L, L))
+        r = Request(tag, kind, L, nup, ndown, _d(h), _d(V), _d(U) if U is not None else None, econst, precision,
+                    ncv, tol)
+        return r, (h, V, U)
+
+    def submit(self, tag: int, kind: int, h, V, nup: int, ndown: int, U=None, econst: float = 0.0,
+               precision: int = CF_F64, ncv: int = 0, tol: float = 0.0):
+        r, keep = self._request(tag, kind, h, V, nup, ndown, U, econst, precision, ncv, tol)
+        _check(lib().cf_server_submit(self.ptr, C.byref(r)))  # arrays are copied inside
+
+    @staticmethod
+    def footprint(kind: int, L: int, nup: int, ndown: int, precision: int = CF_F64, ncv: int = 0) -> int:
+        z = np.zeros(1)
+        r = Request(0, kind, L, nup, ndown, _d(z), _d(z), None, 0.0, precision, ncv, 0.0)
+        b = C.c_uint64()
+        _check(lib().cf_request_footprint(C.byref(r), C.byref(b)))
+        return b.value
+
+    def poll(self, max_results: int = 256) -> list[dict]:
+        buf = (Result * max_results)()
+        cnt = C.c_int()
+        _check(lib().cf_server_poll(self.ptr, buf, max_results, C.byref(cnt)))
+        return [buf[i].as_dict() for i in range(cnt.value)]
+
+    def drain(self, timeout: float = 0.0):
+        _check(lib().cf_server_drain(self.ptr, timeout))
+
+    def stats(self) -> dict:
+        v = [C.c_uint64() for _ in range(4)]
+        _check(lib().cf_server_stats(self.ptr, *[C.byref(x) for x in v]))
+        return dict(zip(["submitted", "finished", "hbm_in_use", "hbm_budget"], [x.value for x in v]))
 
 
 class RCILanczos:

@@ -26,7 +26,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 
 SOURCES = ["strings.hip", "build.hip", "spmv.hip", "spmv_fci.hip", "krylov.hip", "rdm.hip", "transform.hip", "sigma.hip",
-           "lanczos.cpp", "integrals.cpp", "capi.cpp"]
+           "lanczos.cpp", "integrals.cpp", "capi.cpp", "server.cpp"]
 
 
 def _compile(src: str) -> str:

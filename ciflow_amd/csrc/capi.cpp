@@ -35,6 +35,20 @@ int fail(int code, const std::string& msg) {
 }
 }  // namespace
 
+namespace cf {
+int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace cf
+
+namespace {
+// Copies ordered on the handle's stream (which does not synchronise with the
+// null stream); return once the copy has completed.
+void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+    if (!bytes) return;
+    CF_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+    CF_HIP(hipStreamSynchronize(s));
+}
+}  // namespace
+
 #define API_BEGIN try {
 #define API_END                                              \
     }                                                        \
@@ -429,7 +443,7 @@ int cf_ctx_create(int device, cf_ctx** out) {
     for (int k = 0; k <= kMaxL; ++k)
         for (int m = 0; m < kBinomCols; ++m) hb[k * kBinomCols + m] = binom_host(k, m);
     c->binom.alloc(hb.size());
-    CF_HIP(hipMemcpy(c->binom.p, hb.data(), hb.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    copy_sync(c->binom.p, hb.data(), hb.size() * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream);
     *out = c.release();
     API_END
 }
@@ -521,10 +535,10 @@ int cf_ham_create(cf_ctx* ctx, int L, int nup, int ndown, const double* h, const
     H->dV.alloc(L4);
     H->dW.alloc(L3);
     H->dJm.alloc(L2);
-    CF_HIP(hipMemcpy(H->dh.p, h, L2 * sizeof(double), hipMemcpyHostToDevice));
-    CF_HIP(hipMemcpy(H->dV.p, V, L4 * sizeof(double), hipMemcpyHostToDevice));
-    CF_HIP(hipMemcpy(H->dW.p, W.data(), L3 * sizeof(double), hipMemcpyHostToDevice));
-    CF_HIP(hipMemcpy(H->dJm.p, Jm.data(), L2 * sizeof(double), hipMemcpyHostToDevice));
+    copy_sync(H->dh.p, h, L2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    copy_sync(H->dV.p, V, L4 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    copy_sync(H->dW.p, W.data(), L3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    copy_sync(H->dJm.p, Jm.data(), L2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
     *out = H.release();
     API_END
 }
@@ -681,7 +695,7 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     }
     exclusive_scan_u32_to_i64(cnt.p, m->rowptr.p, m->n, c->stream);
     int64_t nnz = 0;
-    CF_HIP(hipMemcpy(&nnz, m->rowptr.p + m->n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    copy_sync(&nnz, m->rowptr.p + m->n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
     m->nnz = (uint64_t)nnz;
     m->col.alloc(m->nnz);
     m->val.alloc(m->nnz);
@@ -727,7 +741,7 @@ static uint64_t count_rows(cf_crs* m, uint64_t r0, uint64_t r1, bool upper, std:
         CF_HIP(hipStreamSynchronize(c->stream));
     } else {
         std::vector<int64_t> rp(nr + 1);
-        CF_HIP(hipMemcpy(rp.data(), m->rowptr.p + r0, (nr + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+        copy_sync(rp.data(), m->rowptr.p + r0, (nr + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
         for (uint64_t i = 0; i < nr; ++i) cnt[i] = (uint64_t)(rp[i + 1] - rp[i]);
     }
     uint64_t tot = 0;
@@ -749,7 +763,7 @@ static int export_rows(cf_crs* m, uint64_t r0, uint64_t r1, bool upper, uint64_t
     for (uint64_t i = 0; i < nr; ++i) row[i + 1] = row[i] + per[i];
     if (!nr) return CF_OK;
     DBuf<uint64_t> off(nr);
-    CF_HIP(hipMemcpy(off.p, row, nr * sizeof(uint64_t), hipMemcpyHostToDevice));
+    copy_sync(off.p, row, nr * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream);
     DBuf<uint32_t> oc(tot ? tot : 1);
     DBuf<double> ov(tot ? tot : 1);
     launch_export_rows(m->rowptr.p, m->col.p, m->val.p, r0, r1, upper, off.p, oc.p, ov.p, c->stream);
@@ -893,7 +907,7 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     CF_HIP(hipMemcpyAsync(hv.data(), b.v(0), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CF_HIP(hipStreamSynchronize(c->stream));
     sign_fix(hv);
-    CF_HIP(hipMemcpy(sp->vec.p, hv.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    copy_sync(sp->vec.p, hv.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream);
     sp->vec_valid = true;
     if (eigvec) std::memcpy(eigvec, hv.data(), n * sizeof(double));
     info->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -969,7 +983,7 @@ int cf_space_eigvec(cf_space* sp, double* eigvec) {
     CF_REQUIRE(sp && eigvec, "NULL argument");
     CF_REQUIRE(sp->vec_valid, "no solve has run on this space");
     sp->ham->ctx->use();
-    CF_HIP(hipMemcpy(eigvec, sp->vec.p, sp->n * sizeof(double), hipMemcpyDeviceToHost));
+    copy_sync(eigvec, sp->vec.p, sp->n * sizeof(double), hipMemcpyDeviceToHost, sp->ham->ctx->stream);
     API_END
 }
 

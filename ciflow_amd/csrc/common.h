@@ -19,6 +19,9 @@ struct Error : std::runtime_error {
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+// Records msg as the calling thread's cf_last_error() text; returns code.
+int set_last_error(int code, const std::string& msg);
+
 enum Status {
     OK = 0,
     ERR_ARG = 1,

@@ -1,0 +1,285 @@
+// Continuously batched ground-state solve server.
+//
+// The reference issues one rotated-integral solve after another from its
+// orbital optimisers (OrbitalOptimization::update_cim -> CIMethod::set_ham ->
+// build_ci -> solve, SimulatedAnnealing.cpp:83-87; LocalMinimizer.cpp:73-235),
+// rebuilding the CRS and freeing it after every solve (CIMethod.cpp:117-126,
+// 262).  Here requests enter a queue at any time; `slots` worker threads, each
+// with its own device context and stream, take the next request whose HBM
+// footprint fits the remaining budget (admission), run rotate -> strings ->
+// CRS build -> Lanczos on their stream, and publish the result.  Kernels of
+// different requests overlap on the device; a request's own kernels are
+// deterministic (fixed-order reductions), so its result does not depend on
+// what else is in flight.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ciflow_amd.h"
+#include "common.h"
+
+namespace {
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Job {
+    cf_request req;
+    std::vector<double> h, V, U;
+    uint64_t footprint = 0;
+    double t_submit = 0.0;
+};
+}  // namespace
+
+struct cf_server {
+    int device = 0;
+    uint64_t budget = 0, in_use = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Job> pending;
+    std::deque<cf_result> done;
+    std::vector<std::thread> workers;
+    int running = 0;
+    uint64_t submitted = 0, finished = 0;
+    bool stop = false;
+    std::string first_error;
+};
+
+namespace {
+uint64_t estimate_footprint(const cf_request& r) {
+    const int L = r.L;
+    const uint64_t da = cf::binom_host(L, r.nup), db = cf::binom_host(L, r.ndown);
+    uint64_t n, row;
+    if (r.kind == CF_DOCI) {
+        n = da;
+        row = 1 + (uint64_t)r.nup * (L - r.nup);
+    } else {
+        n = da * db;
+        const uint64_t sa = (uint64_t)r.nup * (L - r.nup), sb = (uint64_t)r.ndown * (L - r.ndown);
+        const uint64_t dda = cf::binom_host(r.nup, 2) * cf::binom_host(L - r.nup, 2);
+        const uint64_t ddb = cf::binom_host(r.ndown, 2) * cf::binom_host(L - r.ndown, 2);
+        row = 1 + sa + sb + sa * sb + dda + ddb;
+    }
+    const uint64_t bytes_nnz = r.precision == CF_F32 ? 16 : 12;
+    const uint64_t crs = n * row * bytes_nnz + (n + 1) * 8 + n * 4;
+    const int ncv = r.ncv > 0 ? r.ncv : 42;
+    const uint64_t krylov = (uint64_t)(ncv + 3) * n * 8;
+    const uint64_t ints = 3 * (uint64_t)L * L * L * L * 8;
+    return crs + krylov + ints + (64ull << 20);
+}
+
+void worker_main(cf_server* S) {
+    cf_ctx* ctx = nullptr;
+    if (cf_ctx_create(S->device, &ctx) != CF_OK) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->first_error.empty()) S->first_error = cf_last_error();
+        S->stop = true;
+        S->cv_work.notify_all();
+        S->cv_done.notify_all();
+        return;
+    }
+    for (;;) {
+        Job job;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            for (;;) {
+                if (S->stop) {
+                    lk.unlock();
+                    cf_ctx_destroy(ctx);
+                    return;
+                }
+                // first queued job that fits (FIFO among those that fit; a job
+                // larger than the whole budget runs alone)
+                auto it = S->pending.begin();
+                for (; it != S->pending.end(); ++it)
+                    if (S->in_use + it->footprint <= S->budget || (S->running == 0 && S->in_use == 0)) break;
+                if (it != S->pending.end()) {
+                    job = std::move(*it);
+                    S->pending.erase(it);
+                    S->in_use += job.footprint;
+                    S->running++;
+                    break;
+                }
+                S->cv_work.wait(lk);
+            }
+        }
+        cf_result res;
+        std::memset(&res, 0, sizeof(res));
+        res.tag = job.req.tag;
+        res.t_submit = job.t_submit;
+        res.t_start = now_s();
+        res.n = 0;
+        const cf_request& r = job.req;
+        cf_ham *ham = nullptr, *hr = nullptr;
+        cf_space* sp = nullptr;
+        cf_crs* m = nullptr;
+        int st = cf_ham_create(ctx, r.L, r.nup, r.ndown, job.h.data(), job.V.data(), r.econst, &ham);
+        if (st == CF_OK && !job.U.empty()) st = cf_ham_rotate(ham, job.U.data(), &hr);
+        if (st == CF_OK) st = cf_space_create(hr ? hr : ham, r.kind, &sp);
+        if (st == CF_OK) st = cf_crs_build(sp, r.precision, 1e-14, &m);
+        if (st == CF_OK) {
+            cf_solve_opts o;
+            cf_solve_opts_default(&o);
+            if (r.ncv > 0) o.ncv = r.ncv;
+            if (r.tol > 0) o.tol = r.tol;
+            o.precision = r.precision;
+            cf_solve_info info;
+            st = cf_solve(m, &o, nullptr, &info);
+            if (st == CF_OK) {
+                res.energy = info.energy;
+                res.eigval = info.eigval;
+                res.resid = info.resid;
+                res.n_matvec = info.n_matvec;
+                res.n_matvec_f32 = info.n_matvec_f32;
+                res.solve_status = info.status;
+            }
+            uint64_t n = 0;
+            cf_space_dims(sp, &n, nullptr, nullptr);
+            res.n = n;
+        }
+        res.status = st;
+        if (st != CF_OK) {
+            std::snprintf(res.error, sizeof(res.error), "%s", cf_last_error());
+        }
+        cf_crs_destroy(m);
+        cf_space_destroy(sp);
+        cf_ham_destroy(hr);
+        cf_ham_destroy(ham);
+        cf_ctx_synchronize(ctx);
+        res.t_end = now_s();
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->in_use -= job.footprint;
+            S->running--;
+            S->finished++;
+            S->done.push_back(res);
+        }
+        S->cv_work.notify_all();
+        S->cv_done.notify_all();
+    }
+}
+}  // namespace
+
+#define SRV_BEGIN try {
+#define SRV_END                                       \
+    }                                                 \
+    catch (const cf::Error& e) {                      \
+        return cf::set_last_error(e.code, e.what());  \
+    }                                                 \
+    catch (const std::exception& e) {                 \
+        return cf::set_last_error(CF_ERR_HIP, e.what()); \
+    }                                                 \
+    return CF_OK;
+
+extern "C" {
+
+int cf_request_footprint(const cf_request* r, uint64_t* bytes) {
+    SRV_BEGIN
+    CF_REQUIRE(r && bytes, "NULL argument");
+    CF_REQUIRE(r->L >= 1 && r->L <= cf::kMaxL && r->nup >= 0 && r->nup <= r->L && r->ndown >= 0 &&
+                   r->ndown <= r->L,
+               "bad request");
+    *bytes = estimate_footprint(*r);
+    SRV_END
+}
+
+int cf_server_create(int device, int slots, double hbm_budget_bytes, cf_server** out) {
+    SRV_BEGIN
+    CF_REQUIRE(out && slots >= 1 && slots <= 64, "bad arguments");
+    std::unique_ptr<cf_server> S(new cf_server());
+    S->device = device;
+    if (hbm_budget_bytes <= 0) {
+        size_t fr = 0, tot = 0;
+        CF_HIP(hipSetDevice(device));
+        CF_HIP(hipMemGetInfo(&fr, &tot));
+        S->budget = (uint64_t)(0.8 * (double)fr);
+    } else {
+        S->budget = (uint64_t)hbm_budget_bytes;
+    }
+    for (int i = 0; i < slots; ++i) S->workers.emplace_back(worker_main, S.get());
+    *out = S.release();
+    SRV_END
+}
+
+int cf_server_submit(cf_server* S, const cf_request* r) {
+    SRV_BEGIN
+    CF_REQUIRE(S && r && r->h && r->V, "NULL argument");
+    CF_REQUIRE(r->L >= 1 && r->L <= cf::kMaxL && (r->kind == CF_FCI || r->kind == CF_DOCI), "bad request");
+    CF_REQUIRE(r->precision == CF_F64 || r->precision == CF_F32, "bad precision");
+    CF_REQUIRE(r->nup >= 0 && r->nup <= r->L && r->ndown >= 0 && r->ndown <= r->L, "electron counts out of range");
+    Job j;
+    j.req = *r;
+    const size_t L2 = (size_t)r->L * r->L;
+    j.h.assign(r->h, r->h + L2);
+    j.V.assign(r->V, r->V + L2 * L2);
+    if (r->U) j.U.assign(r->U, r->U + L2);
+    j.footprint = estimate_footprint(*r);
+    j.t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        CF_REQUIRE(!S->stop, S->first_error.empty() ? "server stopped" : S->first_error);
+        S->pending.push_back(std::move(j));
+        S->submitted++;
+    }
+    S->cv_work.notify_one();
+    SRV_END
+}
+
+int cf_server_poll(cf_server* S, cf_result* out, int max, int* count) {
+    SRV_BEGIN
+    CF_REQUIRE(S && count && (max == 0 || out), "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    int c = 0;
+    while (c < max && !S->done.empty()) {
+        out[c++] = S->done.front();
+        S->done.pop_front();
+    }
+    *count = c;
+    SRV_END
+}
+
+int cf_server_drain(cf_server* S, double timeout_s) {
+    SRV_BEGIN
+    CF_REQUIRE(S, "NULL argument");
+    std::unique_lock<std::mutex> lk(S->mu);
+    const auto pred = [S] { return S->stop || S->finished == S->submitted; };
+    if (timeout_s > 0) {
+        if (!S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+            throw cf::Error(CF_ERR_STATE, "drain timed out");
+    } else {
+        S->cv_done.wait(lk, pred);
+    }
+    CF_REQUIRE(S->first_error.empty(), S->first_error);
+    SRV_END
+}
+
+int cf_server_stats(cf_server* S, uint64_t* submitted, uint64_t* finished, uint64_t* in_use, uint64_t* budget) {
+    SRV_BEGIN
+    CF_REQUIRE(S, "NULL argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (submitted) *submitted = S->submitted;
+    if (finished) *finished = S->finished;
+    if (in_use) *in_use = S->in_use;
+    if (budget) *budget = S->budget;
+    SRV_END
+}
+
+int cf_server_destroy(cf_server* S) {
+    SRV_BEGIN
+    if (!S) return CF_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (auto& t : S->workers) t.join();
+    delete S;
+    SRV_END
+}
+
+}  // extern "C"

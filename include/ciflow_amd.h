@@ -207,6 +207,51 @@ int cf_rdm(cf_space* sp, const double* c, double* one, double* two);
  * Gamma[b][i][j][k][l] = <a+_is a+_jt a_lt a_ks>, b: 0 aa, 1 ab, 2 bb. */
 int cf_rdm_dense(cf_space* sp, const double* c, double* gamma, double* Gamma);
 
+/* ---- served stream of rotated-integral solves: the loop the orbital
+ *      optimisers run one request at a time (OrbitalOptimization::update_cim
+ *      -> CIMethod::set_ham -> build_ci -> solve, SimulatedAnnealing.cpp:83-87,
+ *      LocalMinimizer.cpp:73-235, CIMethod.cpp:117-126, 241-268), here as a
+ *      queue served continuously by `slots` worker threads with one context
+ *      (stream) each on one GPU.  A request is admitted when its estimated HBM
+ *      footprint fits the budget still free.  Results are independent of what
+ *      else runs (every reduction has a fixed order). */
+typedef struct cf_server cf_server;
+typedef struct {
+    uint64_t tag;       /* returned with the result */
+    int kind;           /* CF_FCI / CF_DOCI */
+    int L, nup, ndown;
+    const double* h;    /* L*L (copied at submit) */
+    const double* V;    /* L^4 (copied at submit) */
+    const double* U;    /* L*L rotation applied on the GPU first, or NULL */
+    double econst;
+    int precision;      /* CF_F64 or CF_F32 (FP32 Lanczos + FP64 polish) */
+    int ncv;            /* <= 0: 42 */
+    double tol;         /* <= 0: 1e-12 */
+} cf_request;
+typedef struct {
+    uint64_t tag;
+    int status;         /* CF_OK or the failing entry point's status */
+    int solve_status;   /* cf_solve_info.status */
+    uint64_t n;         /* determinants */
+    double energy, eigval, resid;
+    int n_matvec, n_matvec_f32;
+    double t_submit, t_start, t_end;  /* steady-clock seconds */
+    char error[160];
+} cf_result;
+/* hbm_budget_bytes <= 0: 80% of the device memory free at creation. */
+int cf_server_create(int device, int slots, double hbm_budget_bytes, cf_server** out);
+int cf_server_submit(cf_server* s, const cf_request* r);
+/* Move up to max finished results (completion order) into out. */
+int cf_server_poll(cf_server* s, cf_result* out, int max, int* count);
+/* Wait until every submitted request has finished (timeout_s <= 0: no limit). */
+int cf_server_drain(cf_server* s, double timeout_s);
+int cf_server_stats(cf_server* s, uint64_t* submitted, uint64_t* finished, uint64_t* hbm_in_use,
+                    uint64_t* hbm_budget);
+/* Stops the workers after the request each is running; unstarted requests are dropped. */
+int cf_server_destroy(cf_server* s);
+/* The admission estimate of a request (bytes). */
+int cf_request_footprint(const cf_request* r, uint64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

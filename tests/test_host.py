@@ -99,3 +99,33 @@ def test_expm_skew_matches_scipy_and_is_orthogonal():
     # a large rotation exercises the squaring phase
     K = CA.gen_rotation(6, 9, 1.5)
     assert np.max(np.abs(CA.expm_skew(K) - sl.expm(K))) < 1e-12
+
+
+# --- served stream: host side ------------------------------------------------
+def test_stream_spec_is_deterministic_and_mixed():
+    from ciflow_amd import stream as S
+    a = S.make_stream(2000, rate=100.0)
+    b = S.make_stream(2000, rate=100.0)
+    assert a == b
+    doci = [r for r in a if r.kind == CA.CF_DOCI]
+    fci = [r for r in a if r.kind == CA.CF_FCI]
+    assert abs(len(doci) / len(a) - 0.7) < 0.04
+    assert {r.L for r in doci} == set(range(12, 21)) and {r.L for r in fci} == set(range(8, 12))
+    assert all(r.n == r.L // 2 and r.seed == 1000 + r.k for r in a)
+    t = np.array([r.t_arrive for r in a])
+    assert t[0] == 0.0 and np.all(np.diff(t) >= 0)
+    assert abs(t[-1] / (len(a) - 1) - 0.01) < 0.001   # mean gap 1/rate
+    h, V, U = S.request_arrays(a[0])
+    assert np.max(np.abs(U @ U.T - np.eye(a[0].L))) < 1e-14
+
+
+def test_request_footprint_and_argument_checks():
+    fp = CA.SolveServer.footprint
+    small = fp(CA.CF_DOCI, 12, 6, 6, CA.CF_F32)
+    big = fp(CA.CF_FCI, 11, 5, 5, CA.CF_F32)
+    assert 64 << 20 < small < big
+    # FCI L=11 5a5b: 213,444 rows of 1,261 entries at 16 B (f64 + f32 value, u32 column)
+    assert 4.2e9 < big < 4.6e9
+    assert fp(CA.CF_FCI, 11, 5, 5, CA.CF_F64) < big
+    with pytest.raises(CA.CIFlowError):
+        fp(CA.CF_FCI, 8, 9, 4)
