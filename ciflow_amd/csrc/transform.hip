@@ -40,10 +40,11 @@ __global__ void __launch_bounds__(64) k_quarter_mfma(int L, uint64_t M, const do
         const double bv = (k < L && a0 + r < L) ? Ub[(uint64_t)(a0 + r) * L + k] : 0.0;  // B[k][a]
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
-    // D[i][j]: lane = j + 16*(i/4), element i%4
+    // f64 16x16x4 accumulator layout: D[i][j] sits in lane j + 16*(i%4),
+    // element i/4 (row = 4*element + lane/16, column = lane%16)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const uint64_t m = m0 + 4 * kk + v;
+        const uint64_t m = m0 + 4 * v + kk;
         const int a = a0 + r;
         if (m < M && a < L) ob[m * L + a] = acc[v];
     }
