@@ -80,6 +80,8 @@ struct cf_ctx {
     std::map<std::string, TimerFamily> timers;
     std::vector<hipEvent_t> pool;
     DBuf<double> hx, hy;  // staging for host-pointer products
+    double* pinned = nullptr;  // kPinned doubles of page-locked memory for small read-backs
+    static constexpr int kPinned = 256;
 
     void use() { CF_HIP(hipSetDevice(device)); }
     hipEvent_t event() {
@@ -111,6 +113,7 @@ struct cf_ctx {
                 (void)hipEventDestroy(pr.second);
             }
         for (auto e : pool) (void)hipEventDestroy(e);
+        if (pinned) (void)hipHostFree(pinned);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -257,10 +260,10 @@ struct DeviceBackend : KrylovBackend {
     cf_ctx* c;
     uint64_t n;
     int M;
-    DBuf<double> Vb, w, partial, cvec, c2, Yd, scal;
+    DBuf<double> Vb, w, partial, cvec, c2, Yd, scal, dalpha, dbeta;
     int nb;
     std::function<void(const double*, double*)> op;
-    std::vector<double> hbuf;
+    double* hbuf;  // the context's pinned read-back buffer
 
     DeviceBackend(cf_ctx* ctx, uint64_t n_, int ncv) : c(ctx), n(n_) {
         M = (uint64_t)ncv > n ? (int)n : ncv;
@@ -272,7 +275,9 @@ struct DeviceBackend : KrylovBackend {
         c2.alloc(64);
         Yd.alloc(64 * 64);
         scal.alloc(4);
-        hbuf.resize(8);
+        dalpha.alloc(64);
+        dbeta.alloc(64);
+        hbuf = c->pinned;
     }
     double* v(int i) { return Vb.p + (size_t)i * n; }
     void dots(int k, const double* base, const double* x, double* out) {
@@ -280,10 +285,28 @@ struct DeviceBackend : KrylovBackend {
         launch_reduce_partials(k, nb, partial.p, out, c->stream);
     }
     double fetch(const double* d) {
-        double h;
-        CF_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipMemcpyAsync(hbuf, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         CF_HIP(hipStreamSynchronize(c->stream));
-        return h;
+        return hbuf[0];
+    }
+    bool deferred() const override { return true; }
+    void orth_deferred(int j) override {
+        dots(j + 1, Vb.p, w.p, cvec.p);
+        launch_multi_axpy(n, j + 1, Vb.p, cvec.p, w.p, c->stream);
+        dots(j + 1, Vb.p, w.p, c2.p);
+        launch_multi_axpy(n, j + 1, Vb.p, c2.p, w.p, c->stream);
+        dots(1, w.p, w.p, scal.p);
+        launch_finish_step(n, j, cvec.p, c2.p, scal.p, w.p, v(j + 1), dalpha.p, dbeta.p, c->stream);
+    }
+    void fetch_coeffs(int j0, int j1, double* alpha, double* beta) override {
+        const int k = j1 - j0;
+        CF_HIP(hipMemcpyAsync(hbuf, dalpha.p + j0, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipMemcpyAsync(hbuf + 64, dbeta.p + j0, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < k; ++i) {
+            alpha[i] = hbuf[i];
+            beta[i] = hbuf[64 + i];
+        }
     }
     void start_random(uint64_t seed) override {
         launch_fill_start(n, seed, v(0), c->stream);
@@ -439,6 +462,7 @@ int cf_ctx_create(int device, cf_ctx** out) {
     c->device = device;
     c->use();
     CF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CF_HIP(hipHostMalloc((void**)&c->pinned, cf_ctx::kPinned * sizeof(double), hipHostMallocDefault));
     std::vector<uint64_t> hb((kMaxL + 1) * kBinomCols, 0);
     for (int k = 0; k <= kMaxL; ++k)
         for (int m = 0; m < kBinomCols; ++m) hb[k * kBinomCols + m] = binom_host(k, m);
@@ -861,6 +885,7 @@ void cf_solve_opts_default(cf_solve_opts* o) {
     o->precision = CF_F64;
     o->tol_f32 = 1e-8;
     o->seed = 20240601ULL;
+    o->check_every = 0;
 }
 
 }  // extern "C"
@@ -876,6 +901,7 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     else cf_solve_opts_default(&o);
     CF_REQUIRE(o.ncv >= 2 && o.ncv <= 63, "ncv must be in [2, 63]");
     CF_REQUIRE(o.tol > 0.0, "tol must be > 0");
+    CF_REQUIRE(o.check_every >= 0, "check_every must be >= 0");
     CF_REQUIRE(o.precision == CF_F64 || o.precision == CF_F32, "precision must be CF_F64 or CF_F32");
     cf_ctx* c = sp->ham->ctx;
     const auto t0 = std::chrono::steady_clock::now();
@@ -886,14 +912,17 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     if (o.precision == CF_F32) {
         CF_REQUIRE((bool)op32, "the FP32 operator is not available (build with CF_F32)");
         b.op = op32;
-        ThickRestartLanczos sm(&b, n, o.ncv, o.tol_f32, maxr, o.seed, false);
+        ThickRestartLanczos sm(&b, n, o.ncv, o.tol_f32, maxr, o.seed, false, o.check_every);
         while (sm.step() == ThickRestartLanczos::NEED_APPLY) b.apply(sm.pending());
         nmv32 = sm.n_apply();
         nrs += sm.n_restart();
         keep = true;  // v0 holds the FP32-phase Ritz vector: polish from it
     }
     b.op = op64;
-    ThickRestartLanczos sm(&b, n, o.ncv, o.tol, maxr, o.seed, keep);
+    // the polish starts from a vector converged to tol_f32: test every few
+    // steps instead of only when the basis is full
+    const int check = keep ? (o.check_every > 0 ? o.check_every : 4) : o.check_every;
+    ThickRestartLanczos sm(&b, n, o.ncv, o.tol, maxr, o.seed, keep, check);
     while (sm.step() == ThickRestartLanczos::NEED_APPLY) b.apply(sm.pending());
     info->eigval = sm.eigval();
     info->energy = sm.eigval() + sp->ham->econst;

@@ -105,6 +105,8 @@ void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, doub
 void launch_reduce_partials(int k, int nblocks, const double* partial, double* out, hipStream_t stream);
 int multi_dot_blocks(uint64_t n);
 void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream);
+void launch_finish_step(uint64_t n, int j, const double* c1, const double* c2, const double* wnorm2,
+                        const double* w, double* vnext, double* alpha, double* beta, hipStream_t stream);
 void launch_scale_copy(uint64_t n, const double* src, double s, double* dst, hipStream_t stream);
 void launch_rotate_basis(uint64_t n, int m, int k, double* Vb, const double* Y, hipStream_t stream);
 void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream);

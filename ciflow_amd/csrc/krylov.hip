@@ -112,6 +112,32 @@ void launch_scale_copy(uint64_t n, const double* src, double s, double* dst, hip
     CF_HIP(hipGetLastError());
 }
 
+// End of a Lanczos step without a host round trip: beta = sqrt(max(0, |w|^2)),
+// v_next = w * (1 / beta) (zero when beta is zero), alpha_j = c1[j] + c2[j]
+// (the two Gram-Schmidt coefficients on v_j).  Same arithmetic as the host
+// path (sqrt, 1/beta, then one multiply per element).
+__global__ void k_finish_step(uint64_t n, int j, const double* __restrict__ c1, const double* __restrict__ c2,
+                              const double* __restrict__ wnorm2, const double* __restrict__ w,
+                              double* __restrict__ vnext, double* __restrict__ alpha, double* __restrict__ beta) {
+    const double b = sqrt(fmax(0.0, *wnorm2));
+    const double s = b > 0.0 ? 1.0 / b : 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) vnext[q] = w[q] * s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        alpha[j] = c1[j] + c2[j];
+        beta[j] = b;
+    }
+}
+
+void launch_finish_step(uint64_t n, int j, const double* c1, const double* c2, const double* wnorm2,
+                        const double* w, double* vnext, double* alpha, double* beta, hipStream_t stream) {
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    if (grid < 1) grid = 1;
+    k_finish_step<<<(unsigned)grid, 256, 0, stream>>>(n, j, c1, c2, wnorm2, w, vnext, alpha, beta);
+    CF_HIP(hipGetLastError());
+}
+
 // In place: Vb[:, 0:k] = Vb[:, 0:m] * Y[:, 0:k]   (Y is m x m row-major).
 // Each thread owns element q of every vector, so reading all m inputs before
 // writing the k outputs is race-free.

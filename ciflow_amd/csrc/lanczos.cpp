@@ -3,62 +3,138 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 
 namespace cf {
 
+// Householder reduction to tridiagonal form followed by the implicit QL
+// algorithm with Wilkinson-type shifts (the classical symmetric eigensolver,
+// e.g. Golub & Van Loan, Matrix Computations, 4th ed., sections 8.3.1 and
+// 8.3.3).  O(m^3) with a small constant; the Krylov matrices here have m <= 64.
 void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& Z) {
-    Z.assign((size_t)m * m, 0.0);
-    for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
-    for (int sweep = 0; sweep < 64; ++sweep) {
-        double off = 0.0, tot = 0.0;
-        for (int i = 0; i < m; ++i)
-            for (int j = 0; j < m; ++j) {
-                double a = A[(size_t)i * m + j] * A[(size_t)i * m + j];
-                tot += a;
-                if (i != j) off += a;
+    w.assign(m, 0.0);
+    std::vector<double> e(m, 0.0);
+    auto a = [&](int i, int j) -> double& { return A[(size_t)i * m + j]; };
+    // Householder: A := Q^T A Q tridiagonal, Q accumulated into A at the end
+    for (int i = m - 1; i > 0; --i) {
+        const int l = i - 1;
+        double h = 0.0;
+        if (l > 0) {
+            double scale = 0.0;
+            for (int k = 0; k <= l; ++k) scale += std::fabs(a(i, k));
+            if (scale == 0.0) {
+                e[i] = a(i, l);
+            } else {
+                for (int k = 0; k <= l; ++k) {
+                    a(i, k) /= scale;
+                    h += a(i, k) * a(i, k);
+                }
+                const double f = a(i, l);
+                const double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
+                e[i] = scale * g;
+                h -= f * g;
+                a(i, l) = f - g;
+                double ff = 0.0;
+                for (int j = 0; j <= l; ++j) {
+                    a(j, i) = a(i, j) / h;
+                    double gg = 0.0;
+                    for (int k = 0; k <= j; ++k) gg += a(j, k) * a(i, k);
+                    for (int k = j + 1; k <= l; ++k) gg += a(k, j) * a(i, k);
+                    e[j] = gg / h;
+                    ff += e[j] * a(i, j);
+                }
+                const double hh = ff / (h + h);
+                for (int j = 0; j <= l; ++j) {
+                    const double fj = a(i, j);
+                    const double gj = e[j] - hh * fj;
+                    e[j] = gj;
+                    for (int k = 0; k <= j; ++k) a(j, k) -= fj * e[k] + gj * a(i, k);
+                }
             }
-        if (off == 0.0 || off <= 1e-34 * tot) break;
-        for (int p = 0; p < m - 1; ++p)
-            for (int q = p + 1; q < m; ++q) {
-                const double apq = A[(size_t)p * m + q];
-                if (apq == 0.0) continue;
-                const double app = A[(size_t)p * m + p], aqq = A[(size_t)q * m + q];
-                const double th = (aqq - app) / (2.0 * apq);
-                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
-                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-                for (int r = 0; r < m; ++r) {
-                    double arp = A[(size_t)r * m + p], arq = A[(size_t)r * m + q];
-                    A[(size_t)r * m + p] = c * arp - s * arq;
-                    A[(size_t)r * m + q] = s * arp + c * arq;
-                }
-                for (int r = 0; r < m; ++r) {
-                    double apr = A[(size_t)p * m + r], aqr = A[(size_t)q * m + r];
-                    A[(size_t)p * m + r] = c * apr - s * aqr;
-                    A[(size_t)q * m + r] = s * apr + c * aqr;
-                }
-                for (int r = 0; r < m; ++r) {
-                    double zrp = Z[(size_t)r * m + p], zrq = Z[(size_t)r * m + q];
-                    Z[(size_t)r * m + p] = c * zrp - s * zrq;
-                    Z[(size_t)r * m + q] = s * zrp + c * zrq;
-                }
-            }
+        } else {
+            e[i] = a(i, l);
+        }
+        w[i] = h;
     }
+    w[0] = 0.0;
+    e[0] = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const int l = i - 1;
+        if (w[i] != 0.0) {
+            for (int j = 0; j <= l; ++j) {
+                double g = 0.0;
+                for (int k = 0; k <= l; ++k) g += a(i, k) * a(k, j);
+                for (int k = 0; k <= l; ++k) a(k, j) -= g * a(k, i);
+            }
+        }
+        w[i] = a(i, i);
+        a(i, i) = 1.0;
+        for (int j = 0; j <= l; ++j) a(j, i) = a(i, j) = 0.0;
+    }
+    // implicit QL on (w, e), rotations applied to the columns of A
+    for (int i = 1; i < m; ++i) e[i - 1] = e[i];
+    if (m > 0) e[m - 1] = 0.0;
+    for (int l = 0; l < m; ++l) {
+        int iter = 0;
+        for (;;) {
+            int mm = l;
+            for (; mm < m - 1; ++mm) {
+                const double dd = std::fabs(w[mm]) + std::fabs(w[mm + 1]);
+                if (std::fabs(e[mm]) <= std::numeric_limits<double>::epsilon() * dd) break;
+            }
+            if (mm == l) break;
+            if (++iter > 60) break;  // not reached for symmetric input
+            double g = (w[l + 1] - w[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = w[mm] - w[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int i = mm - 1;
+            for (; i >= l; --i) {
+                double f = s * e[i];
+                const double b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    w[i + 1] -= p;
+                    e[mm] = 0.0;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = w[i + 1] - p;
+                r = (w[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                w[i + 1] = g + p;
+                g = c * r - b;
+                for (int k = 0; k < m; ++k) {
+                    f = a(k, i + 1);
+                    a(k, i + 1) = s * a(k, i) + c * f;
+                    a(k, i) = c * a(k, i) - s * f;
+                }
+            }
+            if (r == 0.0 && i >= l) continue;
+            w[l] -= p;
+            e[l] = g;
+            e[mm] = 0.0;
+        }
+    }
+    // ascending order, vectors as columns of Z
     std::vector<int> idx(m);
     for (int i = 0; i < m; ++i) idx[i] = i;
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](int a, int b) { return A[(size_t)a * m + a] < A[(size_t)b * m + b]; });
-    w.resize(m);
-    std::vector<double> Zs((size_t)m * m);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return w[x] < w[y]; });
+    std::vector<double> ws(m);
+    Z.assign((size_t)m * m, 0.0);
     for (int c = 0; c < m; ++c) {
-        w[c] = A[(size_t)idx[c] * m + idx[c]];
-        for (int r = 0; r < m; ++r) Zs[(size_t)r * m + c] = Z[(size_t)r * m + idx[c]];
+        ws[c] = w[idx[c]];
+        for (int r = 0; r < m; ++r) Z[(size_t)r * m + c] = a(r, idx[c]);
     }
-    Z.swap(Zs);
+    w.swap(ws);
 }
 
 ThickRestartLanczos::ThickRestartLanczos(KrylovBackend* b, uint64_t n, int ncv, double tol, int maxrestart,
-                                         uint64_t seed, bool start_given)
-    : b_(b), n_(n), tol_(tol), maxrestart_(maxrestart), seed_(seed), start_given_(start_given) {
+                                         uint64_t seed, bool start_given, int check_every)
+    : b_(b), n_(n), tol_(tol), maxrestart_(maxrestart), seed_(seed), start_given_(start_given),
+      check_every_(check_every > 0 ? check_every : 0) {
     M_ = (uint64_t)ncv > n ? (int)n : ncv;
     if (M_ < 1) M_ = 1;
     T_.assign((size_t)M_ * M_, 0.0);
@@ -70,7 +146,7 @@ ThickRestartLanczos::Event ThickRestartLanczos::step() {
             case INIT:
                 if (start_given_) b_->start_keep();
                 else b_->start_random(seed_);
-                j_ = k_ = 0;
+                j_ = k_ = filled_ = 0;
                 phase_ = LOOP_APPLY;
                 break;
             case LOOP_APPLY:
@@ -79,6 +155,21 @@ ThickRestartLanczos::Event ThickRestartLanczos::step() {
                 return NEED_APPLY;
             case AFTER_APPLY: {
                 ++napply_;
+                if (b_->deferred()) {
+                    b_->orth_deferred(j_);
+                    ++j_;
+                    if (j_ == M_) {
+                        if (!absorb(j_)) {
+                            m_ = M_;
+                            eig_and_restart();
+                        }
+                        break;
+                    }
+                    if (check_every_ && (j_ - k_) % check_every_ == 0 && (absorb(j_) || converged_early(j_)))
+                        break;
+                    phase_ = LOOP_APPLY;
+                    break;
+                }
                 double a = 0.0, beta = 0.0;
                 b_->orth(j_, &a, &beta);
                 T_[(size_t)j_ * M_ + j_] = a;
@@ -120,6 +211,53 @@ ThickRestartLanczos::Event ThickRestartLanczos::step() {
     }
 }
 
+// Steps filled_..j1-1 ran without host round trips: read their coefficients
+// and replay them in order with the breakdown test of the immediate path.
+// Returns true when a breakdown ended the cycle (products applied after it
+// are discounted: their vectors are not used).
+bool ThickRestartLanczos::absorb(int j1) {
+    const int j0 = filled_;
+    if (j1 <= j0) return false;
+    std::vector<double> al(j1 - j0), be(j1 - j0);
+    b_->fetch_coeffs(j0, j1, al.data(), be.data());
+    for (int jj = j0; jj < j1; ++jj) {
+        const double a = al[jj - j0], beta = be[jj - j0];
+        T_[(size_t)jj * M_ + jj] = a;
+        beta_last_ = beta;
+        anorm_ = std::max(anorm_, std::fabs(a) + beta);
+        if (beta <= 1e-15 * anorm_ || (uint64_t)(jj + 1) == n_) {
+            napply_ -= j_ - 1 - jj;
+            m_ = jj + 1;
+            breakdown_ = true;
+            eig_and_restart();
+            return true;
+        }
+        if (jj + 1 < M_) {
+            T_[(size_t)jj * M_ + jj + 1] = beta;
+            T_[(size_t)(jj + 1) * M_ + jj] = beta;
+        }
+    }
+    filled_ = j1;
+    return false;
+}
+
+// Convergence test on the leading m x m block in the middle of a cycle.
+bool ThickRestartLanczos::converged_early(int m) {
+    std::vector<double> A((size_t)m * m);
+    for (int a = 0; a < m; ++a)
+        for (int c = 0; c < m; ++c) A[(size_t)a * m + c] = T_[(size_t)a * M_ + c];
+    std::vector<double> th, Y;
+    sym_eig(m, A, th, Y);
+    const double res = std::fabs(beta_last_ * Y[(size_t)(m - 1) * m + 0]);
+    if (res > tol_ * std::fabs(th[0])) return false;
+    m_ = m;
+    status_ = 0;
+    b_->rotate(m, 1, Y.data());
+    b_->normalize0();
+    phase_ = FINAL_APPLY;
+    return true;
+}
+
 void ThickRestartLanczos::eig_and_restart() {
     const int m = m_;
     std::vector<double> A((size_t)m * m);
@@ -150,6 +288,7 @@ void ThickRestartLanczos::eig_and_restart() {
     }
     k_ = kk;
     j_ = kk;
+    filled_ = kk;
     ++nrestart_;
     phase_ = LOOP_APPLY;
 }

@@ -35,6 +35,13 @@ struct KrylovBackend {
     virtual void normalize0() = 0;
     // with w = A v_0: theta = v_0.w, resid = ||w - theta v_0||
     virtual void rayleigh(double* theta, double* resid) = 0;
+    // Deferred coefficients (a backend whose vectors live on a device): orth
+    // plus v_{j+1} = w / beta_j entirely on the device, alpha_j and beta_j kept
+    // there; fetch_coeffs returns those of steps [j0, j1) with one wait.  The
+    // arithmetic is that of orth + set_next, so results do not change.
+    virtual bool deferred() const { return false; }
+    virtual void orth_deferred(int /*j*/) {}
+    virtual void fetch_coeffs(int /*j0*/, int /*j1*/, double* /*alpha*/, double* /*beta*/) {}
 };
 
 // Symmetric eigen-decomposition (cyclic Jacobi), ascending; Z columns = vectors.
@@ -43,8 +50,10 @@ void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<d
 class ThickRestartLanczos {
 public:
     enum Event { NEED_APPLY = 1, DONE = 99 };
+    // check_every > 0 (deferred backends): also test convergence every
+    // check_every steps inside a cycle, not only when the basis is full.
     ThickRestartLanczos(KrylovBackend* b, uint64_t n, int ncv, double tol, int maxrestart, uint64_t seed,
-                        bool start_given);
+                        bool start_given, int check_every = 0);
     // Advance until the operator must be applied to v_{pending()} (NEED_APPLY,
     // the result is expected in the backend's w) or the run ended (DONE).
     Event step();
@@ -58,6 +67,8 @@ public:
 private:
     enum Phase { INIT, LOOP_APPLY, AFTER_APPLY, FINAL_APPLY, AFTER_FINAL, FINISHED };
     void eig_and_restart();
+    bool absorb(int j1);
+    bool converged_early(int m);
     KrylovBackend* b_;
     uint64_t n_;
     int M_, m_ = 0, j_ = 0, k_ = 0;
@@ -65,6 +76,7 @@ private:
     int maxrestart_;
     uint64_t seed_;
     bool start_given_;
+    int check_every_ = 0, filled_ = 0;
     Phase phase_ = INIT;
     int pending_ = 0;
     std::vector<double> T_;

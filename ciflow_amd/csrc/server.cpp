@@ -128,6 +128,7 @@ void worker_main(cf_server* S) {
             if (r.ncv > 0) o.ncv = r.ncv;
             if (r.tol > 0) o.tol = r.tol;
             o.precision = r.precision;
+            o.check_every = r.check_every > 0 ? r.check_every : 0;
             cf_solve_info info;
             st = cf_solve(m, &o, nullptr, &info);
             if (st == CF_OK) {

@@ -144,6 +144,10 @@ typedef struct {
     int precision;      /* CF_F64, or CF_F32: FP32-value Lanczos + FP64 polish */
     double tol_f32;     /* convergence of the FP32 phase (1e-8) */
     uint64_t seed;      /* start-vector seed */
+    int check_every;    /* > 0: also test convergence every check_every Lanczos steps
+                           inside a restart cycle; 0: only when the basis is full, as
+                           ARPACK does.  The FP64 polish of a CF_F32 solve always tests
+                           every check_every (4 when 0) steps. */
 } cf_solve_opts;
 
 typedef struct {
@@ -227,6 +231,7 @@ typedef struct {
     int precision;      /* CF_F64 or CF_F32 (FP32 Lanczos + FP64 polish) */
     int ncv;            /* <= 0: 42 */
     double tol;         /* <= 0: 1e-12 */
+    int check_every;    /* cf_solve_opts.check_every */
 } cf_request;
 typedef struct {
     uint64_t tag;
