@@ -74,7 +74,8 @@ class CIFlowError(RuntimeError):
 
 class SolveOpts(C.Structure):
     _fields_ = [("ncv", C.c_int), ("maxrestart", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
-                ("tol_f32", C.c_double), ("seed", C.c_uint64), ("check_every", C.c_int)]
+                ("tol_f32", C.c_double), ("seed", C.c_uint64), ("check_every", C.c_int),
+                ("tol_energy", C.c_double)]
 
 
 class SolveInfo(C.Structure):
@@ -88,7 +89,7 @@ class SolveInfo(C.Structure):
 class Request(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("kind", C.c_int), ("L", C.c_int), ("nup", C.c_int), ("ndown", C.c_int),
                 ("h", dp), ("V", dp), ("U", dp), ("econst", C.c_double), ("precision", C.c_int), ("ncv", C.c_int),
-                ("tol", C.c_double), ("check_every", C.c_int)]
+                ("tol", C.c_double), ("check_every", C.c_int), ("tol_energy", C.c_double), ("direct", C.c_int)]
 
 
 class Result(C.Structure):
@@ -615,25 +616,29 @@ class SolveServer(_Handle):
         self.close()
 
     @staticmethod
-    def _request(tag, kind, h, V, nup, ndown, U=None, econst=0.0, precision=CF_F64, ncv=0, tol=0.0, check_every=0):
+    def _request(tag, kind, h, V, nup, ndown, U=None, econst=0.0, precision=CF_F64, ncv=0, tol=0.0, check_every=0,
+                 tol_energy=0.0, direct=False):
         h = np.ascontiguousarray(h, dtype=np.float64)
         V = np.ascontiguousarray(V, dtype=np.float64)
         U = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
         L = h.shape[0]
         assert h.shape == (L, L) and V.shape == (L, L, L, L) and (U is None or U.shape == (L, L))
         r = Request(tag, kind, L, nup, ndown, _d(h), _d(V), _d(U) if U is not None else None, econst, precision,
-                    ncv, tol, check_every)
+                    ncv, tol, check_every, tol_energy, int(bool(direct)))
         return r, (h, V, U)
 
     def submit(self, tag: int, kind: int, h, V, nup: int, ndown: int, U=None, econst: float = 0.0,
-               precision: int = CF_F64, ncv: int = 0, tol: float = 0.0, check_every: int = 0):
-        r, keep = self._request(tag, kind, h, V, nup, ndown, U, econst, precision, ncv, tol, check_every)
+               precision: int = CF_F64, ncv: int = 0, tol: float = 0.0, check_every: int = 0,
+               tol_energy: float = 0.0, direct: bool = False):
+        r, keep = self._request(tag, kind, h, V, nup, ndown, U, econst, precision, ncv, tol, check_every,
+                                tol_energy, direct)
         _check(lib().cf_server_submit(self.ptr, C.byref(r)))  # arrays are copied inside
 
     @staticmethod
-    def footprint(kind: int, L: int, nup: int, ndown: int, precision: int = CF_F64, ncv: int = 0) -> int:
+    def footprint(kind: int, L: int, nup: int, ndown: int, precision: int = CF_F64, ncv: int = 0,
+                  direct: bool = False) -> int:
         z = np.zeros(1)
-        r = Request(0, kind, L, nup, ndown, _d(z), _d(z), None, 0.0, precision, ncv, 0.0, 0)
+        r = Request(0, kind, L, nup, ndown, _d(z), _d(z), None, 0.0, precision, ncv, 0.0, 0, 0.0, int(bool(direct)))
         b = C.c_uint64()
         _check(lib().cf_request_footprint(C.byref(r), C.byref(b)))
         return b.value

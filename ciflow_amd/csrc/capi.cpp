@@ -82,8 +82,24 @@ struct cf_ctx {
     DBuf<double> hx, hy;  // staging for host-pointer products
     double* pinned = nullptr;  // kPinned doubles of page-locked memory for small read-backs
     static constexpr int kPinned = 256;
+    rocblas_handle blas_ = nullptr;  // created on first use, bound to the stream
 
-    void use() { CF_HIP(hipSetDevice(device)); }
+    rocblas_handle blas() {
+        if (!blas_) {
+            if (rocblas_create_handle(&blas_) != rocblas_status_success) {
+                blas_ = nullptr;
+                throw Error(ERR_HIP, "rocblas_create_handle failed");
+            }
+            if (rocblas_set_stream(blas_, stream) != rocblas_status_success)
+                throw Error(ERR_HIP, "rocblas_set_stream failed");
+        }
+        return blas_;
+    }
+
+    void use() {
+        CF_HIP(hipSetDevice(device));
+        tls_alloc_stream = stream;
+    }
     hipEvent_t event() {
         if (!pool.empty()) {
             hipEvent_t e = pool.back();
@@ -113,8 +129,17 @@ struct cf_ctx {
                 (void)hipEventDestroy(pr.second);
             }
         for (auto e : pool) (void)hipEventDestroy(e);
+        if (blas_) (void)rocblas_destroy_handle(blas_);
         if (pinned) (void)hipHostFree(pinned);
-        if (stream) (void)hipStreamDestroy(stream);
+        // buffers ordered on the stream go back to the pool before it is destroyed
+        binom.release();
+        hx.release();
+        hy.release();
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        if (tls_alloc_stream == stream) tls_alloc_stream = nullptr;
     }
 };
 
@@ -460,8 +485,14 @@ int cf_ctx_create(int device, cf_ctx** out) {
     CF_REQUIRE(device >= 0 && device < ndev, "device index out of range");
     std::unique_ptr<cf_ctx> c(new cf_ctx());
     c->device = device;
-    c->use();
+    CF_HIP(hipSetDevice(device));
+    // keep freed blocks in the device's stream-ordered pool for reuse
+    hipMemPool_t mp = nullptr;
+    CF_HIP(hipDeviceGetDefaultMemPool(&mp, device));
+    uint64_t keep = UINT64_MAX;
+    CF_HIP(hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep));
     CF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->use();
     CF_HIP(hipHostMalloc((void**)&c->pinned, cf_ctx::kPinned * sizeof(double), hipHostMallocDefault));
     std::vector<uint64_t> hb((kMaxL + 1) * kBinomCols, 0);
     for (int k = 0; k <= kMaxL; ++k)
@@ -886,6 +917,7 @@ void cf_solve_opts_default(cf_solve_opts* o) {
     o->tol_f32 = 1e-8;
     o->seed = 20240601ULL;
     o->check_every = 0;
+    o->tol_energy = 0.0;
 }
 
 }  // extern "C"
@@ -902,6 +934,7 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     CF_REQUIRE(o.ncv >= 2 && o.ncv <= 63, "ncv must be in [2, 63]");
     CF_REQUIRE(o.tol > 0.0, "tol must be > 0");
     CF_REQUIRE(o.check_every >= 0, "check_every must be >= 0");
+    CF_REQUIRE(o.tol_energy >= 0.0, "tol_energy must be >= 0");
     CF_REQUIRE(o.precision == CF_F64 || o.precision == CF_F32, "precision must be CF_F64 or CF_F32");
     cf_ctx* c = sp->ham->ctx;
     const auto t0 = std::chrono::steady_clock::now();
@@ -912,7 +945,7 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     if (o.precision == CF_F32) {
         CF_REQUIRE((bool)op32, "the FP32 operator is not available (build with CF_F32)");
         b.op = op32;
-        ThickRestartLanczos sm(&b, n, o.ncv, o.tol_f32, maxr, o.seed, false, o.check_every);
+        ThickRestartLanczos sm(&b, n, o.ncv, o.tol_f32, maxr, o.seed, false, o.check_every, o.tol_energy);
         while (sm.step() == ThickRestartLanczos::NEED_APPLY) b.apply(sm.pending());
         nmv32 = sm.n_apply();
         nrs += sm.n_restart();
@@ -922,7 +955,7 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     // the polish starts from a vector converged to tol_f32: test every few
     // steps instead of only when the basis is full
     const int check = keep ? (o.check_every > 0 ? o.check_every : 4) : o.check_every;
-    ThickRestartLanczos sm(&b, n, o.ncv, o.tol, maxr, o.seed, keep, check);
+    ThickRestartLanczos sm(&b, n, o.ncv, o.tol, maxr, o.seed, keep, check, o.tol_energy);
     while (sm.step() == ThickRestartLanczos::NEED_APPLY) b.apply(sm.pending());
     info->eigval = sm.eigval();
     info->energy = sm.eigval() + sp->ham->econst;
@@ -951,11 +984,8 @@ struct cf_sigma {
     const double* Hb = nullptr;
     const float* Hb32 = nullptr;
     DBuf<float> x32, y32;
-    rocblas_handle blas = nullptr;
+    rocblas_handle blas = nullptr;  // the context's handle
     SigmaArgs args;
-    ~cf_sigma() {
-        if (blas) rocblas_destroy_handle(blas);
-    }
 };
 
 namespace {
@@ -1051,8 +1081,7 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
         S->x32.alloc(sp->n);
         S->y32.alloc(sp->n);
     }
-    CF_BLAS(rocblas_create_handle(&S->blas));
-    CF_BLAS(rocblas_set_stream(S->blas, c->stream));
+    S->blas = c->blas();
     SigmaArgs& a = S->args;
     a.da = sp->da;
     a.db = sp->db;

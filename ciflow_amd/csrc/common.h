@@ -45,30 +45,61 @@ enum Status {
         if (!(cond)) throw ::cf::Error(::cf::ERR_ARG, (msg));  \
     } while (0)
 
+// Stream that device buffers allocated on this host thread are ordered on:
+// a context makes its stream current here, and its buffers then come from the
+// device's stream-ordered pool (hipMallocAsync / hipFreeAsync), so allocating
+// and freeing neither blocks the host nor waits for other streams.  Null:
+// plain hipMalloc / hipFree.
+inline thread_local hipStream_t tls_alloc_stream = nullptr;
+
 // Owning device buffer.
 template <class T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
+    hipStream_t s = nullptr;  // allocation stream (null: hipMalloc)
     DBuf() = default;
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
-    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), s(o.s) {
+        o.p = nullptr;
+        o.n = 0;
+        o.s = nullptr;
+    }
     DBuf& operator=(DBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            s = o.s;
+            o.p = nullptr;
+            o.n = 0;
+            o.s = nullptr;
+        }
         return *this;
     }
     ~DBuf() { release(); }
     void alloc(size_t count) {
         release();
-        if (count) CF_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+        if (count) {
+            if (tls_alloc_stream) {
+                CF_HIP(hipMallocAsync((void**)&p, count * sizeof(T), tls_alloc_stream));
+                s = tls_alloc_stream;
+            } else {
+                CF_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+            }
+        }
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (s) (void)hipFreeAsync(p, s);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        s = nullptr;
     }
     size_t bytes() const { return n * sizeof(T); }
 };

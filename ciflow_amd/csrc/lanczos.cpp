@@ -132,9 +132,9 @@ void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<d
 }
 
 ThickRestartLanczos::ThickRestartLanczos(KrylovBackend* b, uint64_t n, int ncv, double tol, int maxrestart,
-                                         uint64_t seed, bool start_given, int check_every)
+                                         uint64_t seed, bool start_given, int check_every, double tol_energy)
     : b_(b), n_(n), tol_(tol), maxrestart_(maxrestart), seed_(seed), start_given_(start_given),
-      check_every_(check_every > 0 ? check_every : 0) {
+      check_every_(check_every > 0 ? check_every : 0), tol_energy_(tol_energy > 0.0 ? tol_energy : 0.0) {
     M_ = (uint64_t)ncv > n ? (int)n : ncv;
     if (M_ < 1) M_ = 1;
     T_.assign((size_t)M_ * M_, 0.0);
@@ -241,6 +241,11 @@ bool ThickRestartLanczos::absorb(int j1) {
     return false;
 }
 
+bool ThickRestartLanczos::accept(double res, const std::vector<double>& th) const {
+    if (res <= tol_ * std::fabs(th[0])) return true;
+    return tol_energy_ > 0.0 && th.size() >= 2 && res * res <= tol_energy_ * (th[1] - th[0]);
+}
+
 // Convergence test on the leading m x m block in the middle of a cycle.
 bool ThickRestartLanczos::converged_early(int m) {
     std::vector<double> A((size_t)m * m);
@@ -249,7 +254,7 @@ bool ThickRestartLanczos::converged_early(int m) {
     std::vector<double> th, Y;
     sym_eig(m, A, th, Y);
     const double res = std::fabs(beta_last_ * Y[(size_t)(m - 1) * m + 0]);
-    if (res > tol_ * std::fabs(th[0])) return false;
+    if (!accept(res, th)) return false;
     m_ = m;
     status_ = 0;
     b_->rotate(m, 1, Y.data());
@@ -266,7 +271,7 @@ void ThickRestartLanczos::eig_and_restart() {
     std::vector<double> th, Y;
     sym_eig(m, A, th, Y);
     const double res = std::fabs(beta_last_ * Y[(size_t)(m - 1) * m + 0]);
-    const bool converged = res <= tol_ * std::fabs(th[0]) || m < M_ || breakdown_;
+    const bool converged = accept(res, th) || m < M_ || breakdown_;
     const bool limit = nrestart_ >= maxrestart_;
     if (converged || limit) {
         status_ = converged ? 0 : 1;

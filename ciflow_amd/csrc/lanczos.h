@@ -52,8 +52,10 @@ public:
     enum Event { NEED_APPLY = 1, DONE = 99 };
     // check_every > 0 (deferred backends): also test convergence every
     // check_every steps inside a cycle, not only when the basis is full.
+    // tol_energy > 0: also accept the Ritz pair once ||r||^2 / (theta_1 -
+    // theta_0) <= tol_energy, the Kato-Temple bound on its eigenvalue error.
     ThickRestartLanczos(KrylovBackend* b, uint64_t n, int ncv, double tol, int maxrestart, uint64_t seed,
-                        bool start_given, int check_every = 0);
+                        bool start_given, int check_every = 0, double tol_energy = 0.0);
     // Advance until the operator must be applied to v_{pending()} (NEED_APPLY,
     // the result is expected in the backend's w) or the run ended (DONE).
     Event step();
@@ -77,6 +79,8 @@ private:
     uint64_t seed_;
     bool start_given_;
     int check_every_ = 0, filled_ = 0;
+    double tol_energy_ = 0.0;
+    bool accept(double res, const std::vector<double>& th) const;
     Phase phase_ = INIT;
     int pending_ = 0;
     std::vector<double> T_;

@@ -54,6 +54,14 @@ namespace {
 uint64_t estimate_footprint(const cf_request& r) {
     const int L = r.L;
     const uint64_t da = cf::binom_host(L, r.nup), db = cf::binom_host(L, r.ndown);
+    const int ncv = r.ncv > 0 ? r.ncv : 42;
+    const uint64_t ints = 3 * (uint64_t)L * L * L * L * 8;
+    if (r.kind == CF_FCI && r.direct) {
+        // Krylov basis + dense string Hamiltonians (FP64 and FP32) + FP32 vectors + lists
+        const uint64_t n = da * db;
+        const uint64_t dense = (da * da + (r.nup == r.ndown ? 0 : db * db)) * 12;
+        return (uint64_t)(ncv + 3) * n * 8 + dense + 2 * n * 4 + (da + db) * 4096 + ints + (64ull << 20);
+    }
     uint64_t n, row;
     if (r.kind == CF_DOCI) {
         n = da;
@@ -67,9 +75,7 @@ uint64_t estimate_footprint(const cf_request& r) {
     }
     const uint64_t bytes_nnz = r.precision == CF_F32 ? 16 : 12;
     const uint64_t crs = n * row * bytes_nnz + (n + 1) * 8 + n * 4;
-    const int ncv = r.ncv > 0 ? r.ncv : 42;
     const uint64_t krylov = (uint64_t)(ncv + 3) * n * 8;
-    const uint64_t ints = 3 * (uint64_t)L * L * L * L * 8;
     return crs + krylov + ints + (64ull << 20);
 }
 
@@ -118,10 +124,12 @@ void worker_main(cf_server* S) {
         cf_ham *ham = nullptr, *hr = nullptr;
         cf_space* sp = nullptr;
         cf_crs* m = nullptr;
+        cf_sigma* sg = nullptr;
+        const bool direct = r.kind == CF_FCI && r.direct;
         int st = cf_ham_create(ctx, r.L, r.nup, r.ndown, job.h.data(), job.V.data(), r.econst, &ham);
         if (st == CF_OK && !job.U.empty()) st = cf_ham_rotate(ham, job.U.data(), &hr);
         if (st == CF_OK) st = cf_space_create(hr ? hr : ham, r.kind, &sp);
-        if (st == CF_OK) st = cf_crs_build(sp, r.precision, 1e-14, &m);
+        if (st == CF_OK) st = direct ? cf_sigma_create(sp, r.precision, &sg) : cf_crs_build(sp, r.precision, 1e-14, &m);
         if (st == CF_OK) {
             cf_solve_opts o;
             cf_solve_opts_default(&o);
@@ -129,8 +137,9 @@ void worker_main(cf_server* S) {
             if (r.tol > 0) o.tol = r.tol;
             o.precision = r.precision;
             o.check_every = r.check_every > 0 ? r.check_every : 0;
+            o.tol_energy = r.tol_energy > 0.0 ? r.tol_energy : 0.0;
             cf_solve_info info;
-            st = cf_solve(m, &o, nullptr, &info);
+            st = direct ? cf_solve_direct(sg, &o, nullptr, &info) : cf_solve(m, &o, nullptr, &info);
             if (st == CF_OK) {
                 res.energy = info.energy;
                 res.eigval = info.eigval;
@@ -148,6 +157,7 @@ void worker_main(cf_server* S) {
             std::snprintf(res.error, sizeof(res.error), "%s", cf_last_error());
         }
         cf_crs_destroy(m);
+        cf_sigma_destroy(sg);
         cf_space_destroy(sp);
         cf_ham_destroy(hr);
         cf_ham_destroy(ham);

@@ -148,6 +148,8 @@ typedef struct {
                            inside a restart cycle; 0: only when the basis is full, as
                            ARPACK does.  The FP64 polish of a CF_F32 solve always tests
                            every check_every (4 when 0) steps. */
+    double tol_energy;  /* > 0: also stop once ||r||^2 / (theta_1 - theta_0) <= tol_energy
+                           (Kato-Temple bound on the eigenvalue error, Eh); 0: off */
 } cf_solve_opts;
 
 typedef struct {
@@ -232,6 +234,8 @@ typedef struct {
     int ncv;            /* <= 0: 42 */
     double tol;         /* <= 0: 1e-12 */
     int check_every;    /* cf_solve_opts.check_every */
+    double tol_energy;  /* cf_solve_opts.tol_energy */
+    int direct;         /* FCI only: 1 = on-the-fly sigma (cf_solve_direct) instead of a stored CRS */
 } cf_request;
 typedef struct {
     uint64_t tag;
