@@ -1,16 +1,23 @@
-"""Benchmark: CI ground-state solves/s on one MI355X.
+"""Benchmark: CI ground-state solves/s, p50/p99 latency and SpMV roofline on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): FCI, L = 12
-spatial orbitals, 6 alpha + 6 beta electrons, 853,776 determinants, seeded
-synthetic integrals (seed 1).  One step = one pass of the hot path: string
-enumeration + excitation lists, Slater-Condon CRS build on the GPU
-(1,553,018,544 stored nonzeros), Lanczos ground-state solve to
-||r|| <= 1e-12 |E|.  The integrals are uploaded before the timed region.
+Default workload (--workload stream): the served stream of rotated-integral
+ground-state solves -- 70% DOCI with L uniform in [12, 20] and 30% FCI with L
+uniform in [8, 11], half filling, integrals from seed 1000+k rotated by
+U = exp(K), K antisymmetric with N(0, 0.05^2) entries.  Each request is
+served with FP32 matrix values (FP32 sigma for FCI) and an FP64 polish, and
+stops at a Kato-Temple energy bound of 1e-9 Eh.  One step = one request.
+  * value: solves/s of K requests submitted at once (closed batch) to the
+    continuously batched server (--slots worker streams),
+  * p50/p99: latency (submit -> result) of Poisson arrivals at 70% of that
+    rate, over the first min(K, 1024) requests,
+  * one_at_a_time_fp64: the plain path (one request at a time, FP64, ARPACK
+    tolerance 1e-12) on every 4th request, and max |dE| against it,
+  * roofline: the stored-CRS SpMV of FCI L=12 6a6b (1,553,018,544 nonzeros),
+    timed with HIP events on the library stream during one FP64 solve.
+--workload fci12: FCI L=12 6a6b seed 1 build + solve per step.
 
-Prints ONE JSON line (see the keys below).  `roofline` is measured live for
-the dominant kernel (the CRS SpMV) with HIP events on the library's stream;
-`cpu_baseline` times the oracle's restatement of the reference CPU path on a
-bounded sample and extrapolates to one solve.
+Prints ONE JSON line.  cpu_baseline times the oracle's restatement of the
+reference CPU path on a bounded sample and extrapolates (see cpu_baseline_*).
 """
 from __future__ import annotations
 
@@ -26,19 +33,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of peak)"
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=4)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", choices=["stream", "fci12"], default="stream")
+    p.add_argument("--steps", type=int, default=None, help="stream: requests (4096); fci12: solves (4)")
+    p.add_argument("--warmup", type=int, default=None, help="stream: warm-up requests (64); fci12: solves (1)")
+    p.add_argument("--slots", type=int, default=8)
+    p.add_argument("--load", type=float, default=0.7, help="Poisson rate as a fraction of the measured rate")
+    p.add_argument("--rate", type=float, default=0.0, help="Poisson rate (solves/s); overrides --load")
+    p.add_argument("--baseline-every", type=int, default=4)
     p.add_argument("--L", type=int, default=12)
     p.add_argument("--nel", type=int, default=6)
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--precision", choices=["f64", "f32"], default="f64")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-lines", type=int, default=6, help="alpha lines in the CPU-baseline sample")
+    p.add_argument("--no-roofline", action="store_true")
+    p.add_argument("--cpu-lines", type=int, default=6, help="alpha lines in the FCI CPU-baseline sample")
     return p.parse_args()
 
 
@@ -46,56 +60,227 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(h, V, L, nel, nnz_upper, n, n_matvec, lines):
-    """Oracle restatement of the reference CPU path on a bounded sample:
-    FCI::construct_CI_matrix with PushToRow (OpenMP build_parallel over the
-    host threads) for `lines` evenly spaced alpha strings, and the serial
-    upper-triangle SparseMatrix_CRS_Sym::mvprod (the reference's g++ path) on
-    that slice.  Extrapolated per nonzero to one solve: build + n_matvec
-    products."""
+# ---------------------------------------------------------------------------
+# CPU baselines (oracle = the restatement of the reference CPU path)
+# ---------------------------------------------------------------------------
+def _oracle_threads():
     from oracle import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or O.num_threads()
-    da = O.binom(L, nel)
-    pick = np.linspace(0, da - 1, lines).round().astype(int)
-    t_build = 0.0
-    t_mv = 0.0
-    nnz_s = 0
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or O.num_threads()
+
+
+def _sample_build_mv(kind, h, V, nel, ranges, n, threads):
+    """Build the CRS rows of the picked line ranges (build_parallel + PushToRow)
+    and run the serial upper-triangle mvprod over them; returns (t_build,
+    t_mvprod, sampled rows, sampled upper nnz)."""
+    from oracle import oracle as O
     x = np.random.default_rng(0).standard_normal(n)
     y = np.zeros(n)
-    for ia in pick:
+    tb = tm = 0.0
+    rows = nnz = 0
+    for s0, s1 in ranges:
         t0 = time.perf_counter()
-        m = O.build_crs("FCI", h, V, nel, nel, start=int(ia), end=int(ia) + 1, threads=threads)
-        t_build += time.perf_counter() - t0
-        nnz_s += m.nnz
-        # the slice's rows scatter into the whole y, as in the full product
+        m = O.build_crs(kind, h, V, nel, nel, start=int(s0), end=int(s1), threads=threads)
+        tb += time.perf_counter() - t0
+        rows += m.n
+        nnz += m.nnz
         t0 = time.perf_counter()
         O.lib().or_mvprod_sym(m.n, m.row.ctypes.data_as(O.u64p), m.col.ctypes.data_as(O.u32p),
                               m.data.ctypes.data_as(O.dp), x.ctypes.data_as(O.dp), y.ctypes.data_as(O.dp))
-        t_mv += time.perf_counter() - t0
+        tm += time.perf_counter() - t0
+    return tb, tm, rows, nnz
+
+
+def _line_ranges(total, count, width=1):
+    """`count` evenly spaced ranges of `width` lines in [0, total)."""
+    width = max(1, min(width, total // max(count, 1)))
+    starts = np.linspace(0, total - width, min(count, total)).round().astype(int)
+    return [(int(a), int(a) + width) for a in starts]
+
+
+def cpu_baseline_fci(h, V, L, nel, nnz_upper, n, n_matvec, lines):
+    """FCI::construct_CI_matrix with PushToRow (build_parallel over the host
+    threads) for `lines` evenly spaced alpha strings, plus the serial
+    upper-triangle SparseMatrix_CRS_Sym::mvprod (the reference's g++ path) on
+    that slice; extrapolated per nonzero to one solve = build + n_matvec products."""
+    from oracle import oracle as O
+    threads = _oracle_threads()
+    da = O.binom(L, nel)
+    tb, tm, rows, nnz_s = _sample_build_mv("FCI", h, V, nel, _line_ranges(da, lines), n, threads)
     scale = nnz_upper / nnz_s
-    t_solve = t_build * scale + n_matvec * t_mv * scale
+    t_solve = tb * scale + n_matvec * tm * scale
     return {
-        "value": 1.0 / t_solve,
-        "unit": "solves/s",
-        "cores": threads,
-        "kind": "port",
+        "value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
         "sample": (f"oracle FCI::construct_CI_matrix (PushToRow, build_parallel on {threads} threads) + serial "
-                   f"upper-triangle mvprod over {lines} alpha lines ({lines * (n // da)} rows, {nnz_s} upper nnz); "
+                   f"upper-triangle mvprod over {lines} alpha lines ({rows} rows, {nnz_s} upper nnz); "
                    f"extrapolated by nnz to one solve = build + {n_matvec} mvprod"),
-        "build_s_est": t_build * scale,
-        "mvprod_s_est": t_mv * scale,
-        "sample_s": t_build + t_mv,
+        "build_s_est": tb * scale, "mvprod_s_est": tm * scale, "sample_s": tb + tm,
     }
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
-                          "n_gpus": a.gpus}))
-        sys.exit(2)
-    import ciflow_amd as CA
+def cpu_baseline_stream(reqs, nmv_by_class, lines=96):
+    """Per (kind, L) class of the stream: the oracle's rotation of the
+    integrals (OrbitalTransform::fillHamCI restated), the CRS build of
+    `lines` evenly spaced alpha lines (DOCI: rows) with PushToRow on all
+    host threads, and the serial upper-triangle mvprod on them; extrapolated
+    by rows to the class's full matrix and to its mean FP64 matvec count on
+    the GPU (the reference's one-at-a-time FP64 path).  Stream CPU time =
+    sum over the requests of their class time."""
+    from math import comb
 
+    from ciflow_amd import stream as S
+    from oracle import oracle as O
+    threads = _oracle_threads()
+    per = {}
+    t_sample = 0.0
+    for key in sorted({(r.kind, r.L) for r in reqs}):
+        r = next(q for q in reqs if (q.kind, q.L) == key)
+        kind = "DOCI" if r.kind == 1 else "FCI"
+        h, V, U = S.request_arrays(r)
+        t0 = time.perf_counter()
+        hr, Vr = O.rotate_integrals(U, h, V)
+        t_rot = time.perf_counter() - t0
+        d1 = comb(r.L, r.n)
+        n = d1 if kind == "DOCI" else d1 * d1
+        # FCI: `lines` alpha strings (d_beta rows each); DOCI: 32 blocks of up to 4096 rows
+        ranges = _line_ranges(d1, lines) if kind == "FCI" else _line_ranges(d1, 32, 4096)
+        tb, tm, rows, _ = _sample_build_mv(kind, hr, Vr, r.n, ranges, n, threads)
+        scale = n / rows
+        nmv = nmv_by_class.get(key, 0.0)
+        per[key] = t_rot + tb * scale + nmv * tm * scale
+        t_sample += t_rot + tb + tm
+    total = sum(per[(r.kind, r.L)] for r in reqs)
+    return {
+        "value": len(reqs) / total, "unit": "solves/s", "cores": threads, "kind": "port",
+        "sample": (f"per (kind, L) class: oracle integral rotation + CRS build (PushToRow, build_parallel on "
+                   f"{threads} threads) of {lines} alpha lines (DOCI: 32 blocks of <= 4096 rows) + serial "
+                   f"upper-triangle mvprod on "
+                   f"them, extrapolated by rows to the full matrix and to the class's mean FP64 matvec count "
+                   f"({len(per)} classes, {len(reqs)} requests)"),
+        "sample_s": t_sample,
+    }
+
+
+# ---------------------------------------------------------------------------
+# SpMV roofline on the stored CRS (FCI L=12 6a6b)
+# ---------------------------------------------------------------------------
+def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
+    h, V = CA.gen_integrals(L, seed)
+    ham = CA.Hamiltonian(h, V, nel, nel, 0.0, ctx)
+    cim = CA.FCI(ham, value_prec=CA.CF_F64)
+    m = cim.get_mat()
+    n, nnz_full, _ = m.info()
+    x = np.random.default_rng(0).standard_normal(n)
+    m.mvprod(x)  # warm
+    ctx.synchronize()
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    info = cim.solve(precision=CA.CF_F64)
+    ctx.synchronize()
+    nl, ms = ctx.kernel_time("spmv_f64")
+    ctx.enable_timing(False)
+    avg_ms = ms / max(nl, 1)
+    algo = nnz_full * 12 + (n + 1) * 8 + 16 * n
+    ach = algo / (avg_ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "kernel": "k_spmv<double,16> (stored full-row CRS, FP64 values, u32 columns)",
+           "workload": f"FCI L={L} {nel}a{nel}b seed {seed}: n={n}, nnz={nnz_full}, one FP64 Lanczos solve",
+           "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+           "avg_launch_ms": avg_ms, "launches": nl, "algorithmic_bytes": algo,
+           "solve_energy": cim.get_ci_energy(), "solve_n_matvec": info.n_matvec}
+    del cim, m
+    return out
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def bench_stream(a, CA):
+    from ciflow_amd import stream as S
+    K = a.steps if a.steps is not None else 4096
+    W = a.warmup if a.warmup is not None else 64
+    served = dict(precision=CA.CF_F32, check_every=4, tol_energy=1e-9, direct_fci=True)
+    reqs = S.make_stream(K, rate=0.0)
+    warm = S.make_stream(W, rate=0.0, first_seed=500000, mix_seed=99) if W > 0 else []
+    t0 = time.perf_counter()
+    arrs = S.prepare(reqs)
+    warr = S.prepare(warm) if warm else []
+    log(f"prepared {K}+{W} requests in {time.perf_counter() - t0:.1f} s")
+    srv = CA.SolveServer(slots=a.slots)
+    try:
+        if warm:
+            S.run_stream(warm, warr, server=srv, timeout=600, **served)
+        log("timed closed batch ...")
+        out = S.run_stream(reqs, arrs, server=srv, timeout=1200, **served)
+        rate = out["solves_per_s"]
+        log(f"closed batch: {rate:.2f} solves/s ({out['wall_s']:.2f} s)")
+        nlat = min(K, 1024)
+        lam = a.rate if a.rate > 0 else a.load * rate
+        lreqs = S.with_arrivals(reqs[:nlat], lam)
+        lat = S.run_stream(lreqs, arrs[:nlat], server=srv, timeout=1200, **served)
+        log(f"Poisson {lam:.1f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms")
+    finally:
+        srv.close()
+    # the plain path: one request at a time, FP64 values, tolerance 1e-12 (ARPACK semantics)
+    sub = list(range(0, K, max(1, a.baseline_every)))
+    breqs = [reqs[i] for i in sub]
+    base = S.run_stream(breqs, [arrs[i] for i in sub], slots=1, precision=CA.CF_F64, timeout=1200)
+    log(f"one-at-a-time FP64: {base['solves_per_s']:.2f} solves/s over {len(sub)} requests")
+    Eserved = np.array([out["results"][i]["energy"] for i in sub])
+    Ebase = np.array([d["energy"] for d in base["results"]])
+    dE = np.abs(Eserved - Ebase)
+    nmv_by_class = {}
+    for r, d in zip(breqs, base["results"]):
+        nmv_by_class.setdefault((r.kind, r.L), []).append(d["n_matvec"])
+    nmv_by_class = {k: float(np.mean(v)) for k, v in nmv_by_class.items()}
+    res = out["results"]
+    roof = None
+    if not a.no_roofline:
+        log("SpMV roofline (FCI L=12) ...")
+        roof = spmv_roofline(CA, CA.Context(0))
+        log(f"SpMV {roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}%)")
+    cpu = None
+    if not a.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline_stream(reqs, nmv_by_class)
+        log(f"cpu baseline: {cpu['value']:.3f} solves/s (sample {cpu['sample_s']:.1f} s)")
+    kinds = np.array([r.kind for r in reqs])
+    return {
+        "metric": METRIC,
+        "value": rate,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": out["wall_s"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-values/f64-accumulate+f64-polish",
+        "data": "synthetic",
+        "config": {"workload": ("served stream of rotated-integral ground-state solves: 70% DOCI L~U[12,20], "
+                                "30% FCI L~U[8,11] (on-the-fly sigma), half filling, seeds 1000+k, U=exp(K) "
+                                "K~N(0,0.05^2), FP32 values + FP64 polish, Kato-Temple energy stop 1e-9 Eh"),
+                   "requests": K, "doci": int((kinds == 1).sum()), "fci": int((kinds == 0).sum()),
+                   "slots": a.slots, "closed_batch": True,
+                   "inputs": "h, V, U copied from host memory at submission (1.3 MB per L=20 request)"},
+        "p50_ms": lat["p50_ms"],
+        "p99_ms": lat["p99_ms"],
+        "latency_run": {"arrivals": "Poisson", "rate_per_s": lam, "requests": nlat,
+                        "achieved_solves_per_s": lat["solves_per_s"], "mean_ms": lat["mean_ms"]},
+        "closed_batch_latency_ms": {"p50": out["p50_ms"], "p99": out["p99_ms"]},
+        "one_at_a_time_fp64": {"solves_per_s": base["solves_per_s"], "requests": len(sub),
+                               "sample": f"every {a.baseline_every}th request", "tol": 1e-12},
+        "speedup_vs_one_at_a_time_fp64": rate / base["solves_per_s"],
+        "max_abs_dE_vs_fp64": float(dE.max()),
+        "mean_matvec": {"f32": float(np.mean([d["n_matvec_f32"] for d in res])),
+                        "f64": float(np.mean([d["n_matvec"] for d in res]))},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_fci12(a, CA):
+    K = a.steps if a.steps is not None else 4
+    W = a.warmup if a.warmup is not None else 1
     ctx = CA.Context(0)
     h, V = CA.gen_integrals(a.L, a.seed)
     ham = CA.Hamiltonian(h, V, a.nel, a.nel, 0.0, ctx)
@@ -107,18 +292,16 @@ def main():
         info = cim.solve(precision=prec)
         return cim, info
 
-    for i in range(a.warmup):
+    for i in range(W):
         cim, info = step()
-        log(f"warmup {i}: E={cim.get_ci_energy():.12f} matvec={info.n_matvec}+{info.n_matvec_f32} "
-            f"ms={info.ms:.1f}")
+        log(f"warmup {i}: E={cim.get_ci_energy():.12f} matvec={info.n_matvec}+{info.n_matvec_f32} ms={info.ms:.1f}")
         del cim
     ctx.enable_timing(True)
     ctx.reset_timing()
-    lat = []
-    infos = []
+    lat, infos = [], []
     ctx.synchronize()
     t0 = time.perf_counter()
-    for i in range(a.steps):
+    for i in range(K):
         s0 = time.perf_counter()
         cim, info = step()
         ctx.synchronize()
@@ -127,16 +310,14 @@ def main():
         energy = cim.get_ci_energy()
         n, nnz_full, _ = cim.get_mat().info()
         log(f"step {i}: {lat[-1] * 1e3:.1f} ms E={energy:.12f} matvec={info.n_matvec}+{info.n_matvec_f32}")
-        if i + 1 < a.steps:
+        if i + 1 < K:
             del cim
     ctx.synchronize()
     total = time.perf_counter() - t0
-    fam = "spmv_f32" if prec == CA.CF_F32 else "spmv_f64"
     nl64, ms64 = ctx.kernel_time("spmv_f64")
     nl32, ms32 = ctx.kernel_time("spmv_f32")
     ncount, ms_count = ctx.kernel_time("crs_count")
     nfill, ms_fill = ctx.kernel_time("crs_fill")
-    # dominant kernel: the SpMV family with the larger total time
     if ms32 > ms64:
         fam, nl, ms, sval = "spmv_f32", nl32, ms32, 4
     else:
@@ -146,33 +327,30 @@ def main():
     algo_bytes = nnz_full * (sval + 4) + (n + 1) * 8 + 16 * n
     achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
     ctx.enable_timing(False)
-    # 2-RDM of the converged state (reported, outside the timed steps)
     r0 = time.perf_counter()
-    one, two = cim.construct_density(trdm=True)
+    cim.construct_density(trdm=True)
     rdm_ms = (time.perf_counter() - r0) * 1e3
-    L = a.L
-    tr = sum(cim.get_one_rdm(s, i, i) for s in range(2) for i in range(L))
-    n_matvec = infos[-1]["n_matvec"] + infos[-1]["n_matvec_f32"]
+    tr = sum(cim.get_one_rdm(s, i, i) for s in range(2) for i in range(a.L))
     cpu = None
     if not a.no_cpu_baseline:
         log("cpu baseline ...")
-        cpu = cpu_baseline(h, V, a.L, a.nel, nnz_upper, n, infos[-1]["n_matvec"], a.cpu_lines)
+        cpu = cpu_baseline_fci(h, V, a.L, a.nel, nnz_upper, n, infos[-1]["n_matvec"], a.cpu_lines)
         log(f"cpu baseline: {cpu['value']:.3e} solves/s (sample {cpu['sample_s']:.1f} s)")
     lat_ms = np.array(lat) * 1e3
-    out = {
-        "metric": "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of peak)",
-        "value": a.steps / total,
+    return {
+        "metric": METRIC,
+        "value": K / total,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": total / a.steps * 1e3,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": total / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if prec == CA.CF_F64 else "f32-values/f64-accumulate+f64-polish",
         "data": "synthetic",
-        "config": {"workload": f"FCI L={L} {a.nel}a{a.nel}b seed {a.seed}: GPU CRS build + Lanczos ground state",
+        "config": {"workload": f"FCI L={a.L} {a.nel}a{a.nel}b seed {a.seed}: GPU CRS build + Lanczos ground state",
                    "determinants": n, "nnz_full": nnz_full, "nnz_upper": nnz_upper, "tol": 1e-12,
                    "precision": a.precision},
         "p50_ms": float(np.percentile(lat_ms, 50)),
@@ -189,6 +367,17 @@ def main():
                      "launches": nl, "algorithmic_bytes": algo_bytes},
         "cpu_baseline": cpu,
     }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    import ciflow_amd as CA
+
+    out = bench_stream(a, CA) if a.workload == "stream" else bench_fci12(a, CA)
     print(json.dumps(out))
 
 

@@ -159,6 +159,7 @@ def lib():
         "cf_gen_rotation": (C.c_int, [C.c_int, C.c_uint64, C.c_double, dp]),
         "cf_sigma_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
         "cf_sigma_destroy": (C.c_int, [vp]),
+        "cf_sigma_set_threads": (C.c_int, [vp, C.c_int]),
         "cf_sigma_apply_dev": (C.c_int, [vp, C.c_int, vp, vp]),
         "cf_sigma_apply": (C.c_int, [vp, C.c_int, dp, dp]),
         "cf_solve_direct": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
@@ -559,6 +560,10 @@ class FCI_Direct(CIMethod):
 
     def get_name(self):
         return "FCI_Direct"
+
+    def set_threads(self, threads: int):
+        """Workgroup size of the alpha-beta sigma kernel (256, 512, 768, 1024)."""
+        _check(lib().cf_sigma_set_threads(self.sig.ptr, threads))
 
     def build_ci(self):
         raise CIFlowError(CF_ERR_STATE, "FCI_Direct stores no matrix")

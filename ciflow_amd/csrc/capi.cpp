@@ -81,7 +81,8 @@ struct cf_ctx {
     std::vector<hipEvent_t> pool;
     DBuf<double> hx, hy;  // staging for host-pointer products
     double* pinned = nullptr;  // kPinned doubles of page-locked memory for small read-backs
-    static constexpr int kPinned = 256;
+    // [0, 256): scalar read-backs; [256, 256 + 64*64): Ritz-vector coefficients
+    static constexpr int kPinned = 256 + 64 * 64;
     rocblas_handle blas_ = nullptr;  // created on first use, bound to the stream
 
     rocblas_handle blas() {
@@ -169,7 +170,6 @@ struct cf_ham {
     cf_ctx* ctx = nullptr;
     int L = 0, nup = 0, ndown = 0;
     double econst = 0.0;
-    std::vector<double> h, V;
     DBuf<double> dh, dV, dW, dJm;
     HamDev dev() const { return HamDev{L, dh.p, dV.p, dW.p, dJm.p}; }
 };
@@ -354,9 +354,13 @@ struct DeviceBackend : KrylovBackend {
     }
     void set_next(int j1, double beta) override { launch_scale_copy(n, w.p, 1.0 / beta, v(j1), c->stream); }
     void rotate(int m, int k, const double* Y) override {
-        CF_HIP(hipMemcpyAsync(Yd.p, Y, sizeof(double) * m * m, hipMemcpyHostToDevice, c->stream));
+        // staged through page-locked memory (m <= 63); the wait keeps the staging area
+        // valid until the copy has run
+        double* ys = hbuf + 256;
+        std::memcpy(ys, Y, sizeof(double) * m * m);
+        CF_HIP(hipMemcpyAsync(Yd.p, ys, sizeof(double) * m * m, hipMemcpyHostToDevice, c->stream));
         launch_rotate_basis(n, m, k, Vb.p, Yd.p, c->stream);
-        CF_HIP(hipStreamSynchronize(c->stream));  // Y is a host temporary
+        CF_HIP(hipStreamSynchronize(c->stream));
     }
     void move(int from, int to) override { launch_copy(n, v(from), v(to), c->stream); }
     void normalize0() override {
@@ -367,7 +371,8 @@ struct DeviceBackend : KrylovBackend {
     void rayleigh(double* theta, double* resid) override {
         dots(1, v(0), w.p, scal.p);
         double th = fetch(scal.p);
-        CF_HIP(hipMemcpyAsync(cvec.p, &th, sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hbuf[128] = th;  // page-locked source; the fetch below waits for the copy
+        CF_HIP(hipMemcpyAsync(cvec.p, hbuf + 128, sizeof(double), hipMemcpyHostToDevice, c->stream));
         launch_multi_axpy(n, 1, v(0), cvec.p, w.p, c->stream);  // w -= theta v0
         dots(1, w.p, w.p, scal.p);
         double r2 = fetch(scal.p);
@@ -578,8 +583,6 @@ int cf_ham_create(cf_ctx* ctx, int L, int nup, int ndown, const double* h, const
     H->ndown = ndown;
     H->econst = econst;
     const size_t L2 = (size_t)L * L, L3 = L2 * L, L4 = L3 * L;
-    H->h.assign(h, h + L2);
-    H->V.assign(V, V + L4);
     std::vector<double> W(L3), Jm(L2);
     for (int p = 0; p < L; ++p)
         for (int q = 0; q < L; ++q)
@@ -986,6 +989,7 @@ struct cf_sigma {
     DBuf<float> x32, y32;
     rocblas_handle blas = nullptr;  // the context's handle
     SigmaArgs args;
+    int threads = 1024;  // workgroup size of the alpha-beta kernel
 };
 
 namespace {
@@ -1013,14 +1017,14 @@ void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
         CF_BLAS(rocblas_sgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb32, m,
                               S->x32.p, m, &one, S->y32.p, m));
         launch_to_f64(sp->n, S->y32.p, y, c->stream);
-        launch_sigma_ab<float>(S->args, S->x32.p, y, c->stream);
+        launch_sigma_ab<float>(S->args, S->threads, S->x32.p, y, c->stream);
     } else {
         const double one = 1.0, zero = 0.0;
         CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, x, m,
                               S->Ha.p, nn, &zero, y, m));
         CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb, m, x,
                               m, &one, y, m));
-        launch_sigma_ab<double>(S->args, x, y, c->stream);
+        launch_sigma_ab<double>(S->args, S->threads, x, y, c->stream);
     }
 }
 }  // namespace
@@ -1051,8 +1055,8 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     CF_REQUIRE(sp && out, "NULL argument");
     CF_REQUIRE(sp->kind == CF_FCI, "the on-the-fly sigma is for FCI spaces");
     CF_REQUIRE(prec == CF_F64 || prec == CF_F32, "prec must be CF_F64 or CF_F32");
-    CF_REQUIRE(sp->ham->L <= 32 && sp->db < (1ull << 21) && sp->tb->S - 1 <= 64,
-               "on-the-fly sigma supports L <= 32, beta dimension < 2^21, <= 64 beta singles");
+    CF_REQUIRE(sp->ham->L <= 31 && sp->db < (1ull << 21) && sp->tb->S - 1 <= 64,
+               "on-the-fly sigma supports L <= 31, beta dimension < 2^21, <= 64 beta singles");
     CF_REQUIRE(sp->da <= 2147483647ull && sp->db <= 2147483647ull, "dimension too large for the GEMMs");
     cf_ctx* c = sp->ham->ctx;
     c->use();
@@ -1088,10 +1092,7 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     a.L = sp->ham->L;
     a.Sa = sp->ta->S;
     a.Sb = sp->tb->S;
-    a.nchunk = (uint32_t)((sp->db + 1023) / 1024);
-    const uint64_t nb = sp->da * a.nchunk;
-    CF_REQUIRE(nb < (1ull << 31), "too many sigma workgroups");
-    a.nblocks = (uint32_t)nb;
+    CF_REQUIRE(sp->da * ((sp->db + 255) / 256) < (1ull << 31), "too many sigma workgroups");
     a.sa = sp->ta->s.p;
     a.sb = sp->tb->s.p;
     a.suba = sp->ta->sub.p;
@@ -1100,6 +1101,15 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     CF_REQUIRE(sigma_ab_lds_bytes(a, false) <= 160 * 1024, "beta string space too large for the LDS row");
     CF_HIP(hipStreamSynchronize(c->stream));
     *out = S.release();
+    API_END
+}
+
+int cf_sigma_set_threads(cf_sigma* s, int threads) {
+    API_BEGIN
+    CF_REQUIRE(s, "NULL argument");
+    CF_REQUIRE(threads == 256 || threads == 512 || threads == 768 || threads == 1024,
+               "threads must be 256, 512, 768 or 1024");
+    s->threads = threads;
     API_END
 }
 
@@ -1317,11 +1327,7 @@ int cf_ham_rotate(const cf_ham* src, const double* U, cf_ham** out) {
         rotate_integrals_dev(L, 1, src->dh.p, src->dV.p, dU.p, H->dh.p, H->dV.p, tmp.p, ctx->stream);
     }
     launch_derived(L, H->dV.p, H->dW.p, H->dJm.p, ctx->stream);
-    H->h.resize(L2);
-    H->V.resize(L4);
-    CF_HIP(hipMemcpyAsync(H->h.data(), H->dh.p, L2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    CF_HIP(hipMemcpyAsync(H->V.data(), H->dV.p, L4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    CF_HIP(hipStreamSynchronize(ctx->stream));
+    CF_HIP(hipStreamSynchronize(ctx->stream));  // U is the caller's host array
     *out = H.release();
     API_END
 }
@@ -1329,9 +1335,10 @@ int cf_ham_rotate(const cf_ham* src, const double* U, cf_ham** out) {
 int cf_ham_get(const cf_ham* ham, double* h, double* V) {
     API_BEGIN
     CF_REQUIRE(ham, "ham is NULL");
+    ham->ctx->use();
     const size_t L2 = (size_t)ham->L * ham->L;
-    if (h) std::memcpy(h, ham->h.data(), L2 * sizeof(double));
-    if (V) std::memcpy(V, ham->V.data(), L2 * L2 * sizeof(double));
+    if (h) copy_sync(h, ham->dh.p, L2 * sizeof(double), hipMemcpyDeviceToHost, ham->ctx->stream);
+    if (V) copy_sync(V, ham->dV.p, L2 * L2 * sizeof(double), hipMemcpyDeviceToHost, ham->ctx->stream);
     API_END
 }
 

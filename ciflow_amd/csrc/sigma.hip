@@ -49,13 +49,41 @@ void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream
 template void launch_dense_string_ham<double>(uint64_t, int, const Exc*, double*, hipStream_t);
 template void launch_dense_string_ham<float>(uint64_t, int, const Exc*, float*, hipStream_t);
 
-constexpr int kSigThreads = 1024;
-constexpr int kMaxSingles = 64;  // n (L - n) <= 64
+constexpr int kSigMaxThreads = 1024;
+constexpr int kMaxSingles = 64;  // n (L - n) <= 64; L <= 31 (q = L marks the zero slot)
+
+// Staged row elements per thread held in registers while the previous action
+// computes (db <= NT * kMaxPrefetch uses a prefetching kernel).
+constexpr int kMaxPrefetch = 16;
+
+// V slice of one alpha action, staged in LDS: self -> sum over p in Ia of
+// V[p][q][p][s]; single p -> r -> s_a V[p][q][r][s].
+template <int NT>
+__device__ __forceinline__ void stage_vslice(const SigmaArgs& a, const Exc& x, str_t Sa, double* Vs) {
+    const int L = a.L;
+    if (threadIdx.x == 0) Vs[L * L] = 0.0;
+    for (int i = threadIdx.x; i < L * L; i += NT) {
+        const int q = i / L, s = i % L;
+        double v = 0.0;
+        if (x.type == 0) {
+            for (str_t t = Sa; t; t &= t - 1) {
+                const int p = __builtin_ctzll(t);
+                v += a.V[(((size_t)p * L + q) * L + p) * L + s];
+            }
+        } else {
+            v = a.V[(((size_t)x.f * L + q) * L + x.t) * L + s];
+            if (x.sign < 0) v = -v;
+        }
+        Vs[i] = v;
+    }
+}
 
 // sigma_ab += ..., CT = double or float (staged C precision); accumulation FP64.
-template <typename CT>
-__global__ void __launch_bounds__(kSigThreads) k_sigma_ab(SigmaArgs a, const CT* __restrict__ C,
-                                                         double* __restrict__ sig) {
+// The next action's C row is loaded into registers (PF elements per thread)
+// while the current one is processed from LDS, so the row fetch overlaps the
+// gathers; PF = 0 stages each row after the previous action has finished.
+template <typename CT, int PF, int NT>
+__global__ void __launch_bounds__(NT) k_sigma_ab(SigmaArgs a, const CT* __restrict__ C, double* __restrict__ sig) {
     extern __shared__ unsigned char smem[];
     const uint64_t db = a.db;
     const uint32_t nchunk = a.nchunk;
@@ -63,55 +91,66 @@ __global__ void __launch_bounds__(kSigThreads) k_sigma_ab(SigmaArgs a, const CT*
     const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
     if (lb >= a.nblocks) return;
     const uint64_t Ia = lb / nchunk;
-    const uint64_t Ib = (uint64_t)(lb - Ia * nchunk) * kSigThreads + threadIdx.x;
+    const uint64_t Ib = (uint64_t)(lb - Ia * nchunk) * NT + threadIdx.x;
     const bool act = Ib < db;
     const int L = a.L;
-    double* Vs = (double*)smem;             // L*L
-    CT* row = (CT*)(Vs + L * L);            // db
-    // this thread's beta actions: singles (packed) + number operators (bits of Ib)
-    uint32_t sb[kMaxSingles];
-    const int nsb = a.Sb - 1;
+    double* Vs = (double*)smem;             // L*L, then a zero at [L*L] (and one pad slot)
+    CT* row = (CT*)(Vs + L * L + 2);        // db
+    // this thread's beta actions, packed (J << 11 | q << 6 | s << 1 | sign) in
+    // list order; the self entry points at the zero slot (q = L, s = 0) and the
+    // number operators are summed separately from the bits of Ib.  Every index
+    // is a compile-time constant, so the list stays in registers.
+    uint32_t sb[kMaxSingles + 1];
+    const int Sb = a.Sb;
     str_t sbits = 0;
     if (act) {
         sbits = a.sb[Ib];
-        const Exc* lb_ = a.subb + Ib * (uint64_t)a.Sb;
-        int w = 0;
+        const Exc* lb_ = a.subb + Ib * (uint64_t)Sb;
 #pragma unroll
         for (int e = 0; e < kMaxSingles + 1; ++e) {
-            if (e < a.Sb) {
+            if (e < Sb) {
                 const Exc x = lb_[e];
-                if (x.type == 1 && w < kMaxSingles) {
-                    sb[w] = (x.J << 11) | ((uint32_t)x.f << 6) | ((uint32_t)x.t << 1) | (x.sign < 0 ? 1u : 0u);
-                    ++w;
-                }
+                sb[e] = x.type == 1
+                            ? (x.J << 11) | ((uint32_t)x.f << 6) | ((uint32_t)x.t << 1) | (x.sign < 0 ? 1u : 0u)
+                            : ((uint32_t)Ib << 11) | ((uint32_t)L << 6);
             }
         }
     }
     const Exc* la = a.suba + Ia * (uint64_t)a.Sa;
     const str_t Sa = a.sa[Ia];
+    CT pf[PF > 0 ? PF : 1];
+    if (PF > 0) {
+        const CT* src = C + (uint64_t)la[0].J * db;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const uint64_t i = (uint64_t)k * NT + threadIdx.x;
+            pf[k] = i < db ? src[i] : CT(0);
+        }
+    }
     double acc = 0.0;
     for (int e = 0; e < a.Sa; ++e) {
         const Exc x = la[e];
-        const uint64_t Ja = x.J;
         __syncthreads();
-        // V slice: self -> sum over p in Ia of V[p][q][p][s]; single p->r -> s_a V[p][q][r][s]
-        for (int i = threadIdx.x; i < L * L; i += kSigThreads) {
-            const int q = i / L, s = i % L;
-            double v = 0.0;
-            if (x.type == 0) {
-                for (str_t t = Sa; t; t &= t - 1) {
-                    const int p = __builtin_ctzll(t);
-                    v += a.V[(((size_t)p * L + q) * L + p) * L + s];
-                }
-            } else {
-                v = a.V[(((size_t)x.f * L + q) * L + x.t) * L + s];
-                if (x.sign < 0) v = -v;
+        stage_vslice<NT>(a, x, Sa, Vs);
+        if (PF > 0) {
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const uint64_t i = (uint64_t)k * NT + threadIdx.x;
+                if (i < db) row[i] = pf[k];
             }
-            Vs[i] = v;
+        } else {
+            const CT* src = C + (uint64_t)x.J * db;
+            for (uint64_t i = threadIdx.x; i < db; i += NT) row[i] = src[i];
         }
-        const CT* src = C + Ja * db;
-        for (uint64_t i = threadIdx.x; i < db; i += kSigThreads) row[i] = src[i];
         __syncthreads();
+        if (PF > 0 && e + 1 < a.Sa) {
+            const CT* src = C + (uint64_t)la[e + 1].J * db;
+#pragma unroll
+            for (int k = 0; k < PF; ++k) {
+                const uint64_t i = (uint64_t)k * NT + threadIdx.x;
+                pf[k] = i < db ? src[i] : CT(0);
+            }
+        }
         if (act) {
             double t = 0.0;
             // beta number operators: J_b = Ib
@@ -122,8 +161,8 @@ __global__ void __launch_bounds__(kSigThreads) k_sigma_ab(SigmaArgs a, const CT*
             }
             t += dsum * (double)row[Ib];
 #pragma unroll
-            for (int j = 0; j < kMaxSingles; ++j) {
-                if (j < nsb) {
+            for (int j = 0; j < kMaxSingles + 1; ++j) {
+                if (j < Sb) {
                     const uint32_t c = sb[j];
                     const int q = (c >> 6) & 31, s = (c >> 1) & 31;
                     double v = Vs[q * L + s] * (double)row[c >> 11];
@@ -137,18 +176,39 @@ __global__ void __launch_bounds__(kSigThreads) k_sigma_ab(SigmaArgs a, const CT*
 }
 
 size_t sigma_ab_lds_bytes(const SigmaArgs& a, bool f32) {
-    return (size_t)a.L * a.L * sizeof(double) + a.db * (f32 ? sizeof(float) : sizeof(double));
+    return ((size_t)a.L * a.L + 2) * sizeof(double) + a.db * (f32 ? sizeof(float) : sizeof(double));
+}
+
+template <typename CT, int NT>
+static void launch_nt(SigmaArgs a, const CT* C, double* sig, hipStream_t s) {
+    a.nchunk = (uint32_t)((a.db + NT - 1) / NT);
+    const uint64_t nb = a.da * (uint64_t)a.nchunk;
+    if (nb > 0xFFFFFFF0ull) throw Error(ERR_UNSUPPORTED, "too many sigma workgroups");
+    a.nblocks = (uint32_t)nb;
+    const uint64_t grid = (nb + 7) / 8 * 8;
+    const size_t lds = sigma_ab_lds_bytes(a, sizeof(CT) == 4);
+    const uint64_t per_thread = (a.db + NT - 1) / NT;
+    if (per_thread <= 4)
+        k_sigma_ab<CT, 4, NT><<<(unsigned)grid, NT, lds, s>>>(a, C, sig);
+    else if (per_thread <= kMaxPrefetch)
+        k_sigma_ab<CT, kMaxPrefetch, NT><<<(unsigned)grid, NT, lds, s>>>(a, C, sig);
+    else
+        k_sigma_ab<CT, 0, NT><<<(unsigned)grid, NT, lds, s>>>(a, C, sig);
+    CF_HIP(hipGetLastError());
 }
 
 template <typename CT>
-void launch_sigma_ab(const SigmaArgs& a, const CT* C, double* sig, hipStream_t s) {
-    const uint64_t grid = (a.nblocks + 7) / 8 * 8;
-    const size_t lds = sigma_ab_lds_bytes(a, sizeof(CT) == 4);
-    k_sigma_ab<CT><<<(unsigned)grid, kSigThreads, lds, s>>>(a, C, sig);
-    CF_HIP(hipGetLastError());
+void launch_sigma_ab(const SigmaArgs& a, int threads, const CT* C, double* sig, hipStream_t s) {
+    switch (threads) {
+        case 256: launch_nt<CT, 256>(a, C, sig, s); break;
+        case 512: launch_nt<CT, 512>(a, C, sig, s); break;
+        case 768: launch_nt<CT, 768>(a, C, sig, s); break;
+        case 1024: launch_nt<CT, 1024>(a, C, sig, s); break;
+        default: throw Error(ERR_ARG, "sigma workgroup size must be 256, 512, 768 or 1024");
+    }
 }
-template void launch_sigma_ab<double>(const SigmaArgs&, const double*, double*, hipStream_t);
-template void launch_sigma_ab<float>(const SigmaArgs&, const float*, double*, hipStream_t);
+template void launch_sigma_ab<double>(const SigmaArgs&, int, const double*, double*, hipStream_t);
+template void launch_sigma_ab<float>(const SigmaArgs&, int, const float*, double*, hipStream_t);
 
 __global__ void k_to_f32(uint64_t n, const double* __restrict__ x, float* __restrict__ y) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

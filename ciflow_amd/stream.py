@@ -18,7 +18,7 @@ from __future__ import annotations
 import concurrent.futures as cf
 import threading
 import time
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 
 import numpy as np
 
@@ -49,6 +49,14 @@ def make_stream(count: int, rate: float = 0.0, frac_doci: float = 0.7, doci_L=(1
         L = int(Ld[k] if is_doci[k] else Lf[k])
         out.append(StreamRequest(k, CF_DOCI if is_doci[k] else CF_FCI, L, L // 2, first_seed + k, float(t[k])))
     return out
+
+
+def with_arrivals(reqs: list[StreamRequest], rate: float, seed: int = 7) -> list[StreamRequest]:
+    """The same requests with Poisson arrival times at `rate` per second (first at t = 0)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, len(reqs)) if rate > 0 else np.zeros(len(reqs))
+    t = np.cumsum(gaps) - (gaps[0] if len(reqs) else 0.0)
+    return [replace(r, t_arrive=float(ti)) for r, ti in zip(reqs, t)]
 
 
 def request_arrays(r: StreamRequest, sigma: float = 0.05):
@@ -119,4 +127,5 @@ def run_stream(reqs: list[StreamRequest], arrays: list[tuple], slots: int = 4, p
             "mean_ms": float(lat.mean())}
 
 
-__all__ = ["StreamRequest", "make_stream", "request_arrays", "prepare", "run_stream", "CF_F32", "CF_F64"]
+__all__ = ["StreamRequest", "make_stream", "with_arrivals", "request_arrays", "prepare", "run_stream", "CF_F32",
+           "CF_F64"]

@@ -178,6 +178,9 @@ int cf_space_eigvec(cf_space* sp, double* eigvec);
 typedef struct cf_sigma cf_sigma;
 int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out);
 int cf_sigma_destroy(cf_sigma* s);
+/* Workgroup size of the alpha-beta kernel: 256, 512, 768 or 1024 beta strings
+ * per workgroup (tuning; no reference counterpart). */
+int cf_sigma_set_threads(cf_sigma* s, int threads);
 /* y = H x, device pointers on the context stream; value_prec CF_F32 uses
  * FP32 operands with FP64 accumulation of the alpha-beta part. */
 int cf_sigma_apply_dev(cf_sigma* s, int value_prec, const double* dx, double* dy);
