@@ -1086,6 +1086,9 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
         S->y32.alloc(sp->n);
     }
     S->blas = c->blas();
+    // measured on MI355X: 512 beta strings per workgroup up to L = 14 (d_beta <= 3432),
+    // 768 above (L = 16: 771 ms against 1458 ms at 1024)
+    S->threads = sp->db <= 4096 ? 512 : 768;
     SigmaArgs& a = S->args;
     a.da = sp->da;
     a.db = sp->db;
