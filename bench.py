@@ -39,7 +39,7 @@ METRIC = "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of pe
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["stream", "fci12"], default="stream")
+    p.add_argument("--workload", choices=["stream", "fci12", "direct16"], default="stream")
     p.add_argument("--steps", type=int, default=None, help="stream: requests (4096); fci12: solves (4)")
     p.add_argument("--warmup", type=int, default=None, help="stream: warm-up requests (64); fci12: solves (1)")
     p.add_argument("--slots", type=int, default=8)
@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--precision", choices=["f64", "f32"], default="f64")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
+    p.add_argument("--no-fp64", action="store_true", help="direct16: skip the FP64 solve")
     p.add_argument("--cpu-lines", type=int, default=6, help="alpha lines in the FCI CPU-baseline sample")
     return p.parse_args()
 
@@ -369,6 +370,59 @@ def bench_fci12(a, CA):
     }
 
 
+def bench_direct16(a, CA):
+    """FCI L=16 8a8b seed 3 (165,636,900 determinants) on the on-the-fly sigma:
+    FP64 solve and FP32-sigma + FP64 polish, at most 40 Krylov vectors, then
+    the 1-RDM of the ground state.  One step = one FP32+polish solve."""
+    L, nel, seed = 16, 8, 3
+    ctx = CA.Context(0)
+    h, V = CA.gen_integrals(L, seed)
+    ham = CA.Hamiltonian(h, V, nel, nel, 0.0, ctx)
+    t0 = time.perf_counter()
+    d = CA.FCI_Direct(ham, value_prec=CA.CF_F32)
+    ctx.synchronize()
+    t_setup = time.perf_counter() - t0
+    n = d.n
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    log(f"direct FCI L={L}: n={n}, setup {t_setup:.2f} s; FP32+polish solve ...")
+    t0 = time.perf_counter()
+    i32 = d.solve(precision=CA.CF_F32, ncv=39)
+    t32 = time.perf_counter() - t0
+    e32 = d.get_ci_energy()
+    log(f"  E={e32:.12f} matvec {i32.n_matvec}+{i32.n_matvec_f32} {t32:.1f} s")
+    n64, ms64 = ctx.kernel_time("sigma_f64")
+    n32, ms32 = ctx.kernel_time("sigma_f32")
+    t0 = time.perf_counter()
+    one, _ = d.construct_density(trdm=False)
+    t_rdm = time.perf_counter() - t0
+    tr = float(sum(d.get_one_rdm(s_, i, i) for s_ in range(2) for i in range(L)))
+    out64 = None
+    if not a.no_fp64:
+        log("  FP64 solve ...")
+        ctx.reset_timing()
+        t0 = time.perf_counter()
+        i64 = d.solve(precision=CA.CF_F64, ncv=39)
+        t64 = time.perf_counter() - t0
+        e64 = d.get_ci_energy()
+        n64b, ms64b = ctx.kernel_time("sigma_f64")
+        out64 = {"energy": e64, "n_matvec": i64.n_matvec, "seconds": t64, "sigma_ms": ms64b / max(n64b, 1),
+                 "dE_f32_polish_vs_f64": abs(e32 - e64)}
+        log(f"  E={e64:.12f} matvec {i64.n_matvec} {t64:.1f} s")
+    ctx.enable_timing(False)
+    return {
+        "metric": METRIC, "value": 1.0 / t32, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": t32 * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-sigma/f64-accumulate+f64-polish", "data": "synthetic",
+        "config": {"workload": f"direct FCI L={L} {nel}a{nel}b seed {seed}, on-the-fly sigma, ncv 39",
+                   "determinants": n},
+        "energy": e32, "n_matvec": i32.n_matvec, "n_matvec_f32": i32.n_matvec_f32,
+        "sigma_ms": {"f32": ms32 / max(n32, 1), "f64": ms64 / max(n64, 1)},
+        "setup_s": t_setup, "rdm1_s": t_rdm, "rdm1_trace": tr, "fp64": out64,
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -377,7 +431,8 @@ def main():
         sys.exit(2)
     import ciflow_amd as CA
 
-    out = bench_stream(a, CA) if a.workload == "stream" else bench_fci12(a, CA)
+    run = {"stream": bench_stream, "fci12": bench_fci12, "direct16": bench_direct16}[a.workload]
+    out = run(a, CA)
     print(json.dumps(out))
 
 

@@ -144,6 +144,7 @@ def lib():
         "cf_crs_export_full": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p, u32p, dp]),
         "cf_mvprod": (C.c_int, [vp, dp, dp]),
         "cf_mvprod_dev": (C.c_int, [vp, C.c_int, vp, vp]),
+        "cf_mvprod_prec": (C.c_int, [vp, C.c_int, dp, dp]),
         "cf_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
         "cf_solve": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
         "cf_rci_create": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_uint64, C.POINTER(vp)]),
@@ -391,10 +392,14 @@ class SparseMatrix_CRS_Sym:
         """0 automatic, 1 plain row gather, 2 FCI alpha-block staged."""
         _check(lib().cf_crs_set_spmv_kernel(self.ptr, kind, tile_rows))
 
-    def mvprod(self, x: np.ndarray) -> np.ndarray:
+    def mvprod(self, x: np.ndarray, precision: int = CF_F64) -> np.ndarray:
+        """y = A x (SparseMatrix_CRS_Sym::mvprod); precision CF_F32 uses the FP32 values."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)
-        _check(lib().cf_mvprod(self.ptr, _d(x), _d(y)))
+        if precision == CF_F64:
+            _check(lib().cf_mvprod(self.ptr, _d(x), _d(y)))
+        else:
+            _check(lib().cf_mvprod_prec(self.ptr, precision, _d(x), _d(y)))
         return y
 
     def arpackDiagonalize(self, opts: SolveOpts | None = None, want_vec: bool = True):
@@ -562,7 +567,7 @@ class FCI_Direct(CIMethod):
         return "FCI_Direct"
 
     def set_threads(self, threads: int):
-        """Workgroup size of the alpha-beta sigma kernel (256, 512, 768, 1024)."""
+        """Workgroup size of the alpha-beta sigma kernel (128, 256, 512, 768, 1024)."""
         _check(lib().cf_sigma_set_threads(self.sig.ptr, threads))
 
     def build_ci(self):

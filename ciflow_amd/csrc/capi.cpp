@@ -203,8 +203,10 @@ struct cf_crs {
     DBuf<uint32_t> col;
     DBuf<double> val;
     DBuf<float> val32;
-    int lpr = 64;
-    int kernel = 0;  // 0 auto, 1 row gather, 2 FCI staged
+    int lpr = 64;     // lanes per row of the scalar row gather
+    int kernel = 0;   // 0 auto, 1 row gather, 2 FCI staged, 3 vectorised row gather
+    // automatic choice per value precision: vectorised (lanes, unroll) or 0 = scalar gather
+    int v4_lpr64 = 0, v4_unr64 = 0, v4_lpr32 = 0, v4_unr32 = 0;
     std::unique_ptr<FciSpmvPlan> plan;
 };
 
@@ -242,14 +244,22 @@ void ensure_staging(cf_ctx* c, uint64_t n) {
 void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
     cf_ctx* c = m->sp->ham->ctx;
     const bool staged = m->kernel == 2 && m->plan;
+    const bool v4_64 = m->kernel == 3 || (m->kernel == 0 && m->v4_lpr64);
+    const bool v4_32 = m->kernel == 3 || (m->kernel == 0 && m->v4_lpr32);
     if (prec == CF_F32) {
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
         if (staged) launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream);
+        else if (v4_32)
+            launch_spmv_v4_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->v4_lpr32 ? m->v4_lpr32 : 16,
+                               m->v4_unr32 ? m->v4_unr32 : 2, c->stream);
         else launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->lpr, c->stream);
     } else {
         KTimer t(c, "spmv_f64");
         if (staged) launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream);
+        else if (v4_64)
+            launch_spmv_v4_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->v4_lpr64 ? m->v4_lpr64 : 16,
+                               m->v4_unr64 ? m->v4_unr64 : 2, c->stream);
         else launch_spmv_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->lpr, c->stream);
     }
 }
@@ -286,6 +296,7 @@ struct DeviceBackend : KrylovBackend {
     uint64_t n;
     int M;
     DBuf<double> Vb, w, partial, cvec, c2, Yd, scal, dalpha, dbeta;
+    DBuf<unsigned> counter;  // last-block counters of the fused multi-dot (left zero)
     int nb;
     std::function<void(const double*, double*)> op;
     double* hbuf;  // the context's pinned read-back buffer
@@ -296,6 +307,8 @@ struct DeviceBackend : KrylovBackend {
         w.alloc(n);
         nb = multi_dot_blocks(n);
         partial.alloc((size_t)64 * nb);
+        counter.alloc(8);
+        CF_HIP(hipMemsetAsync(counter.p, 0, 8 * sizeof(unsigned), c->stream));
         cvec.alloc(64);
         c2.alloc(64);
         Yd.alloc(64 * 64);
@@ -306,8 +319,7 @@ struct DeviceBackend : KrylovBackend {
     }
     double* v(int i) { return Vb.p + (size_t)i * n; }
     void dots(int k, const double* base, const double* x, double* out) {
-        launch_multi_dot(n, k, base, x, partial.p, nb, c->stream);
-        launch_reduce_partials(k, nb, partial.p, out, c->stream);
+        launch_multi_dot_fused(n, k, base, x, partial.p, nb, counter.p, out, c->stream);
     }
     double fetch(const double* d) {
         CF_HIP(hipMemcpyAsync(hbuf, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -775,6 +787,20 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     }
     CF_HIP(hipStreamSynchronize(c->stream));
     m->lpr = pick_lanes_per_row(m->n, m->nnz);
+    // measured on MI355X (FCI L=11/12, DOCI L=20/24): vectorised streams win for
+    // FP32 values at every row length and for FP64 rows of >= 512 entries
+    const double avg = m->n ? (double)m->nnz / (double)m->n : 0.0;
+    if (avg >= 512) {
+        m->v4_lpr64 = 16;
+        m->v4_unr64 = 2;
+    }
+    if (avg >= 128) {
+        m->v4_lpr32 = 8;
+        m->v4_unr32 = 2;
+    } else if (avg >= 24) {
+        m->v4_lpr32 = 16;
+        m->v4_unr32 = 1;
+    }
     *out = m.release();
     API_END
 }
@@ -888,10 +914,24 @@ int cf_mvprod(cf_crs* m, const double* x, double* y) {
     API_END
 }
 
+int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y) {
+    API_BEGIN
+    CF_REQUIRE(m && x && y, "NULL argument");
+    CF_REQUIRE(value_prec == CF_F64 || value_prec == CF_F32, "value_prec must be CF_F64 or CF_F32");
+    cf_ctx* c = m->sp->ham->ctx;
+    c->use();
+    ensure_staging(c, m->n);
+    CF_HIP(hipMemcpyAsync(c->hx.p, x, m->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    spmv(m, value_prec, c->hx.p, c->hy.p);
+    CF_HIP(hipMemcpyAsync(y, c->hy.p, m->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipStreamSynchronize(c->stream));
+    API_END
+}
+
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
     API_BEGIN
     CF_REQUIRE(m, "matrix is NULL");
-    CF_REQUIRE(kind >= 0 && kind <= 2, "kind must be 0, 1 or 2");
+    CF_REQUIRE(kind >= 0 && kind <= 3, "kind must be 0, 1, 2 or 3");
     m->sp->ham->ctx->use();
     if (kind == 2) {
         CF_REQUIRE(m->sp->kind == CF_FCI, "the staged SpMV needs an FCI matrix");
@@ -1087,8 +1127,9 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     }
     S->blas = c->blas();
     // measured on MI355X: 512 beta strings per workgroup up to L = 14 (d_beta <= 3432),
-    // 768 above (L = 16: 771 ms against 1458 ms at 1024)
-    S->threads = sp->db <= 4096 ? 512 : 768;
+    // 768 above (L = 16: 771 ms against 1458 ms at 1024); small beta spaces use the
+    // smallest workgroup that holds them in one chunk
+    S->threads = sp->db <= 128 ? 128 : sp->db <= 256 ? 256 : sp->db <= 4096 ? 512 : 768;
     SigmaArgs& a = S->args;
     a.da = sp->da;
     a.db = sp->db;
@@ -1110,8 +1151,8 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
 int cf_sigma_set_threads(cf_sigma* s, int threads) {
     API_BEGIN
     CF_REQUIRE(s, "NULL argument");
-    CF_REQUIRE(threads == 256 || threads == 512 || threads == 768 || threads == 1024,
-               "threads must be 256, 512, 768 or 1024");
+    CF_REQUIRE(threads == 128 || threads == 256 || threads == 512 || threads == 768 || threads == 1024,
+               "threads must be 128, 256, 512, 768 or 1024");
     s->threads = threads;
     API_END
 }

@@ -98,12 +98,21 @@ void launch_spmv_f64(uint64_t n, const int64_t* rowptr, const uint32_t* col, con
                      double* y, int lanes_per_row, hipStream_t stream);
 void launch_spmv_f32(uint64_t n, const int64_t* rowptr, const uint32_t* col, const float* val, const double* x,
                      double* y, int lanes_per_row, hipStream_t stream);
+// Vectorised-stream row gather: (lanes_per_row, unroll) in {(16,2), (8,2), (16,1)}.
+void launch_spmv_v4_f64(uint64_t n, const int64_t* rowptr, const uint32_t* col, const double* val, const double* x,
+                        double* y, int lanes_per_row, int unroll, hipStream_t stream);
+void launch_spmv_v4_f32(uint64_t n, const int64_t* rowptr, const uint32_t* col, const float* val, const double* x,
+                        double* y, int lanes_per_row, int unroll, hipStream_t stream);
 
 // Krylov vector kernels (vectors are columns of a column-major n x k block).
 void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
                       hipStream_t stream);
 void launch_reduce_partials(int k, int nblocks, const double* partial, double* out, hipStream_t stream);
 int multi_dot_blocks(uint64_t n);
+// out[i] = Vb[:, i] . w for i < k (k <= 64), partials + last-block reduction in
+// one launch; counter: 8 zero-initialised words, left zero on return.
+void launch_multi_dot_fused(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
+                            unsigned* counter, double* out, hipStream_t stream);
 void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream);
 void launch_finish_step(uint64_t n, int j, const double* c1, const double* c2, const double* wnorm2,
                         const double* w, double* vnext, double* alpha, double* beta, hipStream_t stream);

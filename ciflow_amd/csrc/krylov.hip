@@ -10,9 +10,9 @@ constexpr int kDotChunk = 8;  // vectors per block row
 constexpr int kDotThreads = 256;
 
 int multi_dot_blocks(uint64_t n) {
-    uint64_t nb = (n + 4095) / 4096;
+    uint64_t nb = (n + 2047) / 2048;
     if (nb < 1) nb = 1;
-    if (nb > 512) nb = 512;
+    if (nb > 1024) nb = 1024;
     return (int)nb;
 }
 
@@ -49,6 +49,68 @@ __global__ void __launch_bounds__(kDotThreads) k_multi_dot(uint64_t n, int k, co
         for (int q = 0; q < kDotThreads / 64; ++q) s += red[threadIdx.x][q];
         partial[(uint64_t)(i0 + threadIdx.x) * gridDim.x + blockIdx.x] = s;
     }
+}
+
+// Multi-dot with the final reduction in the same launch: every block writes its
+// partials, and the last block of each row of 8 vectors (found with a counter,
+// which it resets for the next launch) sums them in block order, so the result
+// does not depend on which block finishes last.
+__global__ void __launch_bounds__(kDotThreads) k_multi_dot_fused(uint64_t n, int k, const double* __restrict__ Vb,
+                                                                 const double* __restrict__ w, double* partial,
+                                                                 unsigned* counter, double* __restrict__ out) {
+    __shared__ double red[kDotChunk][kDotThreads / 64];
+    __shared__ bool last;
+    const int i0 = blockIdx.y * kDotChunk;
+    const int nb = gridDim.x;
+    double acc[kDotChunk];
+#pragma unroll
+    for (int j = 0; j < kDotChunk; ++j) acc[j] = 0.0;
+    const uint64_t stride = (uint64_t)nb * kDotThreads;
+    for (uint64_t q = (uint64_t)blockIdx.x * kDotThreads + threadIdx.x; q < n; q += stride) {
+        const double wq = w[q];
+#pragma unroll
+        for (int j = 0; j < kDotChunk; ++j)
+            if (i0 + j < k) acc[j] += Vb[(uint64_t)(i0 + j) * n + q] * wq;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kDotChunk; ++j) {
+        double s = wave_sum(acc[j]);
+        if (lane == 0) red[j][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kDotChunk && i0 + (int)threadIdx.x < k) {
+        double s = 0.0;
+        for (int q = 0; q < kDotThreads / 64; ++q) s += red[threadIdx.x][q];
+        partial[(uint64_t)(i0 + threadIdx.x) * nb + blockIdx.x] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned prev = atomicAdd(&counter[blockIdx.y], 1u);
+        last = prev == (unsigned)nb - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // 4 waves: wave j handles vectors j and j+4 of this row, lanes stride the blocks
+    for (int jj = wid; jj < kDotChunk; jj += kDotThreads / 64) {
+        const int i = i0 + jj;
+        if (i >= k) continue;
+        double s = 0.0;
+        for (int b = lane; b < nb; b += 64) s += partial[(uint64_t)i * nb + b];
+        s = wave_sum(s);
+        if (lane == 0) out[i] = s;
+    }
+    if (threadIdx.x == 0) counter[blockIdx.y] = 0u;
+}
+
+void launch_multi_dot_fused(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
+                            unsigned* counter, double* out, hipStream_t stream) {
+    if (k <= 0) return;
+    dim3 grid((unsigned)nblocks, (unsigned)((k + kDotChunk - 1) / kDotChunk));
+    k_multi_dot_fused<<<grid, kDotThreads, 0, stream>>>(n, k, Vb, w, partial, counter, out);
+    CF_HIP(hipGetLastError());
 }
 
 void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,

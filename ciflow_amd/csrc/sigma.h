@@ -19,7 +19,7 @@ struct SigmaArgs {
 template <typename T>
 void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream_t s);
 size_t sigma_ab_lds_bytes(const SigmaArgs& a, bool f32);
-// threads: workgroup size of the alpha-beta kernel (256, 512, 768 or 1024);
+// threads: workgroup size of the alpha-beta kernel (128, 256, 512, 768 or 1024);
 // a workgroup owns one alpha string and `threads` beta strings.
 template <typename CT>
 void launch_sigma_ab(const SigmaArgs& a, int threads, const CT* C, double* sig, hipStream_t s);

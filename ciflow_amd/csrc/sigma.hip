@@ -49,7 +49,6 @@ void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream
 template void launch_dense_string_ham<double>(uint64_t, int, const Exc*, double*, hipStream_t);
 template void launch_dense_string_ham<float>(uint64_t, int, const Exc*, float*, hipStream_t);
 
-constexpr int kSigMaxThreads = 1024;
 constexpr int kMaxSingles = 64;  // n (L - n) <= 64; L <= 31 (q = L marks the zero slot)
 
 // Staged row elements per thread held in registers while the previous action
@@ -200,11 +199,12 @@ static void launch_nt(SigmaArgs a, const CT* C, double* sig, hipStream_t s) {
 template <typename CT>
 void launch_sigma_ab(const SigmaArgs& a, int threads, const CT* C, double* sig, hipStream_t s) {
     switch (threads) {
+        case 128: launch_nt<CT, 128>(a, C, sig, s); break;
         case 256: launch_nt<CT, 256>(a, C, sig, s); break;
         case 512: launch_nt<CT, 512>(a, C, sig, s); break;
         case 768: launch_nt<CT, 768>(a, C, sig, s); break;
         case 1024: launch_nt<CT, 1024>(a, C, sig, s); break;
-        default: throw Error(ERR_ARG, "sigma workgroup size must be 256, 512, 768 or 1024");
+        default: throw Error(ERR_ARG, "sigma workgroup size must be 128, 256, 512, 768 or 1024");
     }
 }
 template void launch_sigma_ab<double>(const SigmaArgs&, int, const double*, double*, hipStream_t);

@@ -129,8 +129,10 @@ int cf_crs_export_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t
  *      arpackDiagonalize calls (:446).  y is overwritten. */
 int cf_mvprod(cf_crs* m, const double* x, double* y);                       /* host pointers, synchronous */
 int cf_mvprod_dev(cf_crs* m, int value_prec, const double* dx, double* dy);  /* device pointers, ctx stream */
+int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* host pointers, CF_F64 or CF_F32 values */
 /* SpMV kernel selection (no reference counterpart): 0 = automatic, 1 = plain
- * row gather, 2 = FCI alpha-block staged (FCI matrices only). */
+ * row gather, 2 = FCI alpha-block staged (FCI matrices only), 3 = row gather
+ * with vectorised column/value streams. */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
 
 /* ---- ground state: CIMethod::solve -> SparseMatrix_CRS::arpackDiagonalize
@@ -178,7 +180,7 @@ int cf_space_eigvec(cf_space* sp, double* eigvec);
 typedef struct cf_sigma cf_sigma;
 int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out);
 int cf_sigma_destroy(cf_sigma* s);
-/* Workgroup size of the alpha-beta kernel: 256, 512, 768 or 1024 beta strings
+/* Workgroup size of the alpha-beta kernel: 128, 256, 512, 768 or 1024 beta strings
  * per workgroup (tuning; no reference counterpart). */
 int cf_sigma_set_threads(cf_sigma* s, int threads);
 /* y = H x, device pointers on the context stream; value_prec CF_F32 uses

@@ -320,6 +320,34 @@ def test_staged_spmv_falls_back_on_dropped_entries():
     assert abs(cim.get_ci_energy() - golden()["beh2"]["fci_energy"]) < 1e-10
 
 
+@pytest.mark.parametrize("case", ["fci_L8_4a4b_config1", "fci_beh2", "doci_beh2", "doci_L10_5", "fci_L11_5a5b",
+                                  "doci_L20_10"])
+def test_vectorised_spmv_matches_scalar_gather(case):
+    """Kernel 3 (vectorised column/value streams, unaligned row heads and
+    tails) and the automatic choice against the scalar row gather, FP64 and
+    FP32 values; D2h rows (dropped entries) have lengths that are not
+    multiples of 4."""
+    say("v4 spmv", case)
+    if case in CASES:
+        kind, make = CASES[case]
+        h, V, na, nb = make()
+    elif case == "fci_L11_5a5b":
+        (h, V), kind, na, nb = CA.gen_integrals(11, 1), "FCI", 5, 5
+    else:
+        (h, V), kind, na, nb = CA.gen_integrals(20, 1), "DOCI", 10, 10
+    cls = CA.FCI if kind == "FCI" else CA.DOCI
+    cim = cls(CA.Hamiltonian(h, V, na, nb), value_prec=CA.CF_F32)
+    m = cim.get_mat()
+    x = np.random.default_rng(5).standard_normal(cim.get_dim())
+    for prec in (CA.CF_F64, CA.CF_F32):
+        m.set_spmv_kernel(1)
+        y1 = m.mvprod(x, prec)
+        for kern in (3, 0):
+            m.set_spmv_kernel(kern)
+            y2 = m.mvprod(x, prec)
+            assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec, kern)
+
+
 def test_config2_energy_matches_oracle():
     """FCI L=12 6a6b seed 1: GPU solve vs the oracle's full CPU solve (golden)."""
     g = golden()["config2"]
@@ -330,6 +358,26 @@ def test_config2_energy_matches_oracle():
     assert abs(cim.get_ci_energy() - g["energy"]) < 1e-10
     info = cim.solve(precision=CA.CF_F32)
     say("config2 f32+polish", info.as_dict())
+    assert abs(cim.get_ci_energy() - g["energy"]) < 1e-8
+
+
+def test_config3_doci_L24_energy_matches_oracle():
+    """DOCI L=24 12 pairs seed 2 (2,704,156 determinants): GPU build + solve vs
+    the oracle's full CPU solve (golden), FP64, FP32+polish and the served
+    energy stop."""
+    g = golden()["config3"]
+    h, V = CA.gen_integrals(24, 2)
+    cim = CA.DOCI(CA.Hamiltonian(h, V, 12, 12), value_prec=CA.CF_F32)
+    n, nnz_full, nnz_upper = cim.get_mat().info()
+    assert n == 2704156 and nnz_upper == g["nnz_upper"] and nnz_full == 2 * nnz_upper - n
+    info = cim.solve()
+    say("config3 f64", info.as_dict())
+    assert abs(cim.get_ci_energy() - g["energy"]) < 1e-10
+    info = cim.solve(precision=CA.CF_F32)
+    say("config3 f32+polish", info.as_dict())
+    assert abs(cim.get_ci_energy() - g["energy"]) < 1e-8
+    info = cim.solve(precision=CA.CF_F32, tol_energy=1e-9, check_every=4)
+    say("config3 f32+energy stop", info.as_dict())
     assert abs(cim.get_ci_energy() - g["energy"]) < 1e-8
 
 
