@@ -39,13 +39,14 @@ METRIC = "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of pe
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["stream", "fci12", "direct16"], default="stream")
+    p.add_argument("--workload", choices=["stream", "fci12", "direct16", "spmv"], default="stream")
     p.add_argument("--steps", type=int, default=None, help="stream: requests (4096); fci12: solves (4)")
     p.add_argument("--warmup", type=int, default=None, help="stream: warm-up requests (64); fci12: solves (1)")
     p.add_argument("--slots", type=int, default=8)
     p.add_argument("--load", type=float, default=0.7, help="Poisson rate as a fraction of the measured rate")
     p.add_argument("--rate", type=float, default=0.0, help="Poisson rate (solves/s); overrides --load")
-    p.add_argument("--baseline-every", type=int, default=4)
+    p.add_argument("--baseline-every", type=int, default=4, help="0: skip the one-at-a-time FP64 baseline")
+    p.add_argument("--no-latency", action="store_true", help="skip the Poisson latency run")
     p.add_argument("--L", type=int, default=12)
     p.add_argument("--nel", type=int, default=6)
     p.add_argument("--seed", type=int, default=1)
@@ -215,23 +216,28 @@ def bench_stream(a, CA):
         log(f"closed batch: {rate:.2f} solves/s ({out['wall_s']:.2f} s)")
         nlat = min(K, 1024)
         lam = a.rate if a.rate > 0 else a.load * rate
-        lreqs = S.with_arrivals(reqs[:nlat], lam)
-        lat = S.run_stream(lreqs, arrs[:nlat], server=srv, timeout=1200, **served)
-        log(f"Poisson {lam:.1f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms")
+        lat = {"p50_ms": None, "p99_ms": None, "solves_per_s": None, "mean_ms": None}
+        if not a.no_latency:
+            lreqs = S.with_arrivals(reqs[:nlat], lam)
+            lat = S.run_stream(lreqs, arrs[:nlat], server=srv, timeout=1200, **served)
+            log(f"Poisson {lam:.1f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms")
     finally:
         srv.close()
     # the plain path: one request at a time, FP64 values, tolerance 1e-12 (ARPACK semantics)
-    sub = list(range(0, K, max(1, a.baseline_every)))
-    breqs = [reqs[i] for i in sub]
-    base = S.run_stream(breqs, [arrs[i] for i in sub], slots=1, precision=CA.CF_F64, timeout=1200)
-    log(f"one-at-a-time FP64: {base['solves_per_s']:.2f} solves/s over {len(sub)} requests")
-    Eserved = np.array([out["results"][i]["energy"] for i in sub])
-    Ebase = np.array([d["energy"] for d in base["results"]])
-    dE = np.abs(Eserved - Ebase)
+    base = None
+    dE = np.zeros(1)
     nmv_by_class = {}
-    for r, d in zip(breqs, base["results"]):
-        nmv_by_class.setdefault((r.kind, r.L), []).append(d["n_matvec"])
-    nmv_by_class = {k: float(np.mean(v)) for k, v in nmv_by_class.items()}
+    sub = list(range(0, K, a.baseline_every)) if a.baseline_every > 0 else []
+    if sub:
+        breqs = [reqs[i] for i in sub]
+        base = S.run_stream(breqs, [arrs[i] for i in sub], slots=1, precision=CA.CF_F64, timeout=1200)
+        log(f"one-at-a-time FP64: {base['solves_per_s']:.2f} solves/s over {len(sub)} requests")
+        Eserved = np.array([out["results"][i]["energy"] for i in sub])
+        Ebase = np.array([d["energy"] for d in base["results"]])
+        dE = np.abs(Eserved - Ebase)
+        for r, d in zip(breqs, base["results"]):
+            nmv_by_class.setdefault((r.kind, r.L), []).append(d["n_matvec"])
+        nmv_by_class = {k: float(np.mean(v)) for k, v in nmv_by_class.items()}
     res = out["results"]
     roof = None
     if not a.no_roofline:
@@ -239,7 +245,7 @@ def bench_stream(a, CA):
         roof = spmv_roofline(CA, CA.Context(0))
         log(f"SpMV {roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}%)")
     cpu = None
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and nmv_by_class:
         log("cpu baseline ...")
         cpu = cpu_baseline_stream(reqs, nmv_by_class)
         log(f"cpu baseline: {cpu['value']:.3f} solves/s (sample {cpu['sample_s']:.1f} s)")
@@ -268,10 +274,10 @@ def bench_stream(a, CA):
         "latency_run": {"arrivals": "Poisson", "rate_per_s": lam, "requests": nlat,
                         "achieved_solves_per_s": lat["solves_per_s"], "mean_ms": lat["mean_ms"]},
         "closed_batch_latency_ms": {"p50": out["p50_ms"], "p99": out["p99_ms"]},
-        "one_at_a_time_fp64": {"solves_per_s": base["solves_per_s"], "requests": len(sub),
-                               "sample": f"every {a.baseline_every}th request", "tol": 1e-12},
-        "speedup_vs_one_at_a_time_fp64": rate / base["solves_per_s"],
-        "max_abs_dE_vs_fp64": float(dE.max()),
+        "one_at_a_time_fp64": ({"solves_per_s": base["solves_per_s"], "requests": len(sub),
+                                "sample": f"every {a.baseline_every}th request", "tol": 1e-12} if base else None),
+        "speedup_vs_one_at_a_time_fp64": rate / base["solves_per_s"] if base else None,
+        "max_abs_dE_vs_fp64": float(dE.max()) if base else None,
         "mean_matvec": {"f32": float(np.mean([d["n_matvec_f32"] for d in res])),
                         "f64": float(np.mean([d["n_matvec"] for d in res]))},
         "roofline": roof,
@@ -423,6 +429,43 @@ def bench_direct16(a, CA):
     }
 
 
+def bench_spmv(a, CA):
+    """Stored-CRS SpMV alone on FCI L=12 6a6b (config 2): 3 FP64 and 3 FP32-value
+    products with the automatic kernel, then a 2 GiB device-to-device copy
+    (torch) -- the reference kernel for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings to bytes."""
+    ctx = CA.Context(0)
+    h, V = CA.gen_integrals(12, 1)
+    cim = CA.FCI(CA.Hamiltonian(h, V, 6, 6, 0.0, ctx), value_prec=CA.CF_F32)
+    m = cim.get_mat()
+    n, nnz, _ = m.info()
+    x = np.random.default_rng(0).standard_normal(n)
+    out = {}
+    for prec, fam, sval in ((CA.CF_F64, "spmv_f64", 8), (CA.CF_F32, "spmv_f32", 4)):
+        m.mvprod(x, prec)
+        ctx.enable_timing(True)
+        ctx.reset_timing()
+        for _ in range(3):
+            m.mvprod(x, prec)
+        nl, ms = ctx.kernel_time(fam)
+        ctx.enable_timing(False)
+        algo = nnz * (sval + 4) + (n + 1) * 8 + 16 * n
+        out[fam] = {"avg_launch_ms": ms / nl, "algorithmic_bytes": algo, "GBps": algo / (ms / nl * 1e-3) / 1e9}
+        log(f"{fam}: {ms / nl:.3f} ms {out[fam]['GBps']:.0f} GB/s")
+    copy_bytes = 2 << 30
+    try:
+        import torch
+        src = torch.ones(copy_bytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_bytes_each_way"] = copy_bytes
+    except Exception as e:  # torch without a GPU: no calibration copy
+        out["copy_error"] = str(e)
+    return {"metric": METRIC, "workload": "FCI L=12 6a6b stored-CRS SpMV", "n": n, "nnz": nnz, **out}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -431,7 +474,7 @@ def main():
         sys.exit(2)
     import ciflow_amd as CA
 
-    run = {"stream": bench_stream, "fci12": bench_fci12, "direct16": bench_direct16}[a.workload]
+    run = {"stream": bench_stream, "fci12": bench_fci12, "direct16": bench_direct16, "spmv": bench_spmv}[a.workload]
     out = run(a, CA)
     print(json.dumps(out))
 

@@ -105,6 +105,48 @@ void launch_doci_diag(uint64_t d, const str_t* s, const double* d1, HamDev H, do
 // ---------------------------------------------------------------------------
 // FCI rows: one wavefront per row.
 // ---------------------------------------------------------------------------
+// Element u of the segment that alpha-list entry ea opens in row (ia, ib):
+// column and value with the reference's formulas and summation order.
+__device__ __forceinline__ void fci_entry(const FciRowsArgs& a, uint64_t ia, uint64_t ib, str_t Sa, str_t Sb,
+                                          const Exc& ea, const Exc* lb, const Exc* sbl, uint32_t u, uint32_t& col,
+                                          double& val) {
+    const int L = a.H.L;
+    if (ea.type == 0) {
+        const Exc eb = lb[u];
+        col = (uint32_t)(ia * a.db + eb.J);
+        if (eb.type == 0) {
+            val = a.diag[ia * a.db + ib];
+        } else if (eb.type == 1) {
+            int p = eb.f < eb.t ? eb.f : eb.t, q = eb.f < eb.t ? eb.t : eb.f;
+            const double* w = a.H.W + ((size_t)p * L + q) * L;
+            val = eb.x;
+            for (str_t s = Sa; s; s &= s - 1) val += w[__builtin_ctzll(s)];
+            if (eb.sign < 0) val *= -1;
+        } else {
+            val = eb.x;
+        }
+    } else if (ea.type == 1) {
+        const Exc eb = sbl[u];
+        col = (uint32_t)((uint64_t)ea.J * a.db + eb.J);
+        if (eb.type == 0) {
+            int p = ea.f < ea.t ? ea.f : ea.t, q = ea.f < ea.t ? ea.t : ea.f;
+            const double* w = a.H.W + ((size_t)p * L + q) * L;
+            val = ea.x;
+            for (str_t s = Sb; s; s &= s - 1) val += w[__builtin_ctzll(s)];
+            if (ea.sign < 0) val *= -1;
+        } else {
+            size_t i0, i1, i2, i3;
+            if (ea.J > ia) { i0 = ea.f; i1 = eb.f; i2 = ea.t; i3 = eb.t; }
+            else { i0 = ea.t; i1 = eb.t; i2 = ea.f; i3 = eb.f; }
+            val = a.H.V[((i0 * L + i1) * L + i2) * L + i3];
+            if ((ea.sign < 0) != (eb.sign < 0)) val *= -1;
+        }
+    } else {
+        col = (uint32_t)((uint64_t)ea.J * a.db + ib);
+        val = ea.x;
+    }
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_fci_rows(FciRowsArgs a) {
     const int lane = threadIdx.x & 63;
@@ -117,7 +159,6 @@ __global__ void __launch_bounds__(256) k_fci_rows(FciRowsArgs a) {
     const Exc* la = a.fa + ia * (uint64_t)a.Ea;
     const Exc* lb = a.fb + ib * (uint64_t)a.Eb;
     const Exc* sbl = a.subb + ib * (uint64_t)a.Sb;
-    const int L = a.H.L;
     int64_t base = 0;
     if (FILL) base = a.rowptr[r];
     uint32_t cnt = 0;
@@ -133,42 +174,7 @@ __global__ void __launch_bounds__(256) k_fci_rows(FciRowsArgs a) {
                 if (so[mid] <= t) lo = mid;
                 else hi = mid;
             }
-            const Exc ea = la[lo];
-            const uint32_t u = t - so[lo];
-            if (ea.type == 0) {
-                const Exc eb = lb[u];
-                col = (uint32_t)(ia * a.db + eb.J);
-                if (eb.type == 0) {
-                    val = a.diag[r];
-                } else if (eb.type == 1) {
-                    int p = eb.f < eb.t ? eb.f : eb.t, q = eb.f < eb.t ? eb.t : eb.f;
-                    const double* w = a.H.W + ((size_t)p * L + q) * L;
-                    val = eb.x;
-                    for (str_t s = Sa; s; s &= s - 1) val += w[__builtin_ctzll(s)];
-                    if (eb.sign < 0) val *= -1;
-                } else {
-                    val = eb.x;
-                }
-            } else if (ea.type == 1) {
-                const Exc eb = sbl[u];
-                col = (uint32_t)((uint64_t)ea.J * a.db + eb.J);
-                if (eb.type == 0) {
-                    int p = ea.f < ea.t ? ea.f : ea.t, q = ea.f < ea.t ? ea.t : ea.f;
-                    const double* w = a.H.W + ((size_t)p * L + q) * L;
-                    val = ea.x;
-                    for (str_t s = Sb; s; s &= s - 1) val += w[__builtin_ctzll(s)];
-                    if (ea.sign < 0) val *= -1;
-                } else {
-                    size_t i0, i1, i2, i3;
-                    if (ea.J > ia) { i0 = ea.f; i1 = eb.f; i2 = ea.t; i3 = eb.t; }
-                    else { i0 = ea.t; i1 = eb.t; i2 = ea.f; i3 = eb.f; }
-                    val = a.H.V[((i0 * L + i1) * L + i2) * L + i3];
-                    if ((ea.sign < 0) != (eb.sign < 0)) val *= -1;
-                }
-            } else {
-                col = (uint32_t)((uint64_t)ea.J * a.db + ib);
-                val = ea.x;
-            }
+            fci_entry(a, ia, ib, Sa, Sb, la[lo], lb, sbl, t - so[lo], col, val);
             keep = fabs(val) > a.zero;
         }
         const uint64_t mask = __ballot(keep);
@@ -181,6 +187,45 @@ __global__ void __launch_bounds__(256) k_fci_rows(FciRowsArgs a) {
         cnt += (uint32_t)__popcll(mask);
     }
     if (!FILL && lane == 0) a.rowcnt[r - a.row0] = cnt;
+}
+
+// Position-major copy of the complete FCI rows of one alpha string:
+// ell[(ia*R + t)*db + ib] = element t of row (ia, ib), 0 where the element is
+// not stored (|v| <= zero).  One thread per beta string, so the writes of a
+// wavefront are contiguous.
+__global__ void __launch_bounds__(256) k_fci_ell(FciRowsArgs a, uint32_t chunks, double* __restrict__ ell,
+                                                 float* __restrict__ ell32) {
+    const uint64_t ia = blockIdx.x / chunks;
+    const uint64_t ib = (uint64_t)(blockIdx.x - ia * chunks) * 256 + threadIdx.x;
+    if (ia >= a.da || ib >= a.db) return;
+    const str_t Sa = a.sa[ia], Sb = a.sb[ib];
+    const uint32_t* so = a.segoff + ia * (uint64_t)(a.Ea + 1);
+    const uint64_t R = so[a.Ea];
+    const Exc* la = a.fa + ia * (uint64_t)a.Ea;
+    const Exc* lb = a.fb + ib * (uint64_t)a.Eb;
+    const Exc* sbl = a.subb + ib * (uint64_t)a.Sb;
+    for (int k = 0; k < a.Ea; ++k) {
+        const Exc ea = la[k];
+        const uint32_t len = so[k + 1] - so[k];
+        for (uint32_t u = 0; u < len; ++u) {
+            uint32_t col;
+            double val;
+            fci_entry(a, ia, ib, Sa, Sb, ea, lb, sbl, u, col, val);
+            if (!(fabs(val) > a.zero)) val = 0.0;
+            const uint64_t pos = (ia * R + so[k] + u) * a.db + ib;
+            if (ell) ell[pos] = val;
+            if (ell32) ell32[pos] = (float)val;
+        }
+    }
+}
+
+void launch_fci_ell(const FciRowsArgs& a, double* ell, float* ell32, hipStream_t stream) {
+    const uint64_t chunks = (a.db + 255) / 256;
+    const uint64_t grid = a.da * chunks;
+    if (!grid) return;
+    if (grid > 0xffffffffull) throw Error(ERR_UNSUPPORTED, "ELL build grid too large");
+    k_fci_ell<<<(unsigned)grid, 256, 0, stream>>>(a, (uint32_t)chunks, ell, ell32);
+    CF_HIP(hipGetLastError());
 }
 
 void launch_fci_rows(const FciRowsArgs& a, bool fill, hipStream_t stream) {

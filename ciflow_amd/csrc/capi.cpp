@@ -15,6 +15,7 @@
 #include "lanczos.h"
 #include "rdm.h"
 #include "sigma.h"
+#include "spmv_ell.h"
 #include "spmv_fci.h"
 
 #include <rocblas/rocblas.h>
@@ -204,9 +205,18 @@ struct cf_crs {
     DBuf<double> val;
     DBuf<float> val32;
     int lpr = 64;     // lanes per row of the scalar row gather
-    int kernel = 0;   // 0 auto, 1 row gather, 2 FCI staged, 3 vectorised row gather
+    int kernel = 0;   // 0 auto, 1 row gather, 2 FCI staged, 3 vectorised row gather, 4 FCI ELL
     // automatic choice per value precision: vectorised (lanes, unroll) or 0 = scalar gather
     int v4_lpr64 = 0, v4_unr64 = 0, v4_lpr32 = 0, v4_unr32 = 0;
+    // FCI position-major complete rows (kernel 4), built on demand
+    FciRowsArgs fargs{};
+    bool has_fargs = false;
+    DBuf<double> ell;
+    DBuf<float> ell32;
+    DBuf<uint16_t> jsub, jfull;
+    EllArgs eargs;
+    int ell_threads = 512;
+    bool ell_ready = false;
     std::unique_ptr<FciSpmvPlan> plan;
 };
 
@@ -246,22 +256,56 @@ void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
     const bool staged = m->kernel == 2 && m->plan;
     const bool v4_64 = m->kernel == 3 || (m->kernel == 0 && m->v4_lpr64);
     const bool v4_32 = m->kernel == 3 || (m->kernel == 0 && m->v4_lpr32);
+    const bool ell = m->kernel == 4 && m->ell_ready;
     if (prec == CF_F32) {
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
-        if (staged) launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream);
+        if (ell) launch_spmv_ell_f32(m->eargs, m->ell_threads, m->ell32.p, dx, dy, c->stream);
+        else if (staged) launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream);
         else if (v4_32)
             launch_spmv_v4_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->v4_lpr32 ? m->v4_lpr32 : 16,
                                m->v4_unr32 ? m->v4_unr32 : 2, c->stream);
         else launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->lpr, c->stream);
     } else {
         KTimer t(c, "spmv_f64");
-        if (staged) launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream);
+        if (ell) launch_spmv_ell_f64(m->eargs, m->ell_threads, m->ell.p, dx, dy, c->stream);
+        else if (staged) launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream);
         else if (v4_64)
             launch_spmv_v4_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->v4_lpr64 ? m->v4_lpr64 : 16,
                                m->v4_unr64 ? m->v4_unr64 : 2, c->stream);
         else launch_spmv_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->lpr, c->stream);
     }
+}
+
+// Position-major complete FCI rows + beta index tables (kernel 4).
+void make_ell(cf_crs* m) {
+    cf_space* sp = m->sp;
+    cf_ctx* c = sp->ham->ctx;
+    CF_REQUIRE(sp->kind == CF_FCI && m->has_fargs, "the ELL SpMV needs an FCI matrix");
+    CF_REQUIRE(sp->db <= kEllMaxDb, "the ELL SpMV needs d_beta <= 8192");
+    const int Ea = sp->ta->E, Sb = sp->tb->S, Eb = sp->tb->E;
+    const int nsa = sp->ta->S - 1, nda = Ea - 1 - nsa;
+    const uint64_t R = (uint64_t)Eb + (uint64_t)nsa * Sb + nda;
+    m->jsub.alloc((uint64_t)Sb * sp->db);
+    m->jfull.alloc((uint64_t)Eb * sp->db);
+    launch_ell_index(sp->db, Sb, Eb, sp->tb->sub.p, sp->tb->full.p, m->jsub.p, m->jfull.p, c->stream);
+    m->ell.alloc(m->n * R);
+    if (m->val32.p) m->ell32.alloc(m->n * R);
+    {
+        KTimer t(c, "crs_fill");
+        launch_fci_ell(m->fargs, m->ell.p, m->ell32.p, c->stream);
+    }
+    EllArgs& e = m->eargs;
+    e.da = sp->da;
+    e.db = sp->db;
+    e.Ea = Ea;
+    e.Sb = Sb;
+    e.Eb = Eb;
+    e.fa = sp->ta->full.p;
+    e.segoff = sp->segoff.p;
+    e.jsub = m->jsub.p;
+    e.jfull = m->jfull.p;
+    m->ell_ready = true;
 }
 
 // Staged FCI SpMV plan (alpha-block structure of the rows).
@@ -746,6 +790,9 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
         fa.row0 = 0;
         fa.nrows = m->n;
         fa.rowcnt = cnt.p;
+        m->fargs = fa;
+        m->fargs.rowcnt = nullptr;
+        m->has_fargs = true;
         KTimer t(c, "crs_count");
         launch_fci_rows(fa, false, c->stream);
     } else {
@@ -931,8 +978,15 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y) {
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
     API_BEGIN
     CF_REQUIRE(m, "matrix is NULL");
-    CF_REQUIRE(kind >= 0 && kind <= 3, "kind must be 0, 1, 2 or 3");
+    CF_REQUIRE(kind >= 0 && kind <= 4, "kind must be 0, 1, 2, 3 or 4");
     m->sp->ham->ctx->use();
+    if (kind == 4) {
+        const int nt = tile_rows > 0 ? tile_rows : 512;
+        CF_REQUIRE(nt == 256 || nt == 512 || nt == 1024, "ELL workgroup size must be 256, 512 or 1024");
+        if (!m->ell_ready) make_ell(m);
+        m->ell_threads = nt;
+        CF_HIP(hipStreamSynchronize(m->sp->ham->ctx->stream));
+    }
     if (kind == 2) {
         CF_REQUIRE(m->sp->kind == CF_FCI, "the staged SpMV needs an FCI matrix");
         const int W = tile_rows > 0 ? tile_rows : 16;

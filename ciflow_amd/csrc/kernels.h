@@ -66,6 +66,9 @@ struct FciRowsArgs {
     float* val32;
 };
 void launch_fci_rows(const FciRowsArgs& a, bool fill, hipStream_t stream);
+// Position-major (per alpha string) copy of the complete rows, zeros where
+// |v| <= zero; either output may be null.
+void launch_fci_ell(const FciRowsArgs& a, double* ell, float* ell32, hipStream_t stream);
 
 struct DociRowsArgs {
     uint64_t d;

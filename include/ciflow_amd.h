@@ -132,7 +132,9 @@ int cf_mvprod_dev(cf_crs* m, int value_prec, const double* dx, double* dy);  /* 
 int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* host pointers, CF_F64 or CF_F32 values */
 /* SpMV kernel selection (no reference counterpart): 0 = automatic, 1 = plain
  * row gather, 2 = FCI alpha-block staged (FCI matrices only), 3 = row gather
- * with vectorised column/value streams. */
+ * with vectorised column/value streams, 4 = FCI position-major complete rows
+ * with LDS-staged x blocks (FCI, d_beta <= 8192; builds an extra value copy;
+ * tile_rows = workgroup size 256/512/1024). */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
 
 /* ---- ground state: CIMethod::solve -> SparseMatrix_CRS::arpackDiagonalize

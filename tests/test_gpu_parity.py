@@ -348,6 +348,32 @@ def test_vectorised_spmv_matches_scalar_gather(case):
             assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec, kern)
 
 
+@pytest.mark.parametrize("case,nt", [("fci_L8_4a4b_config1", 512), ("fci_beh2", 256), ("fci_L7_3a2b", 512),
+                                     ("fci_L11_5a5b", 512), ("fci_L11_5a5b", 1024), ("fci_L12_6a6b", 256)])
+def test_ell_spmv_matches_scalar_gather(case, nt):
+    """Kernel 4 (position-major complete rows, LDS-staged x blocks): zeros at
+    the dropped D2h positions, nalpha != nbeta, several workgroup sizes."""
+    say("ell spmv", case, nt)
+    if case in CASES:
+        _, make = CASES[case]
+        h, V, na, nb = make()
+    else:
+        L, na, nb = {"fci_L7_3a2b": (7, 3, 2), "fci_L11_5a5b": (11, 5, 5), "fci_L12_6a6b": (12, 6, 6)}[case]
+        h, V = CA.gen_integrals(L, 1)
+    cim = CA.FCI(CA.Hamiltonian(h, V, na, nb), value_prec=CA.CF_F32)
+    m = cim.get_mat()
+    x = np.random.default_rng(6).standard_normal(cim.get_dim())
+    for prec in (CA.CF_F64, CA.CF_F32):
+        m.set_spmv_kernel(1)
+        y1 = m.mvprod(x, prec)
+        m.set_spmv_kernel(4, nt)
+        y2 = m.mvprod(x, prec)
+        assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec)
+    if case == "fci_beh2":
+        cim.solve()
+        assert abs(cim.get_ci_energy() - golden()["beh2"]["fci_energy"]) < 1e-10
+
+
 def test_config2_energy_matches_oracle():
     """FCI L=12 6a6b seed 1: GPU solve vs the oracle's full CPU solve (golden)."""
     g = golden()["config2"]
