@@ -181,9 +181,13 @@ def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
     nl, ms = ctx.kernel_time("spmv_f64")
     ctx.enable_timing(False)
     avg_ms = ms / max(nl, 1)
-    algo = nnz_full * 12 + (n + 1) * 8 + 16 * n
+    kern, algo = m.spmv_bytes(CA.CF_F64)
     ach = algo / (avg_ms * 1e-3) / 1e9
-    out = {"bound": "hbm", "kernel": "k_spmv<double,16> (stored full-row CRS, FP64 values, u32 columns)",
+    names = {1: "k_spmv (CRS row gather)", 3: "k_spmv_v4 (CRS row gather, vectorised streams)",
+             4: "k_spmv_ell (FCI position-major complete rows, LDS-staged x blocks, FP64 values)"}
+    crs_bytes = nnz_full * 12 + (n + 1) * 8 + 16 * n
+    out = {"bound": "hbm", "kernel": names.get(kern, str(kern)), "crs_equivalent_bytes": crs_bytes,
+           "crs_equivalent_GBps": crs_bytes / (avg_ms * 1e-3) / 1e9,
            "workload": f"FCI L={L} {nel}a{nel}b seed {seed}: n={n}, nnz={nnz_full}, one FP64 Lanczos solve",
            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
            "avg_launch_ms": avg_ms, "launches": nl, "algorithmic_bytes": algo,
@@ -331,7 +335,7 @@ def bench_fci12(a, CA):
         fam, nl, ms, sval = "spmv_f64", nl64, ms64, 8
     avg_ms = ms / max(nl, 1)
     nnz_upper = cim.get_mat().datasize()
-    algo_bytes = nnz_full * (sval + 4) + (n + 1) * 8 + 16 * n
+    kern, algo_bytes = cim.get_mat().spmv_bytes(CA.CF_F32 if sval == 4 else CA.CF_F64)
     achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
     ctx.enable_timing(False)
     r0 = time.perf_counter()
@@ -369,7 +373,8 @@ def bench_fci12(a, CA):
         "build_ms": {"count": ms_count / max(ncount, 1), "fill": ms_fill / max(nfill, 1)},
         "rdm_ms": rdm_ms,
         "rdm_trace": tr,
-        "roofline": {"bound": "hbm", "kernel": fam, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": f"{fam} (kernel {kern})", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
                      "launches": nl, "algorithmic_bytes": algo_bytes},
         "cpu_baseline": cpu,
@@ -392,8 +397,11 @@ def bench_direct16(a, CA):
     ctx.enable_timing(True)
     ctx.reset_timing()
     log(f"direct FCI L={L}: n={n}, setup {t_setup:.2f} s; FP32+polish solve ...")
+    def prog(nmv, th, r):
+        log(f"  {nmv} products, theta {th:.10f}, residual {r:.2e}, {time.perf_counter() - t0:.1f} s")
+
     t0 = time.perf_counter()
-    i32 = d.solve(precision=CA.CF_F32, ncv=39)
+    i32 = d.solve(precision=CA.CF_F32, ncv=39, progress=prog)
     t32 = time.perf_counter() - t0
     e32 = d.get_ci_energy()
     log(f"  E={e32:.12f} matvec {i32.n_matvec}+{i32.n_matvec_f32} {t32:.1f} s")
@@ -408,7 +416,7 @@ def bench_direct16(a, CA):
         log("  FP64 solve ...")
         ctx.reset_timing()
         t0 = time.perf_counter()
-        i64 = d.solve(precision=CA.CF_F64, ncv=39)
+        i64 = d.solve(precision=CA.CF_F64, ncv=39, progress=prog)
         t64 = time.perf_counter() - t0
         e64 = d.get_ci_energy()
         n64b, ms64b = ctx.kernel_time("sigma_f64")
@@ -449,8 +457,9 @@ def bench_spmv(a, CA):
             m.mvprod(x, prec)
         nl, ms = ctx.kernel_time(fam)
         ctx.enable_timing(False)
-        algo = nnz * (sval + 4) + (n + 1) * 8 + 16 * n
-        out[fam] = {"avg_launch_ms": ms / nl, "algorithmic_bytes": algo, "GBps": algo / (ms / nl * 1e-3) / 1e9}
+        kern, algo = m.spmv_bytes(prec)
+        out[fam] = {"kernel": kern, "avg_launch_ms": ms / nl, "algorithmic_bytes": algo,
+                    "GBps": algo / (ms / nl * 1e-3) / 1e9}
         log(f"{fam}: {ms / nl:.3f} ms {out[fam]['GBps']:.0f} GB/s")
     copy_bytes = 2 << 30
     try:

@@ -72,10 +72,13 @@ class CIFlowError(RuntimeError):
         self.code = code
 
 
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, C.c_double)
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("ncv", C.c_int), ("maxrestart", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
                 ("tol_f32", C.c_double), ("seed", C.c_uint64), ("check_every", C.c_int),
-                ("tol_energy", C.c_double)]
+                ("tol_energy", C.c_double), ("progress", PROGRESS_FN), ("progress_user", C.c_void_p)]
 
 
 class SolveInfo(C.Structure):
@@ -145,6 +148,7 @@ def lib():
         "cf_mvprod": (C.c_int, [vp, dp, dp]),
         "cf_mvprod_dev": (C.c_int, [vp, C.c_int, vp, vp]),
         "cf_mvprod_prec": (C.c_int, [vp, C.c_int, dp, dp]),
+        "cf_crs_spmv_bytes": (C.c_int, [vp, C.c_int, u64p, C.POINTER(C.c_int)]),
         "cf_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
         "cf_solve": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
         "cf_rci_create": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_uint64, C.POINTER(vp)]),
@@ -388,6 +392,13 @@ class SparseMatrix_CRS_Sym:
     def export_full(self, r0: int = 0, r1: int | None = None):
         return self._export(False, r0, self.getn() if r1 is None else r1)
 
+    def spmv_bytes(self, precision: int = CF_F64):
+        """(kernel id 1-4, algorithmic HBM bytes per product) of the current choice."""
+        b = C.c_uint64()
+        k = C.c_int()
+        _check(lib().cf_crs_spmv_bytes(self.ptr, precision, C.byref(b), C.byref(k)))
+        return k.value, b.value
+
     def set_spmv_kernel(self, kind: int, tile_rows: int = 0):
         """0 automatic, 1 plain row gather, 2 FCI alpha-block staged."""
         _check(lib().cf_crs_set_spmv_kernel(self.ptr, kind, tile_rows))
@@ -414,10 +425,16 @@ class SparseMatrix_CRS_Sym:
 
 
 def solve_opts(**kw) -> SolveOpts:
+    """cf_solve_opts with defaults; progress=callable(n_matvec, theta, resid) is
+    wrapped as the C callback (kept alive on the returned object)."""
     o = SolveOpts()
     lib().cf_solve_opts_default(C.byref(o))
+    prog = kw.pop("progress", None)
     for k, v in kw.items():
         setattr(o, k, v)
+    if prog is not None:
+        o._cb = PROGRESS_FN(lambda user, nmv, th, r: prog(nmv, th, r))
+        o.progress = o._cb
     return o
 
 

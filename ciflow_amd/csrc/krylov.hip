@@ -160,6 +160,73 @@ void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, dou
     CF_HIP(hipGetLastError());
 }
 
+// w -= a * v1 + b * v0 with a, b read from device memory (the three-term
+// Lanczos step before the re-orthogonalisation pass).
+__global__ void k_local3(uint64_t n, const double* __restrict__ a, const double* __restrict__ v1,
+                         const double* __restrict__ b, const double* __restrict__ v0, double* __restrict__ w) {
+    const double ca = *a, cb = *b;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+        w[q] -= ca * v1[q] + cb * v0[q];
+}
+
+void launch_local3(uint64_t n, const double* a, const double* v1, const double* b, const double* v0, double* w,
+                   hipStream_t stream) {
+    if (!n) return;
+    uint64_t grid = (n + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    k_local3<<<(unsigned)grid, 256, 0, stream>>>(n, a, v1, b, v0, w);
+    CF_HIP(hipGetLastError());
+}
+
+// w -= Vb[:, 0:k] c, and out = |w|^2 of the result: block partials plus a
+// last-block reduction in block order (deterministic), counter left zero.
+__global__ void __launch_bounds__(256) k_multi_axpy_norm(uint64_t n, int k, const double* __restrict__ Vb,
+                                                         const double* __restrict__ c, double* __restrict__ w,
+                                                         double* partial, unsigned* counter,
+                                                         double* __restrict__ out) {
+    __shared__ double cs[64];
+    __shared__ double red[4];
+    __shared__ bool last;
+    if (threadIdx.x < 64) cs[threadIdx.x] = (int)threadIdx.x < k ? c[threadIdx.x] : 0.0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    double nrm = 0.0;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        double s = 0.0;
+        for (int i = 0; i < k; ++i) s += cs[i] * Vb[(uint64_t)i * n + q];
+        const double wq = w[q] - s;
+        w[q] = wq;
+        nrm += wq * wq;
+    }
+    nrm = wave_sum(nrm);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last || wid != 0) return;
+    __threadfence();
+    double t = 0.0;
+    for (uint32_t b2 = lane; b2 < gridDim.x; b2 += 64) t += partial[b2];
+    t = wave_sum(t);
+    if (lane == 0) {
+        *out = t;
+        *counter = 0u;
+    }
+}
+
+void launch_multi_axpy_norm(uint64_t n, int k, const double* Vb, const double* c, double* w, double* partial,
+                            int nblocks, unsigned* counter, double* out, hipStream_t stream) {
+    if (k > 64) throw Error(ERR_ARG, "multi_axpy_norm: k > 64");
+    if (!n || nblocks < 1) return;
+    k_multi_axpy_norm<<<(unsigned)nblocks, 256, 0, stream>>>(n, k, Vb, c, w, partial, counter, out);
+    CF_HIP(hipGetLastError());
+}
+
 // dst = src * s
 __global__ void k_scale_copy(uint64_t n, const double* __restrict__ src, double s, double* __restrict__ dst) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

@@ -242,6 +242,7 @@ bool ThickRestartLanczos::absorb(int j1) {
 }
 
 bool ThickRestartLanczos::accept(double res, const std::vector<double>& th) const {
+    if (progress_) progress_(napply_, th[0], res);
     if (res <= tol_ * std::fabs(th[0])) return true;
     return tol_energy_ > 0.0 && th.size() >= 2 && res * res <= tol_energy_ * (th[1] - th[0]);
 }

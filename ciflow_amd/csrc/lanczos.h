@@ -15,6 +15,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 namespace cf {
@@ -65,6 +66,8 @@ public:
     int n_apply() const { return napply_; }
     int n_restart() const { return nrestart_; }
     int status() const { return status_; }  // 0 converged, 1 restart limit
+    // Called at every convergence test: (products so far, lowest Ritz value, residual estimate).
+    void set_progress(std::function<void(int, double, double)> f) { progress_ = std::move(f); }
 
 private:
     enum Phase { INIT, LOOP_APPLY, AFTER_APPLY, FINAL_APPLY, AFTER_FINAL, FINISHED };
@@ -80,6 +83,7 @@ private:
     bool start_given_;
     int check_every_ = 0, filled_ = 0;
     double tol_energy_ = 0.0;
+    std::function<void(int, double, double)> progress_;
     bool accept(double res, const std::vector<double>& th) const;
     Phase phase_ = INIT;
     int pending_ = 0;

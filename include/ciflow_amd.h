@@ -136,6 +136,9 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * with LDS-staged x blocks (FCI, d_beta <= 8192; builds an extra value copy;
  * tile_rows = workgroup size 256/512/1024). */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
+/* Kernel the next product with value_prec would run (1-4) and its algorithmic
+ * HBM bytes (matrix stream + x + y), the roofline numerator. */
+int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* kernel);
 
 /* ---- ground state: CIMethod::solve -> SparseMatrix_CRS::arpackDiagonalize
  *      (CIMethod.cpp:241-268, SparseMatrix_CRS.cpp:360-481), ARPACK's
@@ -154,6 +157,10 @@ typedef struct {
                            every check_every (4 when 0) steps. */
     double tol_energy;  /* > 0: also stop once ||r||^2 / (theta_1 - theta_0) <= tol_energy
                            (Kato-Temple bound on the eigenvalue error, Eh); 0: off */
+    /* Called on the solving thread at every convergence test with the products
+     * applied so far, the lowest Ritz value and its residual estimate; NULL: none. */
+    void (*progress)(void* user, int n_matvec, double theta, double resid);
+    void* progress_user;
 } cf_solve_opts;
 
 typedef struct {

@@ -175,6 +175,17 @@ def test_rci_dropin_with_gpu_mvprod():
     assert info == 0 and abs(ev - g["energy"]) < 1e-10
 
 
+def test_progress_callback_reports_convergence_tests():
+    g = golden()["config1"]
+    h, V = CA.gen_integrals(8, 0)
+    cim = CA.FCI(CA.Hamiltonian(h, V, 4, 4))
+    seen = []
+    info = cim.solve(progress=lambda nmv, th, r: seen.append((nmv, th, r)))
+    assert len(seen) >= 1 and seen[-1][0] <= info.n_matvec
+    assert [s[0] for s in seen] == sorted(s[0] for s in seen)
+    assert abs(seen[-1][1] - g["energy"]) < 1e-9
+
+
 def test_fp32_values_with_fp64_polish():
     g = golden()["config1"]
     h, V = CA.gen_integrals(8, 0)
@@ -369,9 +380,10 @@ def test_ell_spmv_matches_scalar_gather(case, nt):
         m.set_spmv_kernel(4, nt)
         y2 = m.mvprod(x, prec)
         assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec)
-    if case == "fci_beh2":
+    if case == "fci_beh2":  # the solve on kernel 4 (fixture Econst added)
         cim.solve()
-        assert abs(cim.get_ci_energy() - golden()["beh2"]["fci_energy"]) < 1e-10
+        econst = fixture("beh2_sto3g.npz")[4]
+        assert abs(cim.get_ci_energy() + econst - golden()["beh2"]["fci_energy"]) < 1e-10
 
 
 def test_config2_energy_matches_oracle():
