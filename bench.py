@@ -39,7 +39,7 @@ METRIC = "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of pe
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["stream", "fci12", "direct16", "spmv"], default="stream")
+    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv"], default="stream")
     p.add_argument("--steps", type=int, default=None, help="stream: requests (4096); fci12: solves (4)")
     p.add_argument("--warmup", type=int, default=None, help="stream: warm-up requests (64); fci12: solves (1)")
     p.add_argument("--slots", type=int, default=8)
@@ -290,8 +290,14 @@ def bench_stream(a, CA):
 
 
 def bench_fci12(a, CA):
+    """One CI space built and solved per step: FCI L=12 6a6b seed 1 (config 2) or,
+    with --workload doci24, DOCI L=24 12 pairs seed 2 (config 3)."""
     K = a.steps if a.steps is not None else 4
     W = a.warmup if a.warmup is not None else 1
+    doci = a.workload == "doci24"
+    if doci:
+        a.L, a.nel, a.seed = 24, 12, 2
+    cls = CA.DOCI if doci else CA.FCI
     ctx = CA.Context(0)
     h, V = CA.gen_integrals(a.L, a.seed)
     ham = CA.Hamiltonian(h, V, a.nel, a.nel, 0.0, ctx)
@@ -299,7 +305,7 @@ def bench_fci12(a, CA):
     ctx.synchronize()
 
     def step():
-        cim = CA.FCI(ham, value_prec=prec)
+        cim = cls(ham, value_prec=prec)
         info = cim.solve(precision=prec)
         return cim, info
 
@@ -343,7 +349,7 @@ def bench_fci12(a, CA):
     rdm_ms = (time.perf_counter() - r0) * 1e3
     tr = sum(cim.get_one_rdm(s, i, i) for s in range(2) for i in range(a.L))
     cpu = None
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and not doci:
         log("cpu baseline ...")
         cpu = cpu_baseline_fci(h, V, a.L, a.nel, nnz_upper, n, infos[-1]["n_matvec"], a.cpu_lines)
         log(f"cpu baseline: {cpu['value']:.3e} solves/s (sample {cpu['sample_s']:.1f} s)")
@@ -361,7 +367,8 @@ def bench_fci12(a, CA):
         "vs_baseline": None,
         "dtype": "f64" if prec == CA.CF_F64 else "f32-values/f64-accumulate+f64-polish",
         "data": "synthetic",
-        "config": {"workload": f"FCI L={a.L} {a.nel}a{a.nel}b seed {a.seed}: GPU CRS build + Lanczos ground state",
+        "config": {"workload": (f"DOCI L={a.L} {a.nel} pairs" if doci else f"FCI L={a.L} {a.nel}a{a.nel}b")
+                   + f" seed {a.seed}: GPU CRS build + Lanczos ground state",
                    "determinants": n, "nnz_full": nnz_full, "nnz_upper": nnz_upper, "tol": 1e-12,
                    "precision": a.precision},
         "p50_ms": float(np.percentile(lat_ms, 50)),
@@ -483,7 +490,8 @@ def main():
         sys.exit(2)
     import ciflow_amd as CA
 
-    run = {"stream": bench_stream, "fci12": bench_fci12, "direct16": bench_direct16, "spmv": bench_spmv}[a.workload]
+    run = {"stream": bench_stream, "fci12": bench_fci12, "doci24": bench_fci12, "direct16": bench_direct16,
+           "spmv": bench_spmv}[a.workload]
     out = run(a, CA)
     print(json.dumps(out))
 

@@ -149,6 +149,7 @@ def lib():
         "cf_mvprod_dev": (C.c_int, [vp, C.c_int, vp, vp]),
         "cf_mvprod_prec": (C.c_int, [vp, C.c_int, dp, dp]),
         "cf_crs_spmv_bytes": (C.c_int, [vp, C.c_int, u64p, C.POINTER(C.c_int)]),
+        "cf_crs_tune_ell": (C.c_int, [vp, C.c_int, C.c_int]),
         "cf_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
         "cf_solve": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
         "cf_rci_create": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_uint64, C.POINTER(vp)]),
@@ -398,6 +399,10 @@ class SparseMatrix_CRS_Sym:
         k = C.c_int()
         _check(lib().cf_crs_spmv_bytes(self.ptr, precision, C.byref(b), C.byref(k)))
         return k.value, b.value
+
+    def tune_ell(self, threads: int, parts: int = 1):
+        """Kernel-4 workgroup size and workgroups per alpha string."""
+        _check(lib().cf_crs_tune_ell(self.ptr, threads, parts))
 
     def set_spmv_kernel(self, kind: int, tile_rows: int = 0):
         """0 automatic, 1 plain row gather, 2 FCI alpha-block staged."""

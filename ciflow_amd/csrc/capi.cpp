@@ -216,6 +216,8 @@ struct cf_crs {
     DBuf<uint16_t> jsub, jfull;
     EllArgs eargs;
     int ell_threads = 512;
+    int ell_parts = 1;
+    DBuf<double> ell_part;  // partial row sums when ell_parts > 1
     bool ell_ready = false;
     std::unique_ptr<FciSpmvPlan> plan;
 };
@@ -260,7 +262,8 @@ void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
     if (prec == CF_F32) {
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
-        if (ell) launch_spmv_ell_f32(m->eargs, m->ell_threads, m->ell32.p, dx, dy, c->stream);
+        if (ell)
+            launch_spmv_ell_f32(m->eargs, m->ell_threads, m->ell_parts, m->ell32.p, dx, dy, m->ell_part.p, c->stream);
         else if (staged) launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream);
         else if (v4_32)
             launch_spmv_v4_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->v4_lpr32 ? m->v4_lpr32 : 16,
@@ -268,7 +271,8 @@ void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
         else launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->lpr, c->stream);
     } else {
         KTimer t(c, "spmv_f64");
-        if (ell) launch_spmv_ell_f64(m->eargs, m->ell_threads, m->ell.p, dx, dy, c->stream);
+        if (ell)
+            launch_spmv_ell_f64(m->eargs, m->ell_threads, m->ell_parts, m->ell.p, dx, dy, m->ell_part.p, c->stream);
         else if (staged) launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream);
         else if (v4_64)
             launch_spmv_v4_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->v4_lpr64 ? m->v4_lpr64 : 16,
@@ -1015,6 +1019,20 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y) {
     spmv(m, value_prec, c->hx.p, c->hy.p);
     CF_HIP(hipMemcpyAsync(y, c->hy.p, m->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CF_HIP(hipStreamSynchronize(c->stream));
+    API_END
+}
+
+int cf_crs_tune_ell(cf_crs* m, int threads, int parts) {
+    API_BEGIN
+    CF_REQUIRE(m, "matrix is NULL");
+    CF_REQUIRE(threads == 256 || threads == 512 || threads == 1024, "threads must be 256, 512 or 1024");
+    CF_REQUIRE(parts >= 1 && parts <= kEllMaxParts, "parts must be in [1, 8]");
+    m->sp->ham->ctx->use();
+    if (!m->ell_ready) make_ell(m);
+    m->ell_threads = threads;
+    if (parts > 1 && m->ell_part.n < (size_t)parts * m->n) m->ell_part.alloc((size_t)parts * m->n);
+    m->ell_parts = parts;
+    CF_HIP(hipStreamSynchronize(m->sp->ham->ctx->stream));
     API_END
 }
 

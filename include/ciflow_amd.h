@@ -136,6 +136,9 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * with LDS-staged x blocks (FCI, d_beta <= 8192; builds an extra value copy;
  * tile_rows = workgroup size 256/512/1024). */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
+/* Kernel 4 tuning: workgroup size (256/512/1024) and workgroups per alpha
+ * string (1-8; > 1 adds a partial-sum pass and parts*n doubles of scratch). */
+int cf_crs_tune_ell(cf_crs* m, int threads, int parts);
 /* Kernel the next product with value_prec would run (1-4) and its algorithmic
  * HBM bytes (matrix stream + x + y), the roofline numerator. */
 int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* kernel);
