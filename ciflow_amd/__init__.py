@@ -149,7 +149,6 @@ def lib():
         "cf_mvprod_dev": (C.c_int, [vp, C.c_int, vp, vp]),
         "cf_mvprod_prec": (C.c_int, [vp, C.c_int, dp, dp]),
         "cf_crs_spmv_bytes": (C.c_int, [vp, C.c_int, u64p, C.POINTER(C.c_int)]),
-        "cf_crs_tune_ell": (C.c_int, [vp, C.c_int, C.c_int]),
         "cf_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
         "cf_solve": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
         "cf_rci_create": (C.c_int, [C.c_uint64, C.c_int, C.c_double, C.c_int, C.c_uint64, C.POINTER(vp)]),
@@ -166,6 +165,7 @@ def lib():
         "cf_sigma_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
         "cf_sigma_destroy": (C.c_int, [vp]),
         "cf_sigma_set_threads": (C.c_int, [vp, C.c_int]),
+        "cf_sigma_set_variant": (C.c_int, [vp, C.c_int]),
         "cf_sigma_apply_dev": (C.c_int, [vp, C.c_int, vp, vp]),
         "cf_sigma_apply": (C.c_int, [vp, C.c_int, dp, dp]),
         "cf_solve_direct": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
@@ -400,10 +400,6 @@ class SparseMatrix_CRS_Sym:
         _check(lib().cf_crs_spmv_bytes(self.ptr, precision, C.byref(b), C.byref(k)))
         return k.value, b.value
 
-    def tune_ell(self, threads: int, parts: int = 1):
-        """Kernel-4 workgroup size and workgroups per alpha string."""
-        _check(lib().cf_crs_tune_ell(self.ptr, threads, parts))
-
     def set_spmv_kernel(self, kind: int, tile_rows: int = 0):
         """0 automatic, 1 plain row gather, 2 FCI alpha-block staged."""
         _check(lib().cf_crs_set_spmv_kernel(self.ptr, kind, tile_rows))
@@ -587,6 +583,10 @@ class FCI_Direct(CIMethod):
 
     def get_name(self):
         return "FCI_Direct"
+
+    def set_variant(self, dma: bool):
+        """Double-buffered alpha-beta kernel with direct-to-LDS row loads (True)."""
+        _check(lib().cf_sigma_set_variant(self.sig.ptr, int(bool(dma))))
 
     def set_threads(self, threads: int):
         """Workgroup size of the alpha-beta sigma kernel (128, 256, 512, 768, 1024)."""

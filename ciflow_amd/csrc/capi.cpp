@@ -216,8 +216,6 @@ struct cf_crs {
     DBuf<uint16_t> jsub, jfull;
     EllArgs eargs;
     int ell_threads = 512;
-    int ell_parts = 1;
-    DBuf<double> ell_part;  // partial row sums when ell_parts > 1
     bool ell_ready = false;
     std::unique_ptr<FciSpmvPlan> plan;
 };
@@ -262,8 +260,7 @@ void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
     if (prec == CF_F32) {
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
-        if (ell)
-            launch_spmv_ell_f32(m->eargs, m->ell_threads, m->ell_parts, m->ell32.p, dx, dy, m->ell_part.p, c->stream);
+        if (ell) launch_spmv_ell_f32(m->eargs, m->ell_threads, m->ell32.p, dx, dy, c->stream);
         else if (staged) launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream);
         else if (v4_32)
             launch_spmv_v4_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->v4_lpr32 ? m->v4_lpr32 : 16,
@@ -271,8 +268,7 @@ void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
         else launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->lpr, c->stream);
     } else {
         KTimer t(c, "spmv_f64");
-        if (ell)
-            launch_spmv_ell_f64(m->eargs, m->ell_threads, m->ell_parts, m->ell.p, dx, dy, m->ell_part.p, c->stream);
+        if (ell) launch_spmv_ell_f64(m->eargs, m->ell_threads, m->ell.p, dx, dy, c->stream);
         else if (staged) launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream);
         else if (v4_64)
             launch_spmv_v4_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->v4_lpr64 ? m->v4_lpr64 : 16,
@@ -1022,20 +1018,6 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y) {
     API_END
 }
 
-int cf_crs_tune_ell(cf_crs* m, int threads, int parts) {
-    API_BEGIN
-    CF_REQUIRE(m, "matrix is NULL");
-    CF_REQUIRE(threads == 256 || threads == 512 || threads == 1024, "threads must be 256, 512 or 1024");
-    CF_REQUIRE(parts >= 1 && parts <= kEllMaxParts, "parts must be in [1, 8]");
-    m->sp->ham->ctx->use();
-    if (!m->ell_ready) make_ell(m);
-    m->ell_threads = threads;
-    if (parts > 1 && m->ell_part.n < (size_t)parts * m->n) m->ell_part.alloc((size_t)parts * m->n);
-    m->ell_parts = parts;
-    CF_HIP(hipStreamSynchronize(m->sp->ham->ctx->stream));
-    API_END
-}
-
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
     API_BEGIN
     CF_REQUIRE(m, "matrix is NULL");
@@ -1151,6 +1133,7 @@ struct cf_sigma {
     rocblas_handle blas = nullptr;  // the context's handle
     SigmaArgs args;
     int threads = 1024;  // workgroup size of the alpha-beta kernel
+    bool dma = false;    // double-buffered variant with direct-to-LDS row loads
 };
 
 namespace {
@@ -1178,14 +1161,16 @@ void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
         CF_BLAS(rocblas_sgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb32, m,
                               S->x32.p, m, &one, S->y32.p, m));
         launch_to_f64(sp->n, S->y32.p, y, c->stream);
-        launch_sigma_ab<float>(S->args, S->threads, S->x32.p, y, c->stream);
+        launch_sigma_ab<float>(S->args, S->threads, S->dma && sigma_ab_dma_lds_bytes(S->args, true) <= 160 * 1024,
+                               S->x32.p, y, c->stream);
     } else {
         const double one = 1.0, zero = 0.0;
         CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, x, m,
                               S->Ha.p, nn, &zero, y, m));
         CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb, m, x,
                               m, &one, y, m));
-        launch_sigma_ab<double>(S->args, S->threads, x, y, c->stream);
+        launch_sigma_ab<double>(S->args, S->threads, S->dma && sigma_ab_dma_lds_bytes(S->args, false) <= 160 * 1024,
+                                x, y, c->stream);
     }
 }
 }  // namespace
@@ -1266,6 +1251,13 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     CF_REQUIRE(sigma_ab_lds_bytes(a, false) <= 160 * 1024, "beta string space too large for the LDS row");
     CF_HIP(hipStreamSynchronize(c->stream));
     *out = S.release();
+    API_END
+}
+
+int cf_sigma_set_variant(cf_sigma* s, int dma) {
+    API_BEGIN
+    CF_REQUIRE(s, "NULL argument");
+    s->dma = dma != 0;
     API_END
 }
 

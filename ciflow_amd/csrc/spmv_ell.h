@@ -19,12 +19,9 @@ void launch_ell_index(uint64_t db, int Sb, int Eb, const Exc* subb, const Exc* f
                       uint16_t* jfull, hipStream_t s);
 // Largest beta dimension the LDS-staged kernel supports.
 constexpr uint64_t kEllMaxDb = 8192;
-constexpr int kEllMaxParts = 8;
-// threads: workgroup size (256/512/1024); parts: workgroups per (alpha string,
-// beta chunk), 1..8; parts > 1 needs part = parts * n doubles of scratch.
-void launch_spmv_ell_f64(const EllArgs& a, int threads, int parts, const double* ell, const double* x, double* y,
-                         double* part, hipStream_t s);
-void launch_spmv_ell_f32(const EllArgs& a, int threads, int parts, const float* ell, const double* x, double* y,
-                         double* part, hipStream_t s);
+void launch_spmv_ell_f64(const EllArgs& a, int threads, const double* ell, const double* x, double* y,
+                         hipStream_t s);
+void launch_spmv_ell_f32(const EllArgs& a, int threads, const float* ell, const double* x, double* y,
+                         hipStream_t s);
 
 }  // namespace cf

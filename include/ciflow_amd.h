@@ -136,9 +136,6 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * with LDS-staged x blocks (FCI, d_beta <= 8192; builds an extra value copy;
  * tile_rows = workgroup size 256/512/1024). */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
-/* Kernel 4 tuning: workgroup size (256/512/1024) and workgroups per alpha
- * string (1-8; > 1 adds a partial-sum pass and parts*n doubles of scratch). */
-int cf_crs_tune_ell(cf_crs* m, int threads, int parts);
 /* Kernel the next product with value_prec would run (1-4) and its algorithmic
  * HBM bytes (matrix stream + x + y), the roofline numerator. */
 int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* kernel);
@@ -195,6 +192,9 @@ int cf_sigma_destroy(cf_sigma* s);
 /* Workgroup size of the alpha-beta kernel: 128, 256, 512, 768 or 1024 beta strings
  * per workgroup (tuning; no reference counterpart). */
 int cf_sigma_set_threads(cf_sigma* s, int threads);
+/* 1: double-buffered alpha-beta kernel with direct-to-LDS row loads (used
+ * where its two buffers fit in LDS); 0: register-prefetch kernel. */
+int cf_sigma_set_variant(cf_sigma* s, int dma);
 /* y = H x, device pointers on the context stream; value_prec CF_F32 uses
  * FP32 operands with FP64 accumulation of the alpha-beta part. */
 int cf_sigma_apply_dev(cf_sigma* s, int value_prec, const double* dx, double* dy);

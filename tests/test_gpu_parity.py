@@ -359,10 +359,9 @@ def test_vectorised_spmv_matches_scalar_gather(case):
             assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec, kern)
 
 
-@pytest.mark.parametrize("case,nt,parts", [("fci_L8_4a4b_config1", 512, 1), ("fci_beh2", 256, 3),
-                                           ("fci_L7_3a2b", 512, 2), ("fci_L11_5a5b", 512, 1),
-                                           ("fci_L11_5a5b", 1024, 4), ("fci_L12_6a6b", 256, 2)])
-def test_ell_spmv_matches_scalar_gather(case, nt, parts):
+@pytest.mark.parametrize("case,nt", [("fci_L8_4a4b_config1", 512), ("fci_beh2", 256), ("fci_L7_3a2b", 512),
+                                     ("fci_L11_5a5b", 512), ("fci_L11_5a5b", 1024), ("fci_L12_6a6b", 256)])
+def test_ell_spmv_matches_scalar_gather(case, nt):
     """Kernel 4 (position-major complete rows, LDS-staged x blocks): zeros at
     the dropped D2h positions, nalpha != nbeta, several workgroup sizes."""
     say("ell spmv", case, nt)
@@ -379,7 +378,6 @@ def test_ell_spmv_matches_scalar_gather(case, nt, parts):
         m.set_spmv_kernel(1)
         y1 = m.mvprod(x, prec)
         m.set_spmv_kernel(4, nt)
-        m.tune_ell(nt, parts)
         y2 = m.mvprod(x, prec)
         assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec)
     if case == "fci_beh2":  # the solve on kernel 4 (fixture Econst added)
