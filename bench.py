@@ -203,7 +203,7 @@ def bench_stream(a, CA):
     from ciflow_amd import stream as S
     K = a.steps if a.steps is not None else 4096
     W = a.warmup if a.warmup is not None else 64
-    served = dict(precision=CA.CF_F32, check_every=4, tol_energy=1e-9, direct_fci=True)
+    served = dict(precision=CA.CF_F32, check_every=2, tol_energy=1e-9, direct_fci=True)
     reqs = S.make_stream(K, rate=0.0)
     warm = S.make_stream(W, rate=0.0, first_seed=500000, mix_seed=99) if W > 0 else []
     t0 = time.perf_counter()

@@ -165,7 +165,6 @@ def lib():
         "cf_sigma_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
         "cf_sigma_destroy": (C.c_int, [vp]),
         "cf_sigma_set_threads": (C.c_int, [vp, C.c_int]),
-        "cf_sigma_set_variant": (C.c_int, [vp, C.c_int]),
         "cf_sigma_apply_dev": (C.c_int, [vp, C.c_int, vp, vp]),
         "cf_sigma_apply": (C.c_int, [vp, C.c_int, dp, dp]),
         "cf_solve_direct": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
@@ -583,10 +582,6 @@ class FCI_Direct(CIMethod):
 
     def get_name(self):
         return "FCI_Direct"
-
-    def set_variant(self, dma: bool):
-        """Double-buffered alpha-beta kernel with direct-to-LDS row loads (True)."""
-        _check(lib().cf_sigma_set_variant(self.sig.ptr, int(bool(dma))))
 
     def set_threads(self, threads: int):
         """Workgroup size of the alpha-beta sigma kernel (128, 256, 512, 768, 1024)."""

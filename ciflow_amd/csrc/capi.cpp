@@ -340,10 +340,7 @@ struct DeviceBackend : KrylovBackend {
     uint64_t n;
     int M;
     DBuf<double> Vb, w, partial, cvec, c2, Yd, scal, dalpha, dbeta;
-    DBuf<unsigned> counter;  // last-block counters: [0,8) fused multi-dot, [8] axpy+norm (left zero)
-    DBuf<double> partial2;   // block partials of the axpy+norm
-    int nb2 = 1;
-    bool first_ = true;      // next orthogonalisation opens a cycle
+    bool first_ = true;  // next orthogonalisation opens a cycle
     int nb;
     std::function<void(const double*, double*)> op;
     double* hbuf;  // the context's pinned read-back buffer
@@ -354,10 +351,7 @@ struct DeviceBackend : KrylovBackend {
         w.alloc(n);
         nb = multi_dot_blocks(n);
         partial.alloc((size_t)64 * nb);
-        counter.alloc(16);
-        CF_HIP(hipMemsetAsync(counter.p, 0, 16 * sizeof(unsigned), c->stream));
-        nb2 = (int)std::min<uint64_t>(std::max<uint64_t>((n + 255) / 256, 1), 8192);
-        partial2.alloc(nb2);
+
         cvec.alloc(64);
         c2.alloc(64);
         Yd.alloc(64 * 64);
@@ -367,8 +361,12 @@ struct DeviceBackend : KrylovBackend {
         hbuf = c->pinned;
     }
     double* v(int i) { return Vb.p + (size_t)i * n; }
+    // Partials + a separate fixed-order reduction: a last-block reduction inside
+    // one launch needs a device-scope release fence per block, which on this GPU
+    // writes back the XCD's L2 and made the fused form 2.5-10x slower.
     void dots(int k, const double* base, const double* x, double* out) {
-        launch_multi_dot_fused(n, k, base, x, partial.p, nb, counter.p, out, c->stream);
+        launch_multi_dot(n, k, base, x, partial.p, nb, c->stream);
+        launch_reduce_partials(k, nb, partial.p, out, c->stream);
     }
     double fetch(const double* d) {
         CF_HIP(hipMemcpyAsync(hbuf, d, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -390,7 +388,8 @@ struct DeviceBackend : KrylovBackend {
         }
         first_ = false;
         dots(j + 1, Vb.p, w.p, c2.p);
-        launch_multi_axpy_norm(n, j + 1, Vb.p, c2.p, w.p, partial2.p, nb2, counter.p + 8, scal.p, c->stream);
+        launch_multi_axpy(n, j + 1, Vb.p, c2.p, w.p, c->stream);
+        dots(1, w.p, w.p, scal.p);
         launch_finish_step(n, j, cvec.p, c2.p, scal.p, w.p, v(j + 1), dalpha.p, dbeta.p, c->stream);
     }
     void fetch_coeffs(int j0, int j1, double* alpha, double* beta) override {
@@ -1133,7 +1132,6 @@ struct cf_sigma {
     rocblas_handle blas = nullptr;  // the context's handle
     SigmaArgs args;
     int threads = 1024;  // workgroup size of the alpha-beta kernel
-    bool dma = false;    // double-buffered variant with direct-to-LDS row loads
 };
 
 namespace {
@@ -1161,16 +1159,14 @@ void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
         CF_BLAS(rocblas_sgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb32, m,
                               S->x32.p, m, &one, S->y32.p, m));
         launch_to_f64(sp->n, S->y32.p, y, c->stream);
-        launch_sigma_ab<float>(S->args, S->threads, S->dma && sigma_ab_dma_lds_bytes(S->args, true) <= 160 * 1024,
-                               S->x32.p, y, c->stream);
+        launch_sigma_ab<float>(S->args, S->threads, S->x32.p, y, c->stream);
     } else {
         const double one = 1.0, zero = 0.0;
         CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, x, m,
                               S->Ha.p, nn, &zero, y, m));
         CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb, m, x,
                               m, &one, y, m));
-        launch_sigma_ab<double>(S->args, S->threads, S->dma && sigma_ab_dma_lds_bytes(S->args, false) <= 160 * 1024,
-                                x, y, c->stream);
+        launch_sigma_ab<double>(S->args, S->threads, x, y, c->stream);
     }
 }
 }  // namespace
@@ -1251,13 +1247,6 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     CF_REQUIRE(sigma_ab_lds_bytes(a, false) <= 160 * 1024, "beta string space too large for the LDS row");
     CF_HIP(hipStreamSynchronize(c->stream));
     *out = S.release();
-    API_END
-}
-
-int cf_sigma_set_variant(cf_sigma* s, int dma) {
-    API_BEGIN
-    CF_REQUIRE(s, "NULL argument");
-    s->dma = dma != 0;
     API_END
 }
 

@@ -112,16 +112,10 @@ void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, doub
                       hipStream_t stream);
 void launch_reduce_partials(int k, int nblocks, const double* partial, double* out, hipStream_t stream);
 int multi_dot_blocks(uint64_t n);
-// out[i] = Vb[:, i] . w for i < k (k <= 64), partials + last-block reduction in
-// one launch; counter: 8 zero-initialised words, left zero on return.
-void launch_multi_dot_fused(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
-                            unsigned* counter, double* out, hipStream_t stream);
+
 void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream);
 void launch_local3(uint64_t n, const double* a, const double* v1, const double* b, const double* v0, double* w,
                    hipStream_t stream);
-// w -= Vb[:, 0:k] c and *out = |w|^2 (partial: nblocks doubles; counter: one zeroed word, left zero)
-void launch_multi_axpy_norm(uint64_t n, int k, const double* Vb, const double* c, double* w, double* partial,
-                            int nblocks, unsigned* counter, double* out, hipStream_t stream);
 void launch_finish_step(uint64_t n, int j, const double* c1, const double* c2, const double* wnorm2,
                         const double* w, double* vnext, double* alpha, double* beta, hipStream_t stream);
 void launch_scale_copy(uint64_t n, const double* src, double s, double* dst, hipStream_t stream);

@@ -51,68 +51,6 @@ __global__ void __launch_bounds__(kDotThreads) k_multi_dot(uint64_t n, int k, co
     }
 }
 
-// Multi-dot with the final reduction in the same launch: every block writes its
-// partials, and the last block of each row of 8 vectors (found with a counter,
-// which it resets for the next launch) sums them in block order, so the result
-// does not depend on which block finishes last.
-__global__ void __launch_bounds__(kDotThreads) k_multi_dot_fused(uint64_t n, int k, const double* __restrict__ Vb,
-                                                                 const double* __restrict__ w, double* partial,
-                                                                 unsigned* counter, double* __restrict__ out) {
-    __shared__ double red[kDotChunk][kDotThreads / 64];
-    __shared__ bool last;
-    const int i0 = blockIdx.y * kDotChunk;
-    const int nb = gridDim.x;
-    double acc[kDotChunk];
-#pragma unroll
-    for (int j = 0; j < kDotChunk; ++j) acc[j] = 0.0;
-    const uint64_t stride = (uint64_t)nb * kDotThreads;
-    for (uint64_t q = (uint64_t)blockIdx.x * kDotThreads + threadIdx.x; q < n; q += stride) {
-        const double wq = w[q];
-#pragma unroll
-        for (int j = 0; j < kDotChunk; ++j)
-            if (i0 + j < k) acc[j] += Vb[(uint64_t)(i0 + j) * n + q] * wq;
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int j = 0; j < kDotChunk; ++j) {
-        double s = wave_sum(acc[j]);
-        if (lane == 0) red[j][wid] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < kDotChunk && i0 + (int)threadIdx.x < k) {
-        double s = 0.0;
-        for (int q = 0; q < kDotThreads / 64; ++q) s += red[threadIdx.x][q];
-        partial[(uint64_t)(i0 + threadIdx.x) * nb + blockIdx.x] = s;
-    }
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(&counter[blockIdx.y], 1u);
-        last = prev == (unsigned)nb - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // 4 waves: wave j handles vectors j and j+4 of this row, lanes stride the blocks
-    for (int jj = wid; jj < kDotChunk; jj += kDotThreads / 64) {
-        const int i = i0 + jj;
-        if (i >= k) continue;
-        double s = 0.0;
-        for (int b = lane; b < nb; b += 64) s += partial[(uint64_t)i * nb + b];
-        s = wave_sum(s);
-        if (lane == 0) out[i] = s;
-    }
-    if (threadIdx.x == 0) counter[blockIdx.y] = 0u;
-}
-
-void launch_multi_dot_fused(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
-                            unsigned* counter, double* out, hipStream_t stream) {
-    if (k <= 0) return;
-    dim3 grid((unsigned)nblocks, (unsigned)((k + kDotChunk - 1) / kDotChunk));
-    k_multi_dot_fused<<<grid, kDotThreads, 0, stream>>>(n, k, Vb, w, partial, counter, out);
-    CF_HIP(hipGetLastError());
-}
-
 void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
                       hipStream_t stream) {
     if (k <= 0) return;
@@ -176,54 +114,6 @@ void launch_local3(uint64_t n, const double* a, const double* v1, const double* 
     uint64_t grid = (n + 255) / 256;
     if (grid > 8192) grid = 8192;
     k_local3<<<(unsigned)grid, 256, 0, stream>>>(n, a, v1, b, v0, w);
-    CF_HIP(hipGetLastError());
-}
-
-// w -= Vb[:, 0:k] c, and out = |w|^2 of the result: block partials plus a
-// last-block reduction in block order (deterministic), counter left zero.
-__global__ void __launch_bounds__(256) k_multi_axpy_norm(uint64_t n, int k, const double* __restrict__ Vb,
-                                                         const double* __restrict__ c, double* __restrict__ w,
-                                                         double* partial, unsigned* counter,
-                                                         double* __restrict__ out) {
-    __shared__ double cs[64];
-    __shared__ double red[4];
-    __shared__ bool last;
-    if (threadIdx.x < 64) cs[threadIdx.x] = (int)threadIdx.x < k ? c[threadIdx.x] : 0.0;
-    __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    double nrm = 0.0;
-    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
-        double s = 0.0;
-        for (int i = 0; i < k; ++i) s += cs[i] * Vb[(uint64_t)i * n + q];
-        const double wq = w[q] - s;
-        w[q] = wq;
-        nrm += wq * wq;
-    }
-    nrm = wave_sum(nrm);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) red[wid] = nrm;
-    __syncthreads();
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last || wid != 0) return;
-    __threadfence();
-    double t = 0.0;
-    for (uint32_t b2 = lane; b2 < gridDim.x; b2 += 64) t += partial[b2];
-    t = wave_sum(t);
-    if (lane == 0) {
-        *out = t;
-        *counter = 0u;
-    }
-}
-
-void launch_multi_axpy_norm(uint64_t n, int k, const double* Vb, const double* c, double* w, double* partial,
-                            int nblocks, unsigned* counter, double* out, hipStream_t stream) {
-    if (k > 64) throw Error(ERR_ARG, "multi_axpy_norm: k > 64");
-    if (!n || nblocks < 1) return;
-    k_multi_axpy_norm<<<(unsigned)nblocks, 256, 0, stream>>>(n, k, Vb, c, w, partial, counter, out);
     CF_HIP(hipGetLastError());
 }
 

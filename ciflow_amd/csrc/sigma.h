@@ -21,11 +21,8 @@ void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream
 size_t sigma_ab_lds_bytes(const SigmaArgs& a, bool f32);
 // threads: workgroup size of the alpha-beta kernel (128, 256, 512, 768 or 1024);
 // a workgroup owns one alpha string and `threads` beta strings.
-// dma: the double-buffered variant with direct-to-LDS row loads (needs
-// sigma_ab_dma_lds_bytes <= 160 KB).
 template <typename CT>
-void launch_sigma_ab(const SigmaArgs& a, int threads, bool dma, const CT* C, double* sig, hipStream_t s);
-size_t sigma_ab_dma_lds_bytes(const SigmaArgs& a, bool f32);
+void launch_sigma_ab(const SigmaArgs& a, int threads, const CT* C, double* sig, hipStream_t s);
 void launch_to_f32(uint64_t n, const double* x, float* y, hipStream_t s);
 void launch_to_f64(uint64_t n, const float* x, double* y, hipStream_t s);
 void launch_row_dots(uint64_t da, uint64_t db, int S, const Exc* sub, const double* C, double* out, hipStream_t s);

@@ -174,122 +174,17 @@ __global__ void __launch_bounds__(NT) k_sigma_ab(SigmaArgs a, const CT* __restri
     if (act) sig[Ia * db + Ib] += acc;
 }
 
-// Row copy global -> LDS with no register round trip (gfx950 global_load_lds):
-// each instruction moves 64 dwords of one wave, lane i to dst + 4 i; the
-// destination holds whole 64-dword chunks (padding lanes re-read the last word).
-template <int NT>
-__device__ __forceinline__ void row_to_lds(const uint32_t* __restrict__ src, uint32_t* dst, uint32_t ndw) {
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t c = wave; c * 64 < ndw; c += NT / 64) {
-        const uint32_t i = c * 64 + lane;
-        __builtin_amdgcn_global_load_lds(src + (i < ndw ? i : ndw - 1), dst + c * 64, 4, 0, 0);
-    }
-}
-
-// As k_sigma_ab, with both staged objects double-buffered: while the threads
-// gather action e from buffers e&1, the next row is already moving into the
-// other buffer by direct-to-LDS loads and the next V slice is written there,
-// so each action costs one barrier and the row fetch is off the critical path.
-template <typename CT, int NT>
-__global__ void __launch_bounds__(NT) k_sigma_ab_dma(SigmaArgs a, const CT* __restrict__ C, double* __restrict__ sig) {
-    extern __shared__ unsigned char smem[];
-    const uint64_t db = a.db;
-    const uint32_t nchunk = a.nchunk;
-    const uint32_t per = gridDim.x >> 3;
-    const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-    if (lb >= a.nblocks) return;
-    const uint64_t Ia = lb / nchunk;
-    const uint64_t Ib = (uint64_t)(lb - Ia * nchunk) * NT + threadIdx.x;
-    const bool act = Ib < db;
-    const int L = a.L;
-    const uint32_t ndw = (uint32_t)(db * sizeof(CT) / 4);
-    const uint32_t rowdw = (ndw + 63) / 64 * 64;
-    double* Vsb[2];
-    Vsb[0] = (double*)smem;
-    Vsb[1] = Vsb[0] + (L * L + 2);
-    uint32_t* rb[2];
-    rb[0] = (uint32_t*)(Vsb[1] + (L * L + 2));
-    rb[1] = rb[0] + rowdw;
-    uint32_t sb[kMaxSingles + 1];
-    const int Sb = a.Sb;
-    str_t sbits = 0;
-    if (act) {
-        sbits = a.sb[Ib];
-        const Exc* lb_ = a.subb + Ib * (uint64_t)Sb;
-#pragma unroll
-        for (int e = 0; e < kMaxSingles + 1; ++e) {
-            if (e < Sb) {
-                const Exc x = lb_[e];
-                sb[e] = x.type == 1
-                            ? (x.J << 11) | ((uint32_t)x.f << 6) | ((uint32_t)x.t << 1) | (x.sign < 0 ? 1u : 0u)
-                            : ((uint32_t)Ib << 11) | ((uint32_t)L << 6);
-            }
-        }
-    }
-    const Exc* la = a.suba + Ia * (uint64_t)a.Sa;
-    const str_t Sa = a.sa[Ia];
-    stage_vslice<NT>(a, la[0], Sa, Vsb[0]);
-    row_to_lds<NT>((const uint32_t*)(C + (uint64_t)la[0].J * db), rb[0], ndw);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    double acc = 0.0;
-    for (int e = 0; e < a.Sa; ++e) {
-        const int cur = e & 1, nxt = cur ^ 1;
-        if (e + 1 < a.Sa) {
-            const Exc x1 = la[e + 1];
-            row_to_lds<NT>((const uint32_t*)(C + (uint64_t)x1.J * db), rb[nxt], ndw);
-            stage_vslice<NT>(a, x1, Sa, Vsb[nxt]);
-        }
-        if (act) {
-            const double* Vs = Vsb[cur];
-            const CT* row = (const CT*)rb[cur];
-            double t = 0.0;
-            double dsum = 0.0;
-            for (str_t u = sbits; u; u &= u - 1) {
-                const int q = __builtin_ctzll(u);
-                dsum += Vs[q * L + q];
-            }
-            t += dsum * (double)row[Ib];
-#pragma unroll
-            for (int j = 0; j < kMaxSingles + 1; ++j) {
-                if (j < Sb) {
-                    const uint32_t c = sb[j];
-                    const int q = (c >> 6) & 31, s = (c >> 1) & 31;
-                    double v = Vs[q * L + s] * (double)row[c >> 11];
-                    t += (c & 1) ? -v : v;
-                }
-            }
-            acc += t;
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-    }
-    if (act) sig[Ia * db + Ib] += acc;
-}
-
-size_t sigma_ab_dma_lds_bytes(const SigmaArgs& a, bool f32) {
-    const size_t ndw = a.db * (f32 ? 4 : 8) / 4, rowdw = (ndw + 63) / 64 * 64;
-    return 2 * ((size_t)a.L * a.L + 2) * sizeof(double) + 2 * rowdw * 4;
-}
-
 size_t sigma_ab_lds_bytes(const SigmaArgs& a, bool f32) {
     return ((size_t)a.L * a.L + 2) * sizeof(double) + a.db * (f32 ? sizeof(float) : sizeof(double));
 }
 
 template <typename CT, int NT>
-static void launch_nt(SigmaArgs a, const CT* C, double* sig, bool dma, hipStream_t s) {
+static void launch_nt(SigmaArgs a, const CT* C, double* sig, hipStream_t s) {
     a.nchunk = (uint32_t)((a.db + NT - 1) / NT);
     const uint64_t nb = a.da * (uint64_t)a.nchunk;
     if (nb > 0xFFFFFFF0ull) throw Error(ERR_UNSUPPORTED, "too many sigma workgroups");
     a.nblocks = (uint32_t)nb;
     const uint64_t grid = (nb + 7) / 8 * 8;
-    if (dma) {
-        const size_t lds2 = sigma_ab_dma_lds_bytes(a, sizeof(CT) == 4);
-        if (lds2 > 160 * 1024) throw Error(ERR_UNSUPPORTED, "double-buffered sigma does not fit in LDS");
-        k_sigma_ab_dma<CT, NT><<<(unsigned)grid, NT, lds2, s>>>(a, C, sig);
-        CF_HIP(hipGetLastError());
-        return;
-    }
     const size_t lds = sigma_ab_lds_bytes(a, sizeof(CT) == 4);
     const uint64_t per_thread = (a.db + NT - 1) / NT;
     if (per_thread <= 4)
@@ -302,18 +197,18 @@ static void launch_nt(SigmaArgs a, const CT* C, double* sig, bool dma, hipStream
 }
 
 template <typename CT>
-void launch_sigma_ab(const SigmaArgs& a, int threads, bool dma, const CT* C, double* sig, hipStream_t s) {
+void launch_sigma_ab(const SigmaArgs& a, int threads, const CT* C, double* sig, hipStream_t s) {
     switch (threads) {
-        case 128: launch_nt<CT, 128>(a, C, sig, dma, s); break;
-        case 256: launch_nt<CT, 256>(a, C, sig, dma, s); break;
-        case 512: launch_nt<CT, 512>(a, C, sig, dma, s); break;
-        case 768: launch_nt<CT, 768>(a, C, sig, dma, s); break;
-        case 1024: launch_nt<CT, 1024>(a, C, sig, dma, s); break;
+        case 128: launch_nt<CT, 128>(a, C, sig, s); break;
+        case 256: launch_nt<CT, 256>(a, C, sig, s); break;
+        case 512: launch_nt<CT, 512>(a, C, sig, s); break;
+        case 768: launch_nt<CT, 768>(a, C, sig, s); break;
+        case 1024: launch_nt<CT, 1024>(a, C, sig, s); break;
         default: throw Error(ERR_ARG, "sigma workgroup size must be 128, 256, 512, 768 or 1024");
     }
 }
-template void launch_sigma_ab<double>(const SigmaArgs&, int, bool, const double*, double*, hipStream_t);
-template void launch_sigma_ab<float>(const SigmaArgs&, int, bool, const float*, double*, hipStream_t);
+template void launch_sigma_ab<double>(const SigmaArgs&, int, const double*, double*, hipStream_t);
+template void launch_sigma_ab<float>(const SigmaArgs&, int, const float*, double*, hipStream_t);
 
 __global__ void k_to_f32(uint64_t n, const double* __restrict__ x, float* __restrict__ y) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

@@ -192,9 +192,6 @@ int cf_sigma_destroy(cf_sigma* s);
 /* Workgroup size of the alpha-beta kernel: 128, 256, 512, 768 or 1024 beta strings
  * per workgroup (tuning; no reference counterpart). */
 int cf_sigma_set_threads(cf_sigma* s, int threads);
-/* 1: double-buffered alpha-beta kernel with direct-to-LDS row loads (used
- * where its two buffers fit in LDS); 0: register-prefetch kernel. */
-int cf_sigma_set_variant(cf_sigma* s, int dma);
 /* y = H x, device pointers on the context stream; value_prec CF_F32 uses
  * FP32 operands with FP64 accumulation of the alpha-beta part. */
 int cf_sigma_apply_dev(cf_sigma* s, int value_prec, const double* dx, double* dy);
