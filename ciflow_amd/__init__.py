@@ -130,6 +130,7 @@ def lib():
         "cf_ctx_enable_timing": (C.c_int, [vp, C.c_int]),
         "cf_ctx_kernel_time": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_int64), dp]),
         "cf_ctx_reset_timing": (C.c_int, [vp]),
+        "cf_ctx_stream_copy": (C.c_int, [vp, C.c_uint64, C.c_int]),
         "cf_gen_integrals": (C.c_int, [C.c_int, C.c_uint64, dp, dp]),
         "cf_ham_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, C.POINTER(vp)]),
         "cf_ham_destroy": (C.c_int, [vp]),

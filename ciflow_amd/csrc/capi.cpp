@@ -620,6 +620,18 @@ int cf_ctx_kernel_time(cf_ctx* ctx, const char* family, int64_t* launches, doubl
     API_END
 }
 
+int cf_ctx_stream_copy(cf_ctx* ctx, uint64_t bytes, int reps) {
+    API_BEGIN
+    CF_REQUIRE(ctx && bytes >= 8 && reps >= 1, "bad arguments");
+    ctx->use();
+    const uint64_t n = bytes / 8;
+    DBuf<double> a(n), b(n);
+    CF_HIP(hipMemsetAsync(a.p, 0, n * 8, ctx->stream));
+    for (int r = 0; r < reps; ++r) launch_copy(n, a.p, b.p, ctx->stream);
+    CF_HIP(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
 int cf_ctx_reset_timing(cf_ctx* ctx) {
     API_BEGIN
     CF_REQUIRE(ctx, "ctx is NULL");

@@ -191,6 +191,8 @@ static void launch_nt(SigmaArgs a, const CT* C, double* sig, hipStream_t s) {
         k_sigma_ab<CT, 4, NT><<<(unsigned)grid, NT, lds, s>>>(a, C, sig);
     else if (per_thread <= kMaxPrefetch)
         k_sigma_ab<CT, kMaxPrefetch, NT><<<(unsigned)grid, NT, lds, s>>>(a, C, sig);
+    else if (sizeof(CT) == 4 && per_thread <= 17)  // FP32 rows, L = 16 at 768 threads (17 per thread)
+        k_sigma_ab<CT, 17, NT><<<(unsigned)grid, NT, lds, s>>>(a, C, sig);
     else
         k_sigma_ab<CT, 0, NT><<<(unsigned)grid, NT, lds, s>>>(a, C, sig);
     CF_HIP(hipGetLastError());

@@ -58,6 +58,10 @@ int cf_ctx_synchronize(cf_ctx* ctx);
 int cf_ctx_enable_timing(cf_ctx* ctx, int enable);
 int cf_ctx_kernel_time(cf_ctx* ctx, const char* family, int64_t* launches, double* total_ms);
 int cf_ctx_reset_timing(cf_ctx* ctx);
+/* A plain device-to-device copy of `bytes` (bytes read + bytes written) run
+ * `reps` times on the context stream: the reference kernel of known traffic
+ * for scaling hardware memory counters.  No reference counterpart. */
+int cf_ctx_stream_copy(cf_ctx* ctx, uint64_t bytes, int reps);
 
 /* ---- integrals: replaces Hamiltonian::getTmat/getVmat/getEconst/getL/
  *      getnup/getndown (Hamiltonian.h:80-136, Hamiltonian.cpp:267-320).
