@@ -165,6 +165,24 @@ def cpu_baseline_stream(reqs, nmv_by_class, lines=96):
 # ---------------------------------------------------------------------------
 # SpMV roofline on the stored CRS (FCI L=12 6a6b)
 # ---------------------------------------------------------------------------
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "spmv_ell_fci_L12_traffic.txt")
+
+
+def measured_traffic(prec_name):
+    """HBM bytes per launch of the FCI L=12 position-major SpMV from the two
+    rocprofv3 --pmc passes summarised in profiles/ (FETCH_SIZE and WRITE_SIZE,
+    scaled by a 2 GiB copy); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for ln in f:
+                if ln.startswith("traffic_bytes_per_launch"):
+                    kv = dict(t.split("=") for t in ln.split()[1:])
+                    return float(kv[prec_name])
+    except (OSError, KeyError, ValueError):
+        return None
+    return None
+
+
 def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
     h, V = CA.gen_integrals(L, seed)
     ham = CA.Hamiltonian(h, V, nel, nel, 0.0, ctx)
@@ -173,6 +191,8 @@ def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
     n, nnz_full, _ = m.info()
     x = np.random.default_rng(0).standard_normal(n)
     m.mvprod(x)  # warm
+    tab = m.autotune(CA.CF_F64, reps=3)  # installs the fastest launch shape
+    best = min(tab, key=lambda t: t[3])
     ctx.synchronize()
     ctx.enable_timing(True)
     ctx.reset_timing()
@@ -184,13 +204,17 @@ def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
     kern, algo = m.spmv_bytes(CA.CF_F64)
     ach = algo / (avg_ms * 1e-3) / 1e9
     names = {1: "k_spmv (CRS row gather)", 3: "k_spmv_v4 (CRS row gather, vectorised streams)",
-             4: "k_spmv_ell (FCI position-major complete rows, LDS-staged x blocks, FP64 values)"}
+             4: "k_spmv_ell2 (FCI position-major complete rows, double-buffered LDS x blocks, FP64 values)"}
     crs_bytes = nnz_full * 12 + (n + 1) * 8 + 16 * n
     out = {"bound": "hbm", "kernel": names.get(kern, str(kern)), "crs_equivalent_bytes": crs_bytes,
            "crs_equivalent_GBps": crs_bytes / (avg_ms * 1e-3) / 1e9,
            "workload": f"FCI L={L} {nel}a{nel}b seed {seed}: n={n}, nnz={nnz_full}, one FP64 Lanczos solve",
-           "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+           "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+           "traffic": measured_traffic("f64") if (L, nel, kern) == (12, 6, 4) else None,
+           "traffic_source": "profiles/spmv_ell_fci_L12_traffic.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)",
            "avg_launch_ms": avg_ms, "launches": nl, "algorithmic_bytes": algo,
+           "autotune": {"candidates": len(tab), "kernel": best[0], "p1": best[1], "p2": best[2],
+                        "ms": best[3]},
            "solve_energy": cim.get_ci_energy(), "solve_n_matvec": info.n_matvec}
     del cim, m
     return out
@@ -382,7 +406,10 @@ def bench_fci12(a, CA):
         "rdm_trace": tr,
         "roofline": {"bound": "hbm", "kernel": f"{fam} (kernel {kern})", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (measured_traffic("f32" if sval == 4 else "f64")
+                                 if (not doci and a.L == 12 and a.nel == 6 and kern == 4) else None),
+                     "avg_launch_ms": avg_ms,
                      "launches": nl, "algorithmic_bytes": algo_bytes},
         "cpu_baseline": cpu,
     }

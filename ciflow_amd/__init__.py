@@ -75,6 +75,11 @@ class CIFlowError(RuntimeError):
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_double, C.c_double)
 
 
+class TuneEntry(C.Structure):
+    """cf_tune_entry: one timed SpMV launch shape."""
+    _fields_ = [("kernel", C.c_int), ("p1", C.c_int), ("p2", C.c_int), ("ms", C.c_double)]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("ncv", C.c_int), ("maxrestart", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
                 ("tol_f32", C.c_double), ("seed", C.c_uint64), ("check_every", C.c_int),
@@ -158,6 +163,8 @@ def lib():
         "cf_rci_destroy": (C.c_int, [vp]),
         "cf_rdm": (C.c_int, [vp, dp, dp, dp]),
         "cf_crs_set_spmv_kernel": (C.c_int, [vp, C.c_int, C.c_int]),
+        "cf_crs_set_spmv_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "cf_crs_autotune": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(TuneEntry), C.c_int, C.POINTER(C.c_int)]),
         "cf_rotate_integrals": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp, dp]),
         "cf_ham_rotate": (C.c_int, [vp, dp, C.POINTER(vp)]),
         "cf_ham_get": (C.c_int, [vp, dp, dp]),
@@ -401,8 +408,22 @@ class SparseMatrix_CRS_Sym:
         return k.value, b.value
 
     def set_spmv_kernel(self, kind: int, tile_rows: int = 0):
-        """0 automatic, 1 plain row gather, 2 FCI alpha-block staged."""
+        """0 automatic, 1 plain row gather, 2 FCI alpha-block staged, 3 vectorised
+        row gather, 4 FCI position-major rows (tile_rows = workgroup size)."""
         _check(lib().cf_crs_set_spmv_kernel(self.ptr, kind, tile_rows))
+
+    def set_spmv_shape(self, precision: int, kernel: int, p1: int, p2: int = 0):
+        """Launch shape for one value precision (see cf_crs_set_spmv_shape)."""
+        _check(lib().cf_crs_set_spmv_shape(self.ptr, precision, kernel, p1, p2))
+
+    def autotune(self, precision: int = CF_F64, reps: int = 3):
+        """Times every supported launch shape, installs the fastest for this
+        value precision and returns [(kernel, p1, p2, ms), ...]."""
+        cap = 64
+        tab = (TuneEntry * cap)()
+        cnt = C.c_int()
+        _check(lib().cf_crs_autotune(self.ptr, precision, reps, tab, cap, C.byref(cnt)))
+        return [(t.kernel, t.p1, t.p2, t.ms) for t in tab[:min(cnt.value, cap)]]
 
     def mvprod(self, x: np.ndarray, precision: int = CF_F64) -> np.ndarray:
         """y = A x (SparseMatrix_CRS_Sym::mvprod); precision CF_F32 uses the FP32 values."""

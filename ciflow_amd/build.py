@@ -35,7 +35,7 @@ def _compile(src: str) -> str:
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", path, "-o", obj]
     else:
-        cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + COMMON + [f"--offload-arch={ARCH}", "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -9,6 +9,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "kernels.h"
@@ -196,6 +197,15 @@ struct cf_space {
     bool vec_valid = false;
 };
 
+// One SpMV launch configuration: kernel 1 (row gather, `lanes` per row),
+// 3 (vectorised row gather, `lanes` per row, `unroll`) or 4 (FCI ELL, `ell`).
+struct SpmvChoice {
+    int kernel = 1;
+    int lanes = 0;
+    int unroll = 0;
+    EllShape ell;
+};
+
 struct cf_crs {
     cf_space* sp = nullptr;
     uint64_t n = 0, nnz = 0;
@@ -206,16 +216,16 @@ struct cf_crs {
     DBuf<float> val32;
     int lpr = 64;     // lanes per row of the scalar row gather
     int kernel = 0;   // 0 auto, 1 row gather, 2 FCI staged, 3 vectorised row gather, 4 FCI ELL
-    // automatic choice per value precision: vectorised (lanes, unroll) or 0 = scalar gather
-    int v4_lpr64 = 0, v4_unr64 = 0, v4_lpr32 = 0, v4_unr32 = 0;
+    // automatic choice per value precision (row-length heuristic, or the autotuner's pick)
+    SpmvChoice auto64, auto32;
     // FCI position-major complete rows (kernel 4), built on demand
     FciRowsArgs fargs{};
     bool has_fargs = false;
     DBuf<double> ell;
     DBuf<float> ell32;
     DBuf<uint16_t> jsub, jfull;
+    DBuf<uint2> ent;
     EllArgs eargs;
-    int ell_threads = 512;
     bool ell_ready = false;
     std::unique_ptr<FciSpmvPlan> plan;
 };
@@ -251,31 +261,49 @@ void ensure_staging(cf_ctx* c, uint64_t n) {
     if (c->hy.n < n) c->hy.alloc(n);
 }
 
-void spmv(cf_crs* m, int prec, const double* dx, double* dy) {
+// The configuration spmv() runs for a value precision: the explicit kernel if
+// one was set, else the automatic choice.
+SpmvChoice resolve_spmv(const cf_crs* m, int prec) {
+    const SpmvChoice& a = prec == CF_F32 ? m->auto32 : m->auto64;
+    SpmvChoice r = a;
+    if (m->kernel) r.kernel = m->kernel;
+    if (r.kernel == 2 && !m->plan) r.kernel = 1;
+    if (r.kernel == 4 && !m->ell_ready) r.kernel = 1;
+    if (r.kernel == 1) r.lanes = a.kernel == 1 && a.lanes && !m->kernel ? a.lanes : m->lpr;
+    if (r.kernel == 3 && a.kernel != 3) {
+        r.lanes = 16;
+        r.unroll = 2;
+    }
+    return r;
+}
+
+void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, double* dy) {
     cf_ctx* c = m->sp->ham->ctx;
-    const bool staged = m->kernel == 2 && m->plan;
-    const bool v4_64 = m->kernel == 3 || (m->kernel == 0 && m->v4_lpr64);
-    const bool v4_32 = m->kernel == 3 || (m->kernel == 0 && m->v4_lpr32);
-    const bool ell = (m->kernel == 4 || m->kernel == 0) && m->ell_ready;
     if (prec == CF_F32) {
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
-        if (ell) launch_spmv_ell_f32(m->eargs, m->ell_threads, m->ell32.p, dx, dy, c->stream);
-        else if (staged) launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream);
-        else if (v4_32)
-            launch_spmv_v4_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->v4_lpr32 ? m->v4_lpr32 : 16,
-                               m->v4_unr32 ? m->v4_unr32 : 2, c->stream);
-        else launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, m->lpr, c->stream);
+        switch (ch.kernel) {
+            case 4: launch_spmv_ell_f32(m->eargs, ch.ell, m->ell32.p, dx, dy, c->stream); break;
+            case 2: launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream); break;
+            case 3:
+                launch_spmv_v4_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, ch.lanes, ch.unroll, c->stream);
+                break;
+            default: launch_spmv_f32(m->n, m->rowptr.p, m->col.p, m->val32.p, dx, dy, ch.lanes, c->stream);
+        }
     } else {
         KTimer t(c, "spmv_f64");
-        if (ell) launch_spmv_ell_f64(m->eargs, m->ell_threads, m->ell.p, dx, dy, c->stream);
-        else if (staged) launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream);
-        else if (v4_64)
-            launch_spmv_v4_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->v4_lpr64 ? m->v4_lpr64 : 16,
-                               m->v4_unr64 ? m->v4_unr64 : 2, c->stream);
-        else launch_spmv_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, m->lpr, c->stream);
+        switch (ch.kernel) {
+            case 4: launch_spmv_ell_f64(m->eargs, ch.ell, m->ell.p, dx, dy, c->stream); break;
+            case 2: launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream); break;
+            case 3:
+                launch_spmv_v4_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, ch.lanes, ch.unroll, c->stream);
+                break;
+            default: launch_spmv_f64(m->n, m->rowptr.p, m->col.p, m->val.p, dx, dy, ch.lanes, c->stream);
+        }
     }
 }
+
+void spmv(cf_crs* m, int prec, const double* dx, double* dy) { spmv_with(m, prec, resolve_spmv(m, prec), dx, dy); }
 
 // Position-major complete FCI rows + beta index tables (kernel 4).
 void make_ell(cf_crs* m) {
@@ -289,6 +317,8 @@ void make_ell(cf_crs* m) {
     m->jsub.alloc((uint64_t)Sb * sp->db);
     m->jfull.alloc((uint64_t)Eb * sp->db);
     launch_ell_index(sp->db, Sb, Eb, sp->tb->sub.p, sp->tb->full.p, m->jsub.p, m->jfull.p, c->stream);
+    m->ent.alloc(sp->da * (uint64_t)Ea);
+    launch_ell_entries(sp->da, Ea, nda, nsa, sp->ta->full.p, sp->segoff.p, m->ent.p, c->stream);
     m->ell.alloc(m->n * R);
     if (m->val32.p) m->ell32.alloc(m->n * R);
     {
@@ -305,6 +335,10 @@ void make_ell(cf_crs* m) {
     e.segoff = sp->segoff.p;
     e.jsub = m->jsub.p;
     e.jfull = m->jfull.p;
+    e.ent = m->ent.p;
+    e.nda = nda;
+    e.nsa = nsa;
+    e.R = R;
     m->ell_ready = true;
 }
 
@@ -547,6 +581,42 @@ struct cf_rci {
 // ---------------------------------------------------------------------------
 // API
 // ---------------------------------------------------------------------------
+namespace {
+void check_shape(const cf_crs* m, const SpmvChoice& ch) {
+    if (ch.kernel == 1) {
+        CF_REQUIRE(ch.lanes == 8 || ch.lanes == 16 || ch.lanes == 32 || ch.lanes == 64,
+                   "kernel 1: lanes per row must be 8, 16, 32 or 64");
+    } else if (ch.kernel == 3) {
+        CF_REQUIRE(ch.lanes == 4 || ch.lanes == 8 || ch.lanes == 16 || ch.lanes == 32,
+                   "kernel 3: lanes per row must be 4, 8, 16 or 32");
+        CF_REQUIRE(ch.unroll == 1 || ch.unroll == 2, "kernel 3: unroll must be 1 or 2");
+    } else if (ch.kernel == 4) {
+        CF_REQUIRE(m->ell_ready, "kernel 4 needs the FCI position-major copy (cf_crs_set_spmv_kernel(m, 4, 0))");
+        const int t = ch.ell.threads, v = ch.ell.vec;
+        CF_REQUIRE(v == 0 || v == 1 || v == 2 || v == 4, "kernel 4: strings per thread must be 0, 1, 2 or 4");
+        if (v == 0) CF_REQUIRE(t == 256 || t == 512 || t == 1024, "kernel 4, vec 0: workgroup size 256, 512 or 1024");
+        else CF_REQUIRE(t % 64 == 0 && t >= 64 && t <= 1024, "kernel 4: workgroup size must be a multiple of 64 <= 1024");
+        CF_REQUIRE(v == 0 || m->sp->db % v == 0, "kernel 4: d_beta must be a multiple of the strings per thread");
+        CF_REQUIRE(v == 0 || (m->sp->db + t - 1) / t <= 16, "kernel 4: workgroup too small for d_beta");
+    } else {
+        CF_REQUIRE(false, "shape kernel must be 1, 3 or 4");
+    }
+}
+
+SpmvChoice make_choice(int kernel, int p1, int p2, uint64_t db, int prec) {
+    SpmvChoice ch;
+    ch.kernel = kernel;
+    if (kernel == 4) {
+        ch.ell.threads = p1 > 0 ? p1 : ell_default_shape(db, prec == CF_F32 ? 4 : 8).threads;
+        ch.ell.vec = p2;
+    } else {
+        ch.lanes = p1;
+        ch.unroll = p2;
+    }
+    return ch;
+}
+}  // namespace
+
 extern "C" {
 
 int cf_version(void) { return 100; }
@@ -864,24 +934,19 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     // measured on MI355X (FCI L=11/12, DOCI L=20/24): vectorised streams win for
     // FP32 values at every row length and for FP64 rows of >= 512 entries
     const double avg = m->n ? (double)m->nnz / (double)m->n : 0.0;
-    if (avg >= 512) {
-        m->v4_lpr64 = 16;
-        m->v4_unr64 = 2;
-    }
-    if (avg >= 128) {
-        m->v4_lpr32 = 8;
-        m->v4_unr32 = 2;
-    } else if (avg >= 24) {
-        m->v4_lpr32 = 16;
-        m->v4_unr32 = 1;
-    }
+    m->auto64.kernel = m->auto32.kernel = 1;
+    m->auto64.lanes = m->auto32.lanes = m->lpr;
+    if (avg >= 512) m->auto64 = SpmvChoice{3, 16, 2, {}};
+    if (avg >= 128) m->auto32 = SpmvChoice{3, 8, 2, {}};
+    else if (avg >= 24) m->auto32 = SpmvChoice{3, 16, 1, {}};
     // FCI matrices with enough alpha strings to fill the GPU also get the
     // position-major copy (kernel 4): measured 2.38 ms against 5.15 ms (FP64) and
     // 1.53 ms against 3.45 ms (FP32) at L = 12, 0.52 / 0.90 ms at L = 11; at
     // L = 10 (252 alpha strings) the row gather stays faster
     if (sp->kind == CF_FCI && sp->da >= 400 && sp->db <= kEllMaxDb) {
         make_ell(m.get());
-        m->ell_threads = sp->db > 512 ? 1024 : 512;
+        m->auto64 = SpmvChoice{4, 0, 0, ell_default_shape(sp->db, 8)};
+        m->auto32 = SpmvChoice{4, 0, 0, ell_default_shape(sp->db, 4)};
     }
     *out = m.release();
     API_END
@@ -1000,18 +1065,95 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
     API_BEGIN
     CF_REQUIRE(m && bytes && kernel, "NULL argument");
     const uint64_t sval = value_prec == CF_F32 ? 4 : 8;
-    const bool ell = (m->kernel == 4 || m->kernel == 0) && m->ell_ready;
-    const bool v4 = m->kernel == 3 || (m->kernel == 0 && (value_prec == CF_F32 ? m->v4_lpr32 : m->v4_lpr64));
-    if (ell) {
+    const SpmvChoice ch = resolve_spmv(m, value_prec);
+    if (ch.kernel == 4) {
         const cf_space* sp = m->sp;
-        const int Ea = sp->ta->E, Sb = sp->tb->S, Eb = sp->tb->E;
-        const uint64_t R = (uint64_t)Eb + (uint64_t)(sp->ta->S - 1) * Sb + (uint64_t)(Ea - sp->ta->S);
-        *bytes = m->n * R * sval + 16 * m->n + (uint64_t)(Sb + Eb) * sp->db * 2;
-        *kernel = 4;
+        const int Sb = sp->tb->S, Eb = sp->tb->E;
+        *bytes = m->n * m->eargs.R * sval + 16 * m->n + (uint64_t)(Sb + Eb) * sp->db * 2;
     } else {
         *bytes = m->nnz * (sval + 4) + (m->n + 1) * 8 + 16 * m->n;
-        *kernel = m->kernel == 2 && m->plan ? 2 : v4 ? 3 : 1;
     }
+    *kernel = ch.kernel;
+    API_END
+}
+
+
+int cf_crs_set_spmv_shape(cf_crs* m, int value_prec, int kernel, int p1, int p2) {
+    API_BEGIN
+    CF_REQUIRE(m, "matrix is NULL");
+    CF_REQUIRE(value_prec == CF_F64 || value_prec == CF_F32, "value_prec must be CF_F64 or CF_F32");
+    const SpmvChoice ch = make_choice(kernel, p1, p2, m->sp->db, value_prec);
+    check_shape(m, ch);
+    (value_prec == CF_F32 ? m->auto32 : m->auto64) = ch;
+    m->kernel = 0;
+    API_END
+}
+
+int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, int cap, int* count) {
+    API_BEGIN
+    CF_REQUIRE(m, "matrix is NULL");
+    CF_REQUIRE(value_prec == CF_F64 || value_prec == CF_F32, "value_prec must be CF_F64 or CF_F32");
+    CF_REQUIRE(value_prec == CF_F64 || m->val32.p, "matrix was built without FP32 values");
+    CF_REQUIRE(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
+    cf_ctx* c = m->sp->ham->ctx;
+    c->use();
+    const uint64_t db = m->sp->db;
+    std::vector<SpmvChoice> cand;
+    for (int l : {8, 16, 32, 64}) cand.push_back(make_choice(1, l, 0, db, value_prec));
+    for (int l : {4, 8, 16, 32})
+        for (int u : {1, 2}) cand.push_back(make_choice(3, l, u, db, value_prec));
+    if (m->ell_ready) {
+        std::vector<int> sizes = {256, 512, 768, 1024};
+        for (int v : {1, 2, 4}) {
+            if (db % v) continue;
+            std::vector<int> ts = sizes;
+            const uint64_t fit = (db + 64 * v - 1) / (64 * v) * 64;  // one chunk, fewest idle lanes
+            if (fit <= 1024) ts.push_back((int)fit);
+            std::sort(ts.begin(), ts.end());
+            ts.erase(std::unique(ts.begin(), ts.end()), ts.end());
+            for (int t : ts)
+                if ((db + t - 1) / t <= 16) cand.push_back(make_choice(4, t, v, db, value_prec));
+        }
+        for (int t : {512, 1024}) cand.push_back(make_choice(4, t, 0, db, value_prec));
+    }
+    ensure_staging(c, m->n);
+    CF_HIP(hipMemsetAsync(c->hx.p, 0, m->n * sizeof(double), c->stream));
+    hipEvent_t e0, e1;
+    CF_HIP(hipEventCreate(&e0));
+    CF_HIP(hipEventCreate(&e1));
+    std::vector<double> ms(cand.size(), 0.0);
+    try {
+        for (size_t i = 0; i < cand.size(); ++i) {
+            spmv_with(m, value_prec, cand[i], c->hx.p, c->hy.p);
+            CF_HIP(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps; ++r) spmv_with(m, value_prec, cand[i], c->hx.p, c->hy.p);
+            CF_HIP(hipEventRecord(e1, c->stream));
+            CF_HIP(hipEventSynchronize(e1));
+            float t = 0.f;
+            CF_HIP(hipEventElapsedTime(&t, e0, e1));
+            ms[i] = (double)t / reps;
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    CF_HIP(hipEventDestroy(e0));
+    CF_HIP(hipEventDestroy(e1));
+    size_t best = 0;
+    for (size_t i = 1; i < cand.size(); ++i)
+        if (ms[i] < ms[best]) best = i;
+    (value_prec == CF_F32 ? m->auto32 : m->auto64) = cand[best];
+    m->kernel = 0;
+    if (table)
+        for (size_t i = 0; i < cand.size() && (int)i < cap; ++i) {
+            const SpmvChoice& ch = cand[i];
+            table[i].kernel = ch.kernel;
+            table[i].p1 = ch.kernel == 4 ? ch.ell.threads : ch.lanes;
+            table[i].p2 = ch.kernel == 4 ? ch.ell.vec : ch.unroll;
+            table[i].ms = ms[i];
+        }
+    if (count) *count = (int)cand.size();
     API_END
 }
 
@@ -1035,10 +1177,17 @@ int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
     CF_REQUIRE(kind >= 0 && kind <= 4, "kind must be 0, 1, 2, 3 or 4");
     m->sp->ham->ctx->use();
     if (kind == 4) {
-        const int nt = tile_rows > 0 ? tile_rows : 512;
-        CF_REQUIRE(nt == 256 || nt == 512 || nt == 1024, "ELL workgroup size must be 256, 512 or 1024");
+        if (tile_rows > 0)
+            CF_REQUIRE(tile_rows % 64 == 0 && tile_rows <= 1024 && (m->sp->db + tile_rows - 1) / tile_rows <= 16,
+                       "ELL workgroup size must be a multiple of 64 <= 1024 covering d_beta in 16 strides");
         if (!m->ell_ready) make_ell(m);
-        m->ell_threads = nt;
+        for (SpmvChoice* a : {&m->auto64, &m->auto32}) {
+            if (a->kernel != 4) a->ell = ell_default_shape(m->sp->db, a == &m->auto32 ? 4 : 8);
+            if (tile_rows > 0) {
+                a->ell.threads = tile_rows;
+                if (a->ell.vec == 0) a->ell.vec = 1;
+            }
+        }
         CF_HIP(hipStreamSynchronize(m->sp->ham->ctx->stream));
     }
     if (kind == 2) {

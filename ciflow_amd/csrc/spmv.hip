@@ -160,10 +160,18 @@ static void launch_v4_t(uint64_t n, const int64_t* rowptr, const uint32_t* col, 
 template <typename VT>
 static void launch_v4(uint64_t n, const int64_t* rowptr, const uint32_t* col, const VT* val, const double* x,
                       double* y, int lpr, int unroll, hipStream_t stream) {
-    if (lpr == 16 && unroll == 2) launch_v4_t<VT, 16, 2>(n, rowptr, col, val, x, y, stream);
-    else if (lpr == 8 && unroll == 2) launch_v4_t<VT, 8, 2>(n, rowptr, col, val, x, y, stream);
-    else if (lpr == 16 && unroll == 1) launch_v4_t<VT, 16, 1>(n, rowptr, col, val, x, y, stream);
-    else throw Error(ERR_ARG, "vectorised spmv: (lanes, unroll) must be (16,2), (8,2) or (16,1)");
+#define CF_V4(L, U) \
+    if (lpr == L && unroll == U) return launch_v4_t<VT, L, U>(n, rowptr, col, val, x, y, stream)
+    CF_V4(16, 2);
+    CF_V4(8, 2);
+    CF_V4(16, 1);
+    CF_V4(4, 2);
+    CF_V4(4, 1);
+    CF_V4(8, 1);
+    CF_V4(32, 1);
+    CF_V4(32, 2);
+#undef CF_V4
+    throw Error(ERR_ARG, "vectorised spmv: lanes must be 4, 8, 16 or 32 and unroll 1 or 2");
 }
 
 void launch_spmv_v4_f64(uint64_t n, const int64_t* rowptr, const uint32_t* col, const double* val, const double* x,

@@ -138,11 +138,33 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * row gather, 2 = FCI alpha-block staged (FCI matrices only), 3 = row gather
  * with vectorised column/value streams, 4 = FCI position-major complete rows
  * with LDS-staged x blocks (FCI, d_beta <= 8192; builds an extra value copy;
- * tile_rows = workgroup size 256/512/1024). */
+ * tile_rows > 0 = workgroup size, a multiple of 64 <= 1024, for both value
+ * precisions). */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
 /* Kernel the next product with value_prec would run (1-4) and its algorithmic
  * HBM bytes (matrix stream + x + y), the roofline numerator. */
 int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* kernel);
+/* Launch shape of one SpMV kernel for one value precision (no reference
+ * counterpart; the reference's mvprod has no launch shape).  kernel 1: p1 =
+ * lanes per row (8, 16, 32, 64); kernel 3: p1 = lanes per row (4, 8, 16,
+ * 32), p2 = unroll (1, 2); kernel 4: p1 = workgroup size (multiple of 64,
+ * <= 1024, 0 = default), p2 = beta strings per thread (1, 2 or 4, dividing
+ * d_beta; 0 = the single-buffered kernel with p1 in 256/512/1024).  The shape
+ * becomes the automatic choice for value_prec and the matrix returns to
+ * automatic selection (kernel 0). */
+int cf_crs_set_spmv_shape(cf_crs* m, int value_prec, int kernel, int p1, int p2);
+/* One timed SpMV configuration of the autotuner. */
+typedef struct {
+    int kernel, p1, p2; /* as cf_crs_set_spmv_shape */
+    double ms;          /* mean kernel time over the timed launches */
+} cf_tune_entry;
+/* Autotuner (no reference counterpart): times every launch shape the matrix
+ * supports for value_prec (row gather lanes per row; vectorised lanes x
+ * unroll; ELL workgroup size x strings per thread when the ELL copy exists)
+ * with HIP events on the context stream, one warm-up and reps timed launches
+ * each, installs the fastest as the automatic choice for value_prec and
+ * returns the table (at most cap entries written, *count = number timed). */
+int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, int cap, int* count);
 
 /* ---- ground state: CIMethod::solve -> SparseMatrix_CRS::arpackDiagonalize
  *      (CIMethod.cpp:241-268, SparseMatrix_CRS.cpp:360-481), ARPACK's

@@ -386,6 +386,42 @@ def test_ell_spmv_matches_scalar_gather(case, nt):
         assert abs(cim.get_ci_energy() + econst - golden()["beh2"]["fci_energy"]) < 1e-10
 
 
+@pytest.mark.parametrize("L,na,nb", [(10, 5, 5), (7, 3, 2)])
+def test_autotune_shapes_agree_and_fastest_is_installed(L, na, nb):
+    """Every launch shape the autotuner times gives the scalar gather's product
+    (within rounding), the fastest becomes the automatic choice, and shapes the
+    kernels cannot run are rejected."""
+    say("autotune", L, na, nb)
+    h, V = CA.gen_integrals(L, 1)
+    cim = CA.FCI(CA.Hamiltonian(h, V, na, nb), value_prec=CA.CF_F32)
+    m = cim.get_mat()
+    m.set_spmv_kernel(4, 0)  # builds the position-major copy
+    x = np.random.default_rng(3).standard_normal(cim.get_dim())
+    for prec in (CA.CF_F64, CA.CF_F32):
+        m.set_spmv_kernel(1)
+        y0 = m.mvprod(x, prec)
+        tab = m.autotune(prec, reps=2)
+        assert {k for k, _, _, _ in tab} == {1, 3, 4}
+        best = min(tab, key=lambda t: t[3])
+        assert m.spmv_bytes(prec)[0] == best[0]
+        yb = m.mvprod(x, prec)
+        assert np.max(np.abs(yb - y0)) <= 1e-13 * np.max(np.abs(y0))
+        for k, p1, p2, ms in tab:
+            assert ms > 0
+            m.set_spmv_shape(prec, k, p1, p2)
+            y = m.mvprod(x, prec)
+            assert np.max(np.abs(y - y0)) <= 1e-13 * np.max(np.abs(y0)), (prec, k, p1, p2)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 4, 100, 1)  # not a multiple of 64
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 3, 64, 1)  # lanes per row
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 4, 256, 3)  # strings per thread
+    if L == 7:  # d_beta = 21 is odd
+        with pytest.raises(CA.CIFlowError):
+            m.set_spmv_shape(CA.CF_F64, 4, 256, 2)
+
+
 def test_config2_energy_matches_oracle():
     """FCI L=12 6a6b seed 1: GPU solve vs the oracle's full CPU solve (golden)."""
     g = golden()["config2"]
