@@ -173,6 +173,7 @@ def lib():
         "cf_sigma_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
         "cf_sigma_destroy": (C.c_int, [vp]),
         "cf_sigma_set_threads": (C.c_int, [vp, C.c_int]),
+        "cf_sigma_set_kernel": (C.c_int, [vp, C.c_int]),
         "cf_sigma_apply_dev": (C.c_int, [vp, C.c_int, vp, vp]),
         "cf_sigma_apply": (C.c_int, [vp, C.c_int, dp, dp]),
         "cf_solve_direct": (C.c_int, [vp, C.POINTER(SolveOpts), dp, C.POINTER(SolveInfo)]),
@@ -608,6 +609,10 @@ class FCI_Direct(CIMethod):
     def set_threads(self, threads: int):
         """Workgroup size of the alpha-beta sigma kernel (128, 256, 512, 768, 1024)."""
         _check(lib().cf_sigma_set_threads(self.sig.ptr, threads))
+
+    def set_kernel(self, kernel: int):
+        """Alpha-beta sigma kernel: 1 staged slices, 2 prefetched slice tables."""
+        _check(lib().cf_sigma_set_kernel(self.sig.ptr, kernel))
 
     def build_ci(self):
         raise CIFlowError(CF_ERR_STATE, "FCI_Direct stores no matrix")

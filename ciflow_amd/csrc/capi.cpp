@@ -1292,7 +1292,9 @@ struct cf_sigma {
     DBuf<float> x32, y32;
     rocblas_handle blas = nullptr;  // the context's handle
     SigmaArgs args;
-    int threads = 1024;  // workgroup size of the alpha-beta kernel
+    DBuf<double> vpair64, vself64;  // slice tables of the prefetching alpha-beta kernel
+    DBuf<float> vpair32, vself32;
+    SigmaShape shape;
 };
 
 namespace {
@@ -1320,14 +1322,14 @@ void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
         CF_BLAS(rocblas_sgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb32, m,
                               S->x32.p, m, &one, S->y32.p, m));
         launch_to_f64(sp->n, S->y32.p, y, c->stream);
-        launch_sigma_ab<float>(S->args, S->threads, S->x32.p, y, c->stream);
+        launch_sigma_ab2<float>(S->args, S->shape, S->x32.p, y, c->stream);
     } else {
         const double one = 1.0, zero = 0.0;
         CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, x, m,
                               S->Ha.p, nn, &zero, y, m));
         CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb, m, x,
                               m, &one, y, m));
-        launch_sigma_ab<double>(S->args, S->threads, x, y, c->stream);
+        launch_sigma_ab2<double>(S->args, S->shape, x, y, c->stream);
     }
 }
 }  // namespace
@@ -1392,7 +1394,7 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     // measured on MI355X: 512 beta strings per workgroup up to L = 14 (d_beta <= 3432),
     // 768 above (L = 16: 771 ms against 1458 ms at 1024); small beta spaces use the
     // smallest workgroup that holds them in one chunk
-    S->threads = sp->db <= 128 ? 128 : sp->db <= 256 ? 256 : sp->db <= 4096 ? 512 : 768;
+    S->shape.threads = sp->db <= 128 ? 128 : sp->db <= 256 ? 256 : sp->db <= 4096 ? 512 : 768;
     SigmaArgs& a = S->args;
     a.da = sp->da;
     a.db = sp->db;
@@ -1406,6 +1408,19 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     a.subb = sp->tb->sub.p;
     a.V = sp->ham->dV.p;
     CF_REQUIRE(sigma_ab_lds_bytes(a, false) <= 160 * 1024, "beta string space too large for the LDS row");
+    const uint64_t L2 = (uint64_t)a.L * a.L;
+    S->vpair64.alloc(L2 * L2);
+    S->vself64.alloc(sp->da * L2);
+    launch_sigma_tables<double>(a, S->vpair64.p, S->vself64.p, c->stream);
+    a.vpair64 = S->vpair64.p;
+    a.vself64 = S->vself64.p;
+    if (S->f32) {
+        S->vpair32.alloc(L2 * L2);
+        S->vself32.alloc(sp->da * L2);
+        launch_sigma_tables<float>(a, S->vpair32.p, S->vself32.p, c->stream);
+        a.vpair32 = S->vpair32.p;
+        a.vself32 = S->vself32.p;
+    }
     CF_HIP(hipStreamSynchronize(c->stream));
     *out = S.release();
     API_END
@@ -1416,7 +1431,15 @@ int cf_sigma_set_threads(cf_sigma* s, int threads) {
     CF_REQUIRE(s, "NULL argument");
     CF_REQUIRE(threads == 128 || threads == 256 || threads == 512 || threads == 768 || threads == 1024,
                "threads must be 128, 256, 512, 768 or 1024");
-    s->threads = threads;
+    s->shape.threads = threads;
+    API_END
+}
+
+int cf_sigma_set_kernel(cf_sigma* s, int kernel) {
+    API_BEGIN
+    CF_REQUIRE(s, "NULL argument");
+    CF_REQUIRE(kernel == 1 || kernel == 2, "kernel must be 1 or 2");
+    s->shape.kernel = kernel;
     API_END
 }
 

@@ -14,6 +14,22 @@ struct SigmaArgs {
     const Exc* suba = nullptr;  // [da][Sa] self + singles
     const Exc* subb = nullptr;  // [db][Sb]
     const double* V = nullptr;  // L^4
+    // slice tables of the prefetching kernel, per staged precision:
+    //   vpair[(p*L + r)*L^2 + q*L + s] = V[p][q][r][s]    (alpha single p -> r)
+    //   vself[Ia*L^2 + q*L + s] = sum_{p in Ia} V[p][q][p][s]  (alpha number operators)
+    const double* vpair64 = nullptr;
+    const double* vself64 = nullptr;
+    const float* vpair32 = nullptr;
+    const float* vself32 = nullptr;
+};
+
+// Alpha-beta kernel choice: 1 = stage the V slice and C row per alpha action
+// (gathers V from L^4), 2 = slice tables with the next action's V slice and C
+// row prefetched into registers (double-buffered LDS up to 8 row elements per
+// thread).
+struct SigmaShape {
+    int threads = 512;
+    int kernel = 2;
 };
 
 template <typename T>
@@ -23,6 +39,13 @@ size_t sigma_ab_lds_bytes(const SigmaArgs& a, bool f32);
 // a workgroup owns one alpha string and `threads` beta strings.
 template <typename CT>
 void launch_sigma_ab(const SigmaArgs& a, int threads, const CT* C, double* sig, hipStream_t s);
+// Builds vpair / vself (CT precision) from V and the alpha strings.
+template <typename CT>
+void launch_sigma_tables(const SigmaArgs& a, CT* vpair, CT* vself, hipStream_t s);
+// Kernel 2 (falls back to kernel 1 where the prefetch registers cannot hold a row).
+template <typename CT>
+void launch_sigma_ab2(const SigmaArgs& a, SigmaShape sh, const CT* C, double* sig, hipStream_t s);
+size_t sigma_ab2_lds_bytes(const SigmaArgs& a, bool f32, bool dbuf);
 void launch_to_f32(uint64_t n, const double* x, float* y, hipStream_t s);
 void launch_to_f64(uint64_t n, const float* x, double* y, hipStream_t s);
 void launch_row_dots(uint64_t da, uint64_t db, int S, const Exc* sub, const double* C, double* out, hipStream_t s);

@@ -212,6 +212,256 @@ void launch_sigma_ab(const SigmaArgs& a, int threads, const CT* C, double* sig, 
 template void launch_sigma_ab<double>(const SigmaArgs&, int, const double*, double*, hipStream_t);
 template void launch_sigma_ab<float>(const SigmaArgs&, int, const float*, double*, hipStream_t);
 
+// ---------------------------------------------------------------------------
+// Kernel 2.  The V slice of an alpha action is a contiguous L^2 block of a
+// table built once per Hamiltonian (vpair for a single p -> r, vself for the
+// alpha number operators of Ia), so the next action's slice and C row are
+// loaded into registers while the current action is processed from LDS; with
+// a double-buffered LDS one barrier per action remains and no thread waits on
+// a global load at a barrier.  Slices are held in the staged precision CT.
+// ---------------------------------------------------------------------------
+template <typename CT>
+__global__ void k_sigma_vpair(int L, const double* __restrict__ V, CT* __restrict__ vpair) {
+    const uint64_t L2 = (uint64_t)L * L, t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L2 * L2) return;
+    const uint64_t pr = t / L2, qs = t - pr * L2;
+    const int p = (int)(pr / L), r = (int)(pr % L), q = (int)(qs / L), s = (int)(qs % L);
+    vpair[t] = (CT)V[(((uint64_t)p * L + q) * L + r) * L + s];
+}
+
+template <typename CT>
+__global__ void k_sigma_vself(uint64_t da, int L, const str_t* __restrict__ sa, const double* __restrict__ V,
+                              CT* __restrict__ vself) {
+    const uint64_t L2 = (uint64_t)L * L, t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= da * L2) return;
+    const uint64_t Ia = t / L2, qs = t - Ia * L2;
+    const int q = (int)(qs / L), s = (int)(qs % L);
+    double v = 0.0;
+    for (str_t u = sa[Ia]; u; u &= u - 1) {
+        const int p = __builtin_ctzll(u);
+        v += V[(((uint64_t)p * L + q) * L + p) * L + s];
+    }
+    vself[t] = (CT)v;
+}
+
+template <typename CT>
+void launch_sigma_tables(const SigmaArgs& a, CT* vpair, CT* vself, hipStream_t s) {
+    const uint64_t L2 = (uint64_t)a.L * a.L;
+    k_sigma_vpair<CT><<<(unsigned)((L2 * L2 + 255) / 256), 256, 0, s>>>(a.L, a.V, vpair);
+    CF_HIP(hipGetLastError());
+    const uint64_t tot = a.da * L2;
+    if (tot) {
+        k_sigma_vself<CT><<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(a.da, a.L, a.sa, a.V, vself);
+        CF_HIP(hipGetLastError());
+    }
+}
+template void launch_sigma_tables<double>(const SigmaArgs&, double*, double*, hipStream_t);
+template void launch_sigma_tables<float>(const SigmaArgs&, float*, float*, hipStream_t);
+
+template <typename CT>
+__device__ __forceinline__ const CT* vpair_tab(const SigmaArgs& a);
+template <>
+__device__ __forceinline__ const double* vpair_tab<double>(const SigmaArgs& a) { return a.vpair64; }
+template <>
+__device__ __forceinline__ const float* vpair_tab<float>(const SigmaArgs& a) { return a.vpair32; }
+template <typename CT>
+__device__ __forceinline__ const CT* vself_tab(const SigmaArgs& a);
+template <>
+__device__ __forceinline__ const double* vself_tab<double>(const SigmaArgs& a) { return a.vself64; }
+template <>
+__device__ __forceinline__ const float* vself_tab<float>(const SigmaArgs& a) { return a.vself32; }
+
+// PF * NT >= d_beta, PV * NT >= L^2 and NS >= the beta list length (checked by
+// the launcher); NS sizes the register copy of the thread's beta list.
+template <typename CT, int PF, int PV, int NT, bool DB, int NS>
+__global__ void __launch_bounds__(NT) k_sigma_ab2(SigmaArgs a, const CT* __restrict__ C, double* __restrict__ sig) {
+    extern __shared__ unsigned char smem[];
+    const uint64_t db = a.db;
+    const uint32_t nchunk = a.nchunk;
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (lb >= a.nblocks) return;
+    const uint64_t Ia = lb / nchunk;
+    const uint64_t Ib = (uint64_t)(lb - Ia * nchunk) * NT + threadIdx.x;
+    const bool act = Ib < db;
+    const int L = a.L, L2 = L * L;
+    // buffer b: V slice at base + b*bsz (L^2 entries, a zero at [L^2]), C row after it
+    const uint64_t bsz = (uint64_t)L2 + 2 + db;
+    CT* base = (CT*)smem;
+    if (threadIdx.x == 0) {
+        base[L2] = CT(0);
+        if (DB) base[bsz + L2] = CT(0);
+    }
+    uint32_t sb[NS];
+    const int Sb = a.Sb;
+    str_t sbits = 0;
+    if (act) {
+        sbits = a.sb[Ib];
+        const Exc* lb_ = a.subb + Ib * (uint64_t)Sb;
+#pragma unroll
+        for (int e = 0; e < NS; ++e) {
+            if (e < Sb) {
+                const Exc x = lb_[e];
+                sb[e] = x.type == 1
+                            ? (x.J << 11) | ((uint32_t)x.f << 6) | ((uint32_t)x.t << 1) | (x.sign < 0 ? 1u : 0u)
+                            : ((uint32_t)Ib << 11) | ((uint32_t)L << 6);
+            }
+        }
+    }
+    const Exc* la = a.suba + Ia * (uint64_t)a.Sa;
+    const CT* vpair = vpair_tab<CT>(a);
+    const CT* vself = vself_tab<CT>(a) + Ia * (uint64_t)L2;
+    CT pv[PV], pf[PF];
+    bool neg_next = false;
+    auto fetch = [&](int e) {
+        const Exc x = la[e];
+        neg_next = x.type == 1 && x.sign < 0;
+        const CT* vsrc = x.type == 0 ? vself : vpair + (uint64_t)(x.f * L + x.t) * L2;
+#pragma unroll
+        for (int k = 0; k < PV; ++k) {
+            const int i = k * NT + threadIdx.x;
+            pv[k] = i < L2 ? vsrc[i] : CT(0);
+        }
+        const CT* src = C + (uint64_t)x.J * db;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const uint64_t i = (uint64_t)k * NT + threadIdx.x;
+            pf[k] = i < db ? src[i] : CT(0);
+        }
+    };
+    auto store = [&](int b) {
+        CT* Vs = base + b * bsz;
+        CT* row = Vs + L2 + 2;
+#pragma unroll
+        for (int k = 0; k < PV; ++k) {
+            const int i = k * NT + threadIdx.x;
+            if (i < L2) Vs[i] = pv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const uint64_t i = (uint64_t)k * NT + threadIdx.x;
+            if (i < db) row[i] = pf[k];
+        }
+    };
+    fetch(0);
+    store(0);
+    bool neg = neg_next;
+    __syncthreads();
+    double acc = 0.0;
+    const int Sa = a.Sa;
+    for (int e = 0; e < Sa; ++e) {
+        const int cur = DB ? (e & 1) : 0;
+        const bool more = e + 1 < Sa;
+        if (more) fetch(e + 1);
+        if (act) {
+            const CT* Vs = base + cur * bsz;
+            const CT* row = Vs + L2 + 2;
+            double dsum = 0.0;  // beta number operators: J_b = Ib
+            for (str_t u = sbits; u; u &= u - 1) {
+                const int q = __builtin_ctzll(u);
+                dsum += (double)Vs[q * L + q];
+            }
+            double t = dsum * (double)row[Ib];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) {
+                if (j < Sb) {
+                    const uint32_t c = sb[j];
+                    const int q = (c >> 6) & 31, s = (c >> 1) & 31;
+                    const double v = (double)Vs[q * L + s] * (double)row[c >> 11];
+                    t += (c & 1) ? -v : v;
+                }
+            }
+            acc += neg ? -t : t;
+        }
+        if (more) {
+            if (!DB) __syncthreads();  // the single buffer is still being read
+            store(DB ? cur ^ 1 : 0);
+            neg = neg_next;
+        }
+        __syncthreads();
+    }
+    if (act) sig[Ia * db + Ib] += acc;
+}
+
+size_t sigma_ab2_lds_bytes(const SigmaArgs& a, bool f32, bool dbuf) {
+    return (dbuf ? 2 : 1) * ((size_t)a.L * a.L + 2 + a.db) * (f32 ? sizeof(float) : sizeof(double));
+}
+
+constexpr size_t kLdsMax = 160 * 1024;
+
+// Supported (NT, PF) pairs; the double buffer is used for PF <= 8 (it fits the
+// LDS for every d_beta <= 8 * NT with L <= 31), a single buffer for PF = 17.
+template <typename CT, int NT, int PF, int PV, int NS>
+static void launch_ab2_t(const SigmaArgs& a, uint64_t grid, const CT* C, double* sig, hipStream_t s) {
+    constexpr bool DB = PF <= 8;
+    const size_t lds = sigma_ab2_lds_bytes(a, sizeof(CT) == 4, DB);
+    if (lds > kLdsMax) throw Error(ERR_UNSUPPORTED, "sigma: beta space too large for the LDS buffers");
+    k_sigma_ab2<CT, PF, PV, NT, DB, NS><<<(unsigned)grid, NT, lds, s>>>(a, C, sig);
+}
+
+template <typename CT, int NT, int PF>
+static bool launch_ab2_pv(const SigmaArgs& a, uint64_t grid, const CT* C, double* sig, hipStream_t s) {
+    const int L2 = a.L * a.L;
+    const int ns = a.Sb;
+#define CF_AB2_NS(PV)                                                         \
+    do {                                                                      \
+        if (ns <= 17) launch_ab2_t<CT, NT, PF, PV, 17>(a, grid, C, sig, s);   \
+        else if (ns <= 33) launch_ab2_t<CT, NT, PF, PV, 33>(a, grid, C, sig, s); \
+        else launch_ab2_t<CT, NT, PF, PV, 65>(a, grid, C, sig, s);            \
+    } while (0)
+    if (L2 <= NT) CF_AB2_NS(1);
+    else if (L2 <= 2 * NT) CF_AB2_NS(2);
+    else return false;
+#undef CF_AB2_NS
+    return true;
+}
+
+template <typename CT>
+void launch_sigma_ab2(const SigmaArgs& a0, SigmaShape sh, const CT* C, double* sig, hipStream_t s) {
+    bool ok = false;
+    const bool tables = sizeof(CT) == 4 ? a0.vpair32 != nullptr : a0.vpair64 != nullptr;
+    // FP64 rows of more than 8 elements per thread spill the prefetch registers
+    // (L = 16: 920 ms against 771 ms for kernel 1, measured on MI355X)
+    const bool fits = sizeof(CT) == 4 || (a0.db + sh.threads - 1) / sh.threads <= 8;
+    if (sh.kernel == 2 && tables && fits && a0.Sb <= kMaxSingles + 1) {
+        SigmaArgs a = a0;
+        const int NTr = sh.threads;
+        a.nchunk = (uint32_t)((a.db + NTr - 1) / NTr);
+        const uint64_t nb = a.da * (uint64_t)a.nchunk;
+        if (nb > 0xFFFFFFF0ull) throw Error(ERR_UNSUPPORTED, "too many sigma workgroups");
+        a.nblocks = (uint32_t)nb;
+        const uint64_t grid = (nb + 7) / 8 * 8;
+        const uint64_t pt = (a.db + NTr - 1) / NTr;  // row elements per thread
+        switch (NTr) {
+            case 128:
+                if (pt <= 1) ok = launch_ab2_pv<CT, 128, 1>(a, grid, C, sig, s);
+                break;
+            case 256:
+                if (pt <= 1) ok = launch_ab2_pv<CT, 256, 1>(a, grid, C, sig, s);
+                break;
+            case 512:
+                if (pt <= 1) ok = launch_ab2_pv<CT, 512, 1>(a, grid, C, sig, s);
+                else if (pt <= 2) ok = launch_ab2_pv<CT, 512, 2>(a, grid, C, sig, s);
+                else if (pt <= 8) ok = launch_ab2_pv<CT, 512, 8>(a, grid, C, sig, s);
+                break;
+            case 768:
+                if (pt <= 8) ok = launch_ab2_pv<CT, 768, 8>(a, grid, C, sig, s);
+                else if (pt <= 17) ok = launch_ab2_pv<CT, 768, 17>(a, grid, C, sig, s);
+                break;
+            case 1024:
+                if (pt <= 2) ok = launch_ab2_pv<CT, 1024, 2>(a, grid, C, sig, s);
+                else if (pt <= 8) ok = launch_ab2_pv<CT, 1024, 8>(a, grid, C, sig, s);
+                else if (pt <= 17) ok = launch_ab2_pv<CT, 1024, 17>(a, grid, C, sig, s);
+                break;
+            default: throw Error(ERR_ARG, "sigma workgroup size must be 128, 256, 512, 768 or 1024");
+        }
+        if (ok) CF_HIP(hipGetLastError());
+    }
+    if (!ok) launch_sigma_ab<CT>(a0, sh.threads, C, sig, s);
+}
+template void launch_sigma_ab2<double>(const SigmaArgs&, SigmaShape, const double*, double*, hipStream_t);
+template void launch_sigma_ab2<float>(const SigmaArgs&, SigmaShape, const float*, double*, hipStream_t);
+
 __global__ void k_to_f32(uint64_t n, const double* __restrict__ x, float* __restrict__ y) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) y[i] = (float)x[i];

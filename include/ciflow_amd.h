@@ -218,6 +218,11 @@ int cf_sigma_destroy(cf_sigma* s);
 /* Workgroup size of the alpha-beta kernel: 128, 256, 512, 768 or 1024 beta strings
  * per workgroup (tuning; no reference counterpart). */
 int cf_sigma_set_threads(cf_sigma* s, int threads);
+/* Alpha-beta kernel (tuning; no reference counterpart): 1 = stage each alpha
+ * action's V slice and C row, 2 (default) = per-Hamiltonian slice tables with
+ * the next action's slice and row prefetched into registers (shapes it does
+ * not cover run kernel 1). */
+int cf_sigma_set_kernel(cf_sigma* s, int kernel);
 /* y = H x, device pointers on the context stream; value_prec CF_F32 uses
  * FP32 operands with FP64 accumulation of the alpha-beta part. */
 int cf_sigma_apply_dev(cf_sigma* s, int value_prec, const double* dx, double* dy);
