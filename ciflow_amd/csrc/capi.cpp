@@ -373,9 +373,11 @@ struct DeviceBackend : KrylovBackend {
     cf_ctx* c;
     uint64_t n;
     int M;
-    DBuf<double> Vb, w, partial, cvec, c2, Yd, scal, dalpha, dbeta;
+    DBuf<double> Vb, w, partial, cvec, c2, Yd, scal, dalpha, dbeta, pnorm;
     bool first_ = true;  // next orthogonalisation opens a cycle
     int nb;
+    bool fused;  // step kernels reduce the partials themselves (n <= kFusedMaxN)
+    int G;       // workgroups of the wide product in the fused step
     std::function<void(const double*, double*)> op;
     double* hbuf;  // the context's pinned read-back buffer
 
@@ -384,7 +386,10 @@ struct DeviceBackend : KrylovBackend {
         Vb.alloc((size_t)(M + 1) * n);
         w.alloc(n);
         nb = multi_dot_blocks(n);
-        partial.alloc((size_t)64 * nb);
+        fused = n <= kFusedMaxN;
+        G = fused_dot_blocks(n);
+        partial.alloc((size_t)64 * std::max(nb, G));
+        pnorm.alloc(256);
 
         cvec.alloc(64);
         c2.alloc(64);
@@ -413,6 +418,22 @@ struct DeviceBackend : KrylovBackend {
     // alpha_j v_j + beta_{j-1} v_{j-1} first and then take one full pass, which
     // halves the basis traffic.  alpha_j = (v_j . w) + correction of the pass.
     void orth_deferred(int j) override {
+        if (fused && j + 1 <= 64) {
+            // five launches: product, three-term update (or first pass), product,
+            // update + |w|^2 parts, normalisation; no separate reductions
+            if (first_ || j == 0) {
+                launch_multi_dot_wide(n, j + 1, Vb.p, w.p, partial.p, G, c->stream);
+                launch_multi_axpy_r(n, j + 1, G, partial.p, Vb.p, w.p, cvec.p, pnorm.p, c->stream);
+            } else {
+                launch_multi_dot_wide(n, 1, v(j), w.p, partial.p, G, c->stream);
+                launch_local3_r(n, G, partial.p, v(j), dbeta.p + (j - 1), v(j - 1), w.p, cvec.p + j, c->stream);
+            }
+            first_ = false;
+            launch_multi_dot_wide(n, j + 1, Vb.p, w.p, partial.p, G, c->stream);
+            launch_multi_axpy_r(n, j + 1, G, partial.p, Vb.p, w.p, c2.p, pnorm.p, c->stream);
+            launch_finish_step_r(n, j, cvec.p, c2.p, pnorm.p, w.p, v(j + 1), dalpha.p, dbeta.p, c->stream);
+            return;
+        }
         if (first_ || j == 0) {
             dots(j + 1, Vb.p, w.p, cvec.p);
             launch_multi_axpy(n, j + 1, Vb.p, cvec.p, w.p, c->stream);

@@ -119,6 +119,26 @@ void launch_local3(uint64_t n, const double* a, const double* v1, const double* 
 void launch_finish_step(uint64_t n, int j, const double* c1, const double* c2, const double* wnorm2,
                         const double* w, double* vnext, double* alpha, double* beta, hipStream_t stream);
 void launch_scale_copy(uint64_t n, const double* src, double s, double* dst, hipStream_t stream);
+
+// Step kernels that reduce the partials of the preceding product themselves
+// (every workgroup sums them in the same fixed order), so a Lanczos step needs
+// no separate reduction launches.  For n <= kFusedMaxN: the product runs with
+// fused_dot_blocks(n) workgroups of 1024 threads (a power of two <= 64).
+constexpr uint64_t kFusedMaxN = 262144;
+int fused_dot_blocks(uint64_t n);
+int fused_axpy_blocks(uint64_t n);
+void launch_multi_dot_wide(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
+                           hipStream_t stream);
+// w -= a v1 + b v0, a = sum of the nblocks partials (written to a_out), b = *b_in
+void launch_local3_r(uint64_t n, int nblocks, const double* partial, const double* v1, const double* b_in,
+                     const double* v0, double* w, double* a_out, hipStream_t stream);
+// w -= Vb[:, 0:k] c, c_i = sum of partial[i*nblocks + 0:nblocks] (written to c_out);
+// pnorm[block] = that workgroup's part of |w|^2 after the update (fused_axpy_blocks(n) workgroups)
+void launch_multi_axpy_r(uint64_t n, int k, int nblocks, const double* partial, const double* Vb, double* w,
+                         double* c_out, double* pnorm, hipStream_t stream);
+// finish_step with |w|^2 = the sum of the fused_axpy_blocks(n) entries of pnorm
+void launch_finish_step_r(uint64_t n, int j, const double* c1, const double* c2, const double* pnorm,
+                          const double* w, double* vnext, double* alpha, double* beta, hipStream_t stream);
 void launch_rotate_basis(uint64_t n, int m, int k, double* Vb, const double* Y, hipStream_t stream);
 void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream);
 void launch_copy(uint64_t n, const double* src, double* dst, hipStream_t stream);

@@ -157,6 +157,165 @@ void launch_finish_step(uint64_t n, int j, const double* c1, const double* c2, c
     CF_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Step kernels with the reduction of the preceding product folded in.
+// ---------------------------------------------------------------------------
+int fused_dot_blocks(uint64_t n) {
+    uint64_t want = (n + 4095) / 4096;  // <= 4 elements per thread
+    int nb = 1;
+    while ((uint64_t)nb < want && nb < 64) nb <<= 1;
+    return nb;
+}
+
+int fused_axpy_blocks(uint64_t n) {
+    uint64_t nb = (n + 255) / 256;
+    if (nb > 256) nb = 256;
+    if (nb < 1) nb = 1;
+    return (int)nb;
+}
+
+__global__ void __launch_bounds__(1024) k_multi_dot_wide(uint64_t n, int k, const double* __restrict__ Vb,
+                                                         const double* __restrict__ w,
+                                                         double* __restrict__ partial) {
+    __shared__ double red[kDotChunk][16];
+    const int i0 = blockIdx.y * kDotChunk;
+    double acc[kDotChunk];
+#pragma unroll
+    for (int j = 0; j < kDotChunk; ++j) acc[j] = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * 1024;
+    for (uint64_t q = (uint64_t)blockIdx.x * 1024 + threadIdx.x; q < n; q += stride) {
+        const double wq = w[q];
+#pragma unroll
+        for (int j = 0; j < kDotChunk; ++j)
+            if (i0 + j < k) acc[j] += Vb[(uint64_t)(i0 + j) * n + q] * wq;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < kDotChunk; ++j) {
+        double s = wave_sum(acc[j]);
+        if (lane == 0) red[j][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < kDotChunk && i0 + (int)threadIdx.x < k) {
+        double s = 0.0;
+        for (int q = 0; q < 16; ++q) s += red[threadIdx.x][q];
+        partial[(uint64_t)(i0 + threadIdx.x) * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+void launch_multi_dot_wide(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
+                           hipStream_t stream) {
+    if (k <= 0) return;
+    if (nblocks < 1 || nblocks > 64 || (nblocks & (nblocks - 1)))
+        throw Error(ERR_ARG, "multi_dot_wide: nblocks must be a power of two <= 64");
+    dim3 grid((unsigned)nblocks, (unsigned)((k + kDotChunk - 1) / kDotChunk));
+    k_multi_dot_wide<<<grid, 1024, 0, stream>>>(n, k, Vb, w, partial);
+    CF_HIP(hipGetLastError());
+}
+
+// cs[i] = sum_b partial[i*G + b] for i < k (G a power of two <= 64, k <= 64),
+// the same arithmetic in every workgroup of 256 threads; all loads are issued
+// before the first shuffle.
+__device__ __forceinline__ void reduce_coeffs(int k, int G, const double* __restrict__ partial, double* cs) {
+    const int per_round = 256 / G;
+    const int gi = threadIdx.x / G, b = threadIdx.x % G;
+    double v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = r * per_round + gi;
+        v[r] = (i < k) ? partial[i * G + b] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        if (r * per_round >= k) break;
+        double x = v[r];
+        for (int off = G >> 1; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        const int i = r * per_round + gi;
+        if (b == 0 && i < k) cs[i] = x;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_local3_r(uint64_t n, int G, const double* __restrict__ partial,
+                                                  const double* __restrict__ v1, const double* __restrict__ b_in,
+                                                  const double* __restrict__ v0, double* __restrict__ w,
+                                                  double* __restrict__ a_out) {
+    __shared__ double cs[1];
+    reduce_coeffs(1, G, partial, cs);
+    __syncthreads();
+    const double ca = cs[0], cb = *b_in;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a_out = ca;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+        w[q] -= ca * v1[q] + cb * v0[q];
+}
+
+void launch_local3_r(uint64_t n, int nblocks, const double* partial, const double* v1, const double* b_in,
+                     const double* v0, double* w, double* a_out, hipStream_t stream) {
+    if (!n) return;
+    k_local3_r<<<(unsigned)fused_axpy_blocks(n), 256, 0, stream>>>(n, nblocks, partial, v1, b_in, v0, w, a_out);
+    CF_HIP(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256) k_multi_axpy_r(uint64_t n, int k, int G, const double* __restrict__ partial,
+                                                      const double* __restrict__ Vb, double* __restrict__ w,
+                                                      double* __restrict__ c_out, double* __restrict__ pnorm) {
+    __shared__ double cs[64];
+    __shared__ double red[4];
+    reduce_coeffs(k, G, partial, cs);
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x < k) c_out[threadIdx.x] = cs[threadIdx.x];
+    double nrm = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        double s = 0.0;
+        for (int i = 0; i < k; ++i) s += cs[i] * Vb[(uint64_t)i * n + q];
+        const double wn = w[q] - s;
+        w[q] = wn;
+        nrm += wn * wn;
+    }
+    nrm = wave_sum(nrm);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) pnorm[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void launch_multi_axpy_r(uint64_t n, int k, int nblocks, const double* partial, const double* Vb, double* w,
+                         double* c_out, double* pnorm, hipStream_t stream) {
+    if (!n || k <= 0) return;
+    if (k > 64 || nblocks < 1 || nblocks > 64 || (nblocks & (nblocks - 1)) || k * nblocks > 16 * 256)
+        throw Error(ERR_ARG, "multi_axpy_r: k <= 64 and a power-of-two nblocks <= 64 with k*nblocks <= 4096");
+    k_multi_axpy_r<<<(unsigned)fused_axpy_blocks(n), 256, 0, stream>>>(n, k, nblocks, partial, Vb, w, c_out, pnorm);
+    CF_HIP(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256) k_finish_step_r(uint64_t n, int j, int np, const double* __restrict__ c1,
+                                                       const double* __restrict__ c2,
+                                                       const double* __restrict__ pnorm,
+                                                       const double* __restrict__ w, double* __restrict__ vnext,
+                                                       double* __restrict__ alpha, double* __restrict__ beta) {
+    __shared__ double red[4];
+    double x = (int)threadIdx.x < np ? pnorm[threadIdx.x] : 0.0;
+    x = wave_sum(x);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    const double w2 = (red[0] + red[1]) + (red[2] + red[3]);
+    const double b = sqrt(fmax(0.0, w2));
+    const double s = b > 0.0 ? 1.0 / b : 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) vnext[q] = w[q] * s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        alpha[j] = c1[j] + c2[j];
+        beta[j] = b;
+    }
+}
+
+void launch_finish_step_r(uint64_t n, int j, const double* c1, const double* c2, const double* pnorm,
+                          const double* w, double* vnext, double* alpha, double* beta, hipStream_t stream) {
+    const int np = fused_axpy_blocks(n);
+    k_finish_step_r<<<(unsigned)np, 256, 0, stream>>>(n, j, np, c1, c2, pnorm, w, vnext, alpha, beta);
+    CF_HIP(hipGetLastError());
+}
+
 // In place: Vb[:, 0:k] = Vb[:, 0:m] * Y[:, 0:k]   (Y is m x m row-major).
 // Each thread owns element q of every vector, so reading all m inputs before
 // writing the k outputs is race-free.
