@@ -39,7 +39,8 @@ METRIC = "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of pe
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv"], default="stream")
+    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "rotate"],
+                   default="stream")
     p.add_argument("--steps", type=int, default=None, help="stream: requests (4096); fci12: solves (4)")
     p.add_argument("--warmup", type=int, default=None, help="stream: warm-up requests (64); fci12: solves (1)")
     p.add_argument("--slots", type=int, default=8)
@@ -501,6 +502,31 @@ def bench_spmv(a, CA):
     return {"metric": METRIC, "workload": "FCI L=12 6a6b stored-CRS SpMV", "n": n, "nnz": nnz, **out}
 
 
+def bench_rotate(a, CA):
+    """Batched FP64-MFMA integral rotation (OrbitalTransform::fillHamCI): 64
+    Hamiltonians at L=20 per call, 8 L^5 + 4 L^3 flop each; kernel time from HIP
+    events on the library stream, 1 warm-up + 5 timed calls."""
+    ctx = CA.Context(0)
+    L, B = 20, 64
+    h1, V1 = CA.gen_integrals(L, 5)
+    h = np.repeat(h1[None], B, axis=0)
+    V = np.repeat(V1[None], B, axis=0)
+    U = np.stack([CA.expm_skew(CA.gen_rotation(L, 100 + b, 0.05)) for b in range(B)])
+    CA.rotate_integrals(h, V, U, ctx)
+    ctx.enable_timing(True)
+    ctx.reset_timing()
+    for _ in range(5):
+        CA.rotate_integrals(h, V, U, ctx)
+    nl, ms = ctx.kernel_time("rotate")
+    ctx.enable_timing(False)
+    flop = B * (8.0 * L ** 5 + 4.0 * L ** 3)
+    tf = flop / (ms / nl * 1e-3) / 1e12
+    log(f"rotate L={L} batch {B}: {ms / nl:.3f} ms per call, {tf:.1f} TFLOP/s")
+    return {"metric": METRIC, "workload": f"batched integral rotation, L={L}, {B} Hamiltonians per call",
+            "avg_call_ms": ms / nl, "flop_per_call": flop, "TFLOPs": tf,
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": 78.6, "unit": "TFLOP/s", "frac": tf / 78.6}}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -510,7 +536,7 @@ def main():
     import ciflow_amd as CA
 
     run = {"stream": bench_stream, "fci12": bench_fci12, "doci24": bench_fci12, "direct16": bench_direct16,
-           "spmv": bench_spmv}[a.workload]
+           "spmv": bench_spmv, "rotate": bench_rotate}[a.workload]
     out = run(a, CA)
     print(json.dumps(out))
 

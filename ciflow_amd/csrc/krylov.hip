@@ -316,35 +316,58 @@ void launch_finish_step_r(uint64_t n, int j, const double* c1, const double* c2,
     CF_HIP(hipGetLastError());
 }
 
-// In place: Vb[:, 0:k] = Vb[:, 0:m] * Y[:, 0:k]   (Y is m x m row-major).
-// Each thread owns element q of every vector, so reading all m inputs before
-// writing the k outputs is race-free.
-__global__ void __launch_bounds__(256) k_rotate_basis(uint64_t n, int m, int k, double* __restrict__ Vb,
-                                                      const double* __restrict__ Y) {
+// In place: Vb[:, 0:k] = Vb[:, 0:m] * Y[:, 0:k]   (Y is m x m row-major), on
+// FP64 MFMA as out^T = Y^T V^T per tile of 16 basis elements, so a wavefront's
+// output tile is (16 new vectors) x (16 elements) and its loads and stores are
+// 16 consecutive elements of one vector.  A wavefront reads all m inputs of
+// its 16 elements before it writes, and tiles are disjoint, so the update is
+// in place.  Reads m n 8 B, writes k n 8 B.
+typedef double v4dk __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_rotate_mfma(uint64_t n, int m, int k, double* __restrict__ Vb,
+                                                     const double* __restrict__ Y) {
     __shared__ double ys[64 * 64];
     for (int i = threadIdx.x; i < m * m; i += blockDim.x) ys[i] = Y[i];
     __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
-        double v[64];
+    const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+    const uint64_t q0 = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+    if (q0 >= n) return;
+    const uint64_t q = q0 + r;
+    const bool inq = q < n;
+    v4dk acc[4];
 #pragma unroll
-        for (int i = 0; i < 64; ++i) v[i] = (i < m) ? Vb[(uint64_t)i * n + q] : 0.0;
-        for (int c = 0; c < k; ++c) {
-            double s = 0.0;
+    for (int t = 0; t < 4; ++t) acc[t] = v4dk{0.0, 0.0, 0.0, 0.0};
+    const int ct = (k + 15) / 16;
+    for (int i0 = 0; i0 < m; i0 += 4) {
+        const int i = i0 + kq;
+        const double bv = (i < m && inq) ? Vb[(uint64_t)i * n + q] : 0.0;  // V^T[i][q]
 #pragma unroll
-            for (int i = 0; i < 64; ++i)
-                if (i < m) s += v[i] * ys[i * m + c];
-            Vb[(uint64_t)c * n + q] = s;
+        for (int t = 0; t < 4; ++t) {
+            if (t < ct) {
+                const int c = t * 16 + r;
+                const double av = (i < m && c < k) ? ys[i * m + c] : 0.0;  // Y^T[c][i]
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    // D[c][e]: lane e + 16 (c % 4), element c / 4 -> vector t*16 + 4 v + kq, element q0 + r
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (t < ct) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = t * 16 + 4 * v + kq;
+                if (c < k && inq) Vb[(uint64_t)c * n + q] = acc[t][v];
+            }
         }
     }
 }
 
 void launch_rotate_basis(uint64_t n, int m, int k, double* Vb, const double* Y, hipStream_t stream) {
     if (m > 64 || k > m) throw Error(ERR_ARG, "rotate_basis: m > 64 or k > m");
-    if (!n) return;
-    uint64_t grid = (n + 255) / 256;
-    if (grid > 8192) grid = 8192;
-    k_rotate_basis<<<(unsigned)grid, 256, 0, stream>>>(n, m, k, Vb, Y);
+    if (!n || k <= 0) return;
+    const uint64_t grid = (n + 63) / 64;  // 4 wavefronts x 16 elements
+    if (grid > 0xffffffffull) throw Error(ERR_UNSUPPORTED, "rotate_basis: n too large");
+    k_rotate_mfma<<<(unsigned)grid, 256, 0, stream>>>(n, m, k, Vb, Y);
     CF_HIP(hipGetLastError());
 }
 

@@ -127,6 +127,98 @@ __global__ void __launch_bounds__(256) k_gram(uint64_t n, int P, const double* _
     if (p + 1 < P && q + 1 < P) o[(p + 1) * P + q + 1] = acc11;
 }
 
+// The same split-K Gram product on FP64 MFMA (v_mfma_f64_16x16x4f64), for
+// P = L^2 <= 256.  A workgroup of 16 wavefronts owns the P x P output of its
+// chunk of K rows (or one of `gridDim.y` interleaved parts of it, so that a
+// wavefront holds at most TPW = 8 accumulator tiles): 32 rows of A and B are
+// staged in LDS per step and every wavefront accumulates its 16 x 16 output
+// tiles (tile = 16 (y + gridDim.y t) + wave), so each element of A and B is
+// read from HBM gridDim.y times per product.
+// 2 n P^2 flop per product.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kGramRows = 32;
+
+template <int TPW>
+__global__ void __launch_bounds__(1024) k_gram_mfma(uint64_t n, int P, const double* __restrict__ A,
+                                                    const double* __restrict__ B, uint64_t rows,
+                                                    double* __restrict__ part) {
+    extern __shared__ double gsm[];
+    const int SP = P + 1;
+    double* As = gsm;
+    double* Bs = gsm + kGramRows * SP;
+    const int T16 = (P + 15) / 16, ntiles = T16 * T16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, kq = lane >> 4;
+    v4d acc[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    const uint64_t k0 = (uint64_t)blockIdx.x * rows;
+    uint64_t k1 = k0 + rows;
+    if (k1 > n) k1 = n;
+    for (uint64_t kb = k0; kb < k1; kb += kGramRows) {
+        for (int i = threadIdx.x; i < kGramRows * P; i += 1024) {
+            const int kk = i / P, pp = i - kk * P;
+            const uint64_t K = kb + kk;
+            const bool ok = K < k1;
+            As[kk * SP + pp] = ok ? A[K * P + pp] : 0.0;
+            Bs[kk * SP + pp] = ok ? B[K * P + pp] : 0.0;
+        }
+        __syncthreads();
+        for (int ks = 0; ks < kGramRows; ks += 4) {
+            const double* ar = As + (ks + kq) * SP;
+            const double* br = Bs + (ks + kq) * SP;
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                const int tile = 16 * ((int)blockIdx.y + (int)gridDim.y * t) + wave;
+                if (tile < ntiles) {
+                    const int ti = tile / T16, tj = tile - ti * T16;
+                    const int p = ti * 16 + r, q = tj * 16 + r;
+                    const double av = p < P ? ar[p] : 0.0;  // A^T[p][K]
+                    const double bv = q < P ? br[q] : 0.0;  // B[K][q]
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double* o = part + (uint64_t)blockIdx.x * P * P;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int tile = 16 * ((int)blockIdx.y + (int)gridDim.y * t) + wave;
+        if (tile < ntiles) {
+            const int ti = tile / T16, tj = tile - ti * T16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {  // D[i][j]: lane j + 16 (i % 4), element i / 4
+                const int p = ti * 16 + 4 * v + kq, q = tj * 16 + r;
+                if (p < P && q < P) o[(uint64_t)p * P + q] = acc[t][v];
+            }
+        }
+    }
+}
+
+// part[chunk] = A^T B over the chunk's rows; MFMA for P <= 256, else k_gram.
+static void launch_gram(uint64_t n, int P, const double* A, const double* B, uint64_t rows, uint64_t nchunk,
+                        double* part, hipStream_t stream) {
+    if (P <= 256) {
+        const int T16 = (P + 15) / 16, per = (T16 * T16 + 15) / 16;
+        const int split = (per + 7) / 8;  // parts of the output per chunk
+        const int tpw = (per + split - 1) / split;
+        const size_t lds = 2 * (size_t)kGramRows * (P + 1) * sizeof(double);
+        const dim3 grid((unsigned)nchunk, (unsigned)split);
+#define CF_GRAM(T) k_gram_mfma<T><<<grid, 1024, lds, stream>>>(n, P, A, B, rows, part)
+        if (tpw <= 1) CF_GRAM(1);
+        else if (tpw <= 2) CF_GRAM(2);
+        else if (tpw <= 4) CF_GRAM(4);
+        else CF_GRAM(8);
+#undef CF_GRAM
+    } else {
+        const int nt = (P + kGT - 1) / kGT;
+        dim3 grid((unsigned)nchunk, (unsigned)(nt * nt));
+        k_gram<<<grid, 256, 0, stream>>>(n, P, A, B, rows, part);
+    }
+    CF_HIP(hipGetLastError());
+}
+
 // out[i] = sum_chunk part[chunk][i]
 __global__ void k_sum_chunks(int nchunk, uint64_t len, const double* __restrict__ part, double* __restrict__ out) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -238,14 +330,11 @@ void rdm_dense(hipStream_t stream, int kind, int L, int nup, int ndown, uint64_t
         CF_HIP(hipStreamSynchronize(stream));
     }
     if (!Gamma) return;
-    const int nt = (P + kGT - 1) / kGT;
     const uint64_t PP = (uint64_t)P * P;
     DBuf<double> part(nchunk * PP), G(3 * PP);
     const double* ops[3][2] = {{Xa.p, Xa.p}, {Xa.p, Xb.p}, {Xb.p, Xb.p}};
     for (int b = 0; b < 3; ++b) {
-        dim3 grid((unsigned)nchunk, (unsigned)(nt * nt));
-        k_gram<<<grid, 256, 0, stream>>>(n, P, ops[b][0], ops[b][1], rows, part.p);
-        CF_HIP(hipGetLastError());
+        launch_gram(n, P, ops[b][0], ops[b][1], rows, nchunk, part.p, stream);
         k_sum_chunks<<<(unsigned)((PP + 255) / 256), 256, 0, stream>>>((int)nchunk, PP, part.p, G.p + b * PP);
         CF_HIP(hipGetLastError());
     }
