@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(_HERE, "libciflow_amd.so")
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "ciflow_amd.h")
 
 CF_OK, CF_ERR_ARG, CF_ERR_HIP, CF_ERR_NOMEM, CF_ERR_NOCONV, CF_ERR_UNSUPPORTED, CF_ERR_STATE = range(7)
-CF_FCI, CF_DOCI = 0, 1
+CF_FCI, CF_DOCI, CF_DETS = 0, 1, 2
 CF_F64, CF_F32 = 0, 1
 
 _lib = None
@@ -140,6 +140,8 @@ def lib():
         "cf_ham_create": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, C.POINTER(vp)]),
         "cf_ham_destroy": (C.c_int, [vp]),
         "cf_space_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+        "cf_space_create_dets": (C.c_int, [vp, C.c_uint64, u64p, u64p, C.POINTER(vp)]),
+        "cf_read_determinants": (C.c_int, [C.c_char_p, C.c_uint64, u64p, u64p, C.POINTER(C.c_uint64)]),
         "cf_space_destroy": (C.c_int, [vp]),
         "cf_space_dims": (C.c_int, [vp, u64p, u64p, u64p]),
         "cf_strings_unrank": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, u64p]),
@@ -470,7 +472,7 @@ class CIMethod:
         self.ham = ham
         self.ctx = ham.ctx
         self.space = _Handle("cf_space_destroy", ham.handle)
-        _check(lib().cf_space_create(ham.ptr, self.kind, C.byref(self.space.ptr)))
+        self._create_space()
         n, da, db = C.c_uint64(), C.c_uint64(), C.c_uint64()
         _check(lib().cf_space_dims(self.ptr, C.byref(n), C.byref(da), C.byref(db)))
         self.n, self.da, self.db = n.value, da.value, db.value
@@ -484,6 +486,9 @@ class CIMethod:
         self._two = None
         if build:
             self.build_ci()
+
+    def _create_space(self):
+        _check(lib().cf_space_create(self.ham.ptr, self.kind, C.byref(self.space.ptr)))
 
     # CIMethod::build_ci -> build_parallel (CIMethod.cpp:489-561)
     def build_ci(self):
@@ -589,6 +594,44 @@ class DOCI(CIMethod):
 
     def get_name(self):
         return "DOCI"
+
+
+def read_determinants(path: str):
+    """Permutator_File::read_file (Permutator.cpp:179-219): the (up, down)
+    strings of a determinant file, as uint64 arrays in file order."""
+    cnt = C.c_uint64()
+    _check(lib().cf_read_determinants(path.encode(), 0, None, None, C.byref(cnt)))
+    up = np.zeros(cnt.value, dtype=np.uint64)
+    down = np.zeros(cnt.value, dtype=np.uint64)
+    _check(lib().cf_read_determinants(path.encode(), cnt.value, up.ctypes.data_as(u64p), down.ctypes.data_as(u64p),
+                                      C.byref(cnt)))
+    return up, down
+
+
+class FCI_File(CIMethod):
+    """Selected CI over a determinant list (CIMethod.h:164-174,
+    CIMethod.cpp:890-917): FCI_File(ham, "dets.txt") or FCI_File(ham, (up, down)).
+    Rows follow the list order; the RDMs are not available."""
+
+    kind = CF_DETS
+
+    def __init__(self, ham: Hamiltonian, dets, value_prec: int = CF_F64, build: bool = True, zero: float = 1e-14):
+        if isinstance(dets, (str, bytes, os.PathLike)):
+            self.filename = os.fspath(dets)
+            up, down = read_determinants(self.filename)
+        else:
+            self.filename = None
+            up, down = dets
+        self.up = np.ascontiguousarray(up, dtype=np.uint64)
+        self.down = np.ascontiguousarray(down, dtype=np.uint64)
+        super().__init__(ham, value_prec=value_prec, build=build, zero=zero)
+
+    def _create_space(self):
+        _check(lib().cf_space_create_dets(self.ham.ptr, len(self.up), self.up.ctypes.data_as(u64p),
+                                          self.down.ctypes.data_as(u64p), C.byref(self.space.ptr)))
+
+    def get_name(self):
+        return "FCI_File"
 
 
 class FCI_Direct(CIMethod):

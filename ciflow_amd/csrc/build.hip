@@ -26,6 +26,7 @@
 // alpha double to the single column ja*db + ib.  segoff[ia] holds the prefix
 // offsets of these segments; a lane finds its segment by binary search.
 #include "kernels.h"
+#include "string_ops.h"
 
 namespace cf {
 
@@ -333,6 +334,113 @@ void launch_doci_rows(const DociRowsArgs& a, bool fill, hipStream_t stream) {
     unsigned grid = (unsigned)((a.nrows + 3) / 4);
     if (fill) k_doci_rows<true><<<grid, 256, 0, stream>>>(a);
     else k_doci_rows<false><<<grid, 256, 0, stream>>>(a);
+    CF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Determinant-list rows (selected CI).  get_ham_element (CIMethod.cpp:459-487)
+// between two arbitrary determinants, restated on bit strings with the
+// reference's formulas and summation order; one wavefront per row, the
+// lanes test 64 columns at a time (popcount of the differences first), and
+// the ballot of |v| > zero gives each survivor its slot.  Lower-triangle
+// entries are evaluated as get_ham_element(det_j, det_i), the orientation of
+// the reference's upper entry, so the stored matrix is exactly symmetric.
+// O(n^2) pair tests per pass, as the reference's construct_CI_matrix.
+// ---------------------------------------------------------------------------
+// one_diff_orbital(s1, s2, other) (CIMethod.cpp:409-427)
+__device__ __forceinline__ double det_one_diff(str_t s1, str_t s2, str_t other, const HamDev& H) {
+    const int L = H.L;
+    const str_t x = s1 ^ s2, com = s1 & s2;
+    const int o0 = __builtin_ctzll(x), o1 = 63 - __builtin_clzll(x);
+    double e = H.h[o0 * L + o1];
+    for (str_t t = com; t; t &= t - 1) {
+        const int z = __builtin_ctzll(t);
+        e += H.V[(((size_t)o0 * L + z) * L + o1) * L + z] - H.V[(((size_t)o0 * L + z) * L + z) * L + o1];
+    }
+    const int diff = stacked(com, o0, o1);
+    const double* w = H.W + ((size_t)o0 * L + o1) * L;  // V(o0, d, o1, d)
+    for (str_t t = other; t; t &= t - 1) e += w[__builtin_ctzll(t)];
+    return (diff & 1) ? -e : e;
+}
+
+// two_diff_orbitals(s1, s2), same spin (CIMethod.cpp:429-442): the orbitals of
+// s1 first, then those of s2, each ascending.
+__device__ __forceinline__ double det_two_same(str_t s1, str_t s2, const HamDev& H) {
+    const int L = H.L;
+    const str_t com = s1 & s2, a = s1 & ~s2, b = s2 & ~s1;
+    const int a0 = __builtin_ctzll(a), a1 = 63 - __builtin_clzll(a);
+    const int b0 = __builtin_ctzll(b), b1 = 63 - __builtin_clzll(b);
+    const int diff = stacked(com, a0, a1) + stacked(com, b0, b1);
+    const double e = H.V[(((size_t)a0 * L + a1) * L + b0) * L + b1] - H.V[(((size_t)a0 * L + a1) * L + b1) * L + b0];
+    return (diff & 1) ? -e : e;
+}
+
+// two_diff_orbitals(up1, up2, down1, down2) (CIMethod.cpp:444-457)
+__device__ __forceinline__ double det_two_mixed(str_t u1, str_t u2, str_t d1, str_t d2, const HamDev& H) {
+    const int L = H.L;
+    const int o0 = __builtin_ctzll(u1 & ~u2), o1 = __builtin_ctzll(u2 & ~u1);
+    const int p0 = __builtin_ctzll(d1 & ~d2), p1 = __builtin_ctzll(d2 & ~d1);
+    const int diff = stacked(u1 & u2, o0, o1) + stacked(d1 & d2, p0, p1);
+    const double e = H.V[(((size_t)o0 * L + p0) * L + o1) * L + p1];
+    return (diff & 1) ? -e : e;
+}
+
+__device__ __forceinline__ double det_element(str_t u1, str_t d1, str_t u2, str_t d2, const HamDev& H) {
+    const int pu = __popcll(u1 ^ u2), pd = __popcll(d1 ^ d2);
+    if (pd > 4 || pu > 4) return 0.0;
+    if (pu == 4 && pd == 0) return det_two_same(u1, u2, H);
+    if (pu == 2) {
+        if (pd == 0) return det_one_diff(u1, u2, d1, H);
+        if (pd == 2) return det_two_mixed(u1, u2, d1, d2, H);
+        return 0.0;
+    }
+    if (pd == 4 && pu == 0) return det_two_same(d1, d2, H);
+    if (pd == 2 && pu == 0) return det_one_diff(d1, d2, u1, H);
+    if (pd == 0 && pu == 0) {  // diagonal (CIMethod.cpp:398-407)
+        double s = diag_one_string(u1, H);
+        s += diag_one_string(d1, H);
+        s += jmix(u1, d1, H);
+        return s;
+    }
+    return 0.0;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_det_rows(DetRowsArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t i = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.n) return;
+    const str_t ui = a.up[i], di = a.down[i];
+    int64_t base = 0;
+    if (FILL) base = a.rowptr[i];
+    uint32_t cnt = 0;
+    for (uint64_t j0 = 0; j0 < a.n; j0 += 64) {
+        const uint64_t j = j0 + lane;
+        bool keep = false;
+        double val = 0.0;
+        if (j < a.n) {
+            const str_t uj = a.up[j], dj = a.down[j];
+            val = j >= i ? det_element(ui, di, uj, dj, a.H) : det_element(uj, dj, ui, di, a.H);
+            keep = fabs(val) > a.zero;
+        }
+        const uint64_t mask = __ballot(keep);
+        if (FILL && keep) {
+            const int64_t pos = base + cnt + lanes_below(mask);
+            a.col[pos] = (uint32_t)j;
+            a.val[pos] = val;
+            if (a.val32) a.val32[pos] = (float)val;
+        }
+        cnt += (uint32_t)__popcll(mask);
+    }
+    if (!FILL && lane == 0) a.rowcnt[i] = cnt;
+}
+
+void launch_det_rows(const DetRowsArgs& a, bool fill, hipStream_t stream) {
+    if (!a.n) return;
+    const uint64_t grid = (a.n + 3) / 4;
+    if (grid > 0xffffffffull) throw Error(ERR_UNSUPPORTED, "determinant list too long");
+    if (fill) k_det_rows<true><<<(unsigned)grid, 256, 0, stream>>>(a);
+    else k_det_rows<false><<<(unsigned)grid, 256, 0, stream>>>(a);
     CF_HIP(hipGetLastError());
 }
 

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <fstream>
 #include <memory>
 #include <string>
 #include <algorithm>
@@ -195,6 +196,7 @@ struct cf_space {
     DBuf<double> diag;
     DBuf<double> vec;  // resident eigenvector after cf_solve
     bool vec_valid = false;
+    DBuf<str_t> dup, ddown;  // CF_DETS: the determinant list, list order
 };
 
 // One SpMV launch configuration: kernel 1 (row gather, `lanes` per row),
@@ -824,6 +826,61 @@ int cf_space_create(cf_ham* ham, int kind, cf_space** out) {
     API_END
 }
 
+int cf_read_determinants(const char* path, uint64_t cap, uint64_t* up, uint64_t* down, uint64_t* count) {
+    API_BEGIN
+    CF_REQUIRE(path && count, "NULL argument");
+    CF_REQUIRE(cap == 0 || (up && down), "NULL output arrays");
+    std::ifstream f(path);
+    if (!f.is_open()) throw Error(ERR_ARG, std::string("cannot open ") + path);
+    std::string line;
+    uint64_t n = 0;
+    while (std::getline(f, line)) {
+        while (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.find('#') != std::string::npos) continue;
+        const size_t cut = line.find('|');
+        if (cut == std::string::npos) continue;
+        if (cut > 64)
+            throw Error(ERR_UNSUPPORTED, "determinant strings longer than 64 orbitals");
+        const uint64_t u = std::strtoull(line.substr(0, cut).c_str(), nullptr, 2);
+        const uint64_t d = std::strtoull(line.substr(cut + 1).c_str(), nullptr, 2);
+        if (n < cap) {
+            up[n] = u;
+            down[n] = d;
+        }
+        ++n;
+    }
+    *count = n;
+    API_END
+}
+
+int cf_space_create_dets(cf_ham* ham, uint64_t n, const uint64_t* up, const uint64_t* down, cf_space** out) {
+    API_BEGIN
+    CF_REQUIRE(ham && out && (n == 0 || (up && down)), "NULL argument");
+    CF_REQUIRE(n >= 1 && n < (1ull << 32), "determinant count must be in [1, 2^32)");
+    const int L = ham->L;
+    const uint64_t mask = L >= 64 ? ~0ull : ((1ull << L) - 1);
+    for (uint64_t i = 0; i < n; ++i) {
+        CF_REQUIRE((up[i] & ~mask) == 0 && (down[i] & ~mask) == 0,
+                   "determinant " + std::to_string(i) + " occupies an orbital >= L");
+        CF_REQUIRE(__builtin_popcountll(up[i]) == ham->nup && __builtin_popcountll(down[i]) == ham->ndown,
+                   "determinant " + std::to_string(i) + " does not hold nup alpha and ndown beta electrons");
+    }
+    cf_ctx* c = ham->ctx;
+    c->use();
+    std::unique_ptr<cf_space> sp(new cf_space());
+    sp->ham = ham;
+    sp->kind = CF_DETS;
+    sp->n = n;
+    sp->da = n;
+    sp->db = 1;
+    sp->dup.alloc(n);
+    sp->ddown.alloc(n);
+    copy_sync(sp->dup.p, up, n * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream);
+    copy_sync(sp->ddown.p, down, n * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream);
+    *out = sp.release();
+    API_END
+}
+
 int cf_space_destroy(cf_space* sp) {
     API_BEGIN
     if (sp) {
@@ -889,7 +946,19 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     const cf_ham* H = sp->ham;
     FciRowsArgs fa{};
     DociRowsArgs dargs{};
-    if (sp->kind == CF_FCI) {
+    DetRowsArgs targs{};
+    if (sp->kind == CF_DETS) {
+        targs.n = sp->n;
+        targs.nup = H->nup;
+        targs.ndown = H->ndown;
+        targs.up = sp->dup.p;
+        targs.down = sp->ddown.p;
+        targs.H = H->dev();
+        targs.zero = zero;
+        targs.rowcnt = cnt.p;
+        KTimer t(c, "crs_count");
+        launch_det_rows(targs, false, c->stream);
+    } else if (sp->kind == CF_FCI) {
         fa.da = sp->da;
         fa.db = sp->db;
         fa.Ea = sp->ta->E;
@@ -935,7 +1004,14 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     m->col.alloc(m->nnz);
     m->val.alloc(m->nnz);
     if (value_prec == CF_F32) m->val32.alloc(m->nnz);
-    if (sp->kind == CF_FCI) {
+    if (sp->kind == CF_DETS) {
+        targs.rowptr = m->rowptr.p;
+        targs.col = m->col.p;
+        targs.val = m->val.p;
+        targs.val32 = m->val32.p;
+        KTimer t(c, "crs_fill");
+        launch_det_rows(targs, true, c->stream);
+    } else if (sp->kind == CF_FCI) {
         fa.rowptr = m->rowptr.p;
         fa.col = m->col.p;
         fa.val = m->val.p;
@@ -1696,6 +1772,7 @@ int cf_ham_get(const cf_ham* ham, double* h, double* V) {
 int cf_rdm(cf_space* sp, const double* cvec, double* one, double* two) {
     API_BEGIN
     CF_REQUIRE(sp && cvec && one, "NULL argument");
+    CF_REQUIRE(sp->kind != CF_DETS, "RDMs are not available for determinant-list spaces");
     cf_ctx* c = sp->ham->ctx;
     c->use();
     const int L = sp->ham->L;
@@ -1714,6 +1791,7 @@ int cf_rdm(cf_space* sp, const double* cvec, double* one, double* two) {
 int cf_rdm_dense(cf_space* sp, const double* cvec, double* gamma, double* Gamma) {
     API_BEGIN
     CF_REQUIRE(sp && cvec && gamma, "NULL argument");
+    CF_REQUIRE(sp->kind != CF_DETS, "RDMs are not available for determinant-list spaces");
     cf_ctx* c = sp->ham->ctx;
     c->use();
     const int L = sp->ham->L;

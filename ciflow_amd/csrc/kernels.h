@@ -87,6 +87,23 @@ struct DociRowsArgs {
 };
 void launch_doci_rows(const DociRowsArgs& a, bool fill, hipStream_t stream);
 
+// Selected-CI rows over a determinant list (FCI_File::construct_CI_matrix,
+// CIMethod.cpp:890-912): row i in list order, full rows, columns ascending.
+struct DetRowsArgs {
+    uint64_t n;
+    int nup, ndown;
+    const str_t* up;
+    const str_t* down;
+    HamDev H;
+    double zero;
+    uint32_t* rowcnt;
+    const int64_t* rowptr;
+    uint32_t* col;
+    double* val;
+    float* val32;
+};
+void launch_det_rows(const DetRowsArgs& a, bool fill, hipStream_t stream);
+
 // Exclusive scan of 32-bit counts into 64-bit offsets (n+1 entries).
 void exclusive_scan_u32_to_i64(const uint32_t* cnt, int64_t* out, uint64_t n, hipStream_t stream);
 

@@ -10,6 +10,7 @@
 // a binomial table: no serial Gosper walk and no start_l skipping
 // (CIMethod.cpp:619-620, 763-764).
 #include "kernels.h"
+#include "string_ops.h"
 
 namespace cf {
 
@@ -70,26 +71,6 @@ void launch_rank(int L, int nel, const str_t* strings, uint64_t count, const uin
     CF_HIP(hipGetLastError());
 }
 
-// One-string part of CIMethod::diagonal: get_diag_cont_one_bitstring
-// (CIMethod.cpp:329-341), same summation order (h terms, then pairs k<l).
-__device__ __forceinline__ double diag_one_string(str_t s, const HamDev& H) {
-    const int L = H.L;
-    double acc = 0.0;
-    for (str_t t = s; t; t &= t - 1) {
-        int k = __builtin_ctzll(t);
-        acc += H.h[k * L + k];
-    }
-    for (str_t t = s; t; t &= t - 1) {
-        int k = __builtin_ctzll(t);
-        for (str_t u = t & (t - 1); u; u &= u - 1) {
-            int l = __builtin_ctzll(u);
-            const size_t kl = ((size_t)k * L + l) * L;
-            acc += H.V[(kl + k) * L + l] - H.V[(kl + l) * L + k];
-        }
-    }
-    return acc;
-}
-
 __global__ void k_diag1(int nel, const str_t* __restrict__ strs, uint64_t d, HamDev H, double* __restrict__ out) {
     (void)nel;
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,13 +83,6 @@ void launch_diag1(int nel, const str_t* strs, uint64_t d, HamDev H, double* out,
     CF_HIP(hipGetLastError());
 }
 
-// get_nstacked (CIMethod.cpp:382-396): common occupied orbitals strictly
-// between p and q.
-__device__ __forceinline__ int stacked(str_t common, int p, int q) {
-    int lo = p < q ? p : q, hi = p < q ? q : p;
-    str_t mask = (((str_t)1 << hi) - 1) & ~(((str_t)1 << lo) - 1);
-    return __popcll(common & mask);
-}
 
 // Excitation lists.  One 64-lane workgroup per string: enumerate self,
 // singles (occ f -> virt t) and same-spin doubles, bitonic-sort them in LDS by

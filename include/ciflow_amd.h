@@ -29,8 +29,9 @@ enum {
     CF_ERR_STATE = 6
 };
 
-/* CI space kinds: CIMethod subclasses FCI and DOCI (CIMethod.h:135-162). */
-enum { CF_FCI = 0, CF_DOCI = 1 };
+/* CI space kinds: CIMethod subclasses FCI and DOCI (CIMethod.h:135-162) and
+ * FCI_File, selected CI over a determinant list (CIMethod.h:164-174). */
+enum { CF_FCI = 0, CF_DOCI = 1, CF_DETS = 2 };
 
 /* Value storage of the CRS matrix and arithmetic of a product. */
 enum {
@@ -103,6 +104,18 @@ int cf_gen_rotation(int L, uint64_t seed, double sigma, double* K);
 int cf_space_create(cf_ham* ham, int kind, cf_space** out);
 int cf_space_destroy(cf_space* sp);
 int cf_space_dims(const cf_space* sp, uint64_t* n, uint64_t* dim_up, uint64_t* dim_down);
+/* ---- selected CI over a determinant list: Permutator_File::read_file
+ *      (Permutator.cpp:179-219) and FCI_File (CIMethod.cpp:890-917).
+ * Reads the "UP|DOWN" lines of a determinant file (binary strings, highest
+ * orbital first; lines holding '#' are comments, lines without '|' are
+ * skipped).  Host only.  Stores at most cap determinants, *count = number in
+ * the file; strings longer than 64 orbitals are CF_ERR_UNSUPPORTED. */
+int cf_read_determinants(const char* path, uint64_t cap, uint64_t* up, uint64_t* down, uint64_t* count);
+/* Space of n determinants (alpha string up[i], beta string down[i], in list
+ * order; each with ham's nup/ndown electrons in its L orbitals).  cf_crs_build
+ * then builds the reference's FCI_File matrix (rows in list order, full rows
+ * here); dims report (n, n, 1).  RDMs are not available for these spaces. */
+int cf_space_create_dets(cf_ham* ham, uint64_t n, const uint64_t* up, const uint64_t* down, cf_space** out);
 /* Unrank strings [first, first+count) of the n-of-L space on the GPU (host out). */
 int cf_strings_unrank(cf_ctx* ctx, int L, int nel, uint64_t first, uint64_t count, uint64_t* out);
 /* Rank arbitrary n-of-L strings on the GPU (host in/out). */

@@ -21,12 +21,16 @@
 //                                 energy identity E_dens == E_CI
 //   or_gen_integrals              committed SHA-256 of the generated arrays and
 //                                 closed-form checks (symmetry, PSD of the ERI)
+//   or_read_dets, or_build_crs_dets  the reference's selected-CI KAT (Be cc-pVDZ,
+//                                 100 determinants, TestDetFile.cpp:32) and the
+//                                 FCI energy identity on the full BeH2 list
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -546,6 +550,80 @@ extern "C" void or_crs_copy(void* m, uint64_t* row, uint32_t* col, double* val) 
     if (val) std::memcpy(val, c->data.data(), c->data.size() * sizeof(double));
 }
 extern "C" void or_crs_free(void* m) { delete (OrCRS*)m; }
+
+// ---------------------------------------------------------------------------
+// Selected CI from a determinant list.
+// ---------------------------------------------------------------------------
+// Permutator_File::read_file (Permutator.cpp:179-219): every line without a
+// '#' is "UP|DOWN", each a binary string with the highest orbital first
+// (strtoull base 2).  Strings of more than 64 orbitals need the reference's
+// multi-word TYPE and are rejected here (return -1); lines without '|' are
+// skipped (the reference would misparse them).  Returns the number of
+// determinants (at most cap are stored).
+extern "C" int64_t or_read_dets(const char* path, uint64_t cap, TYPE* up, TYPE* down) {
+    FILE* f = std::fopen(path, "r");
+    if (!f) return -2;
+    char buf[4096];
+    int64_t n = 0;
+    while (std::fgets(buf, sizeof buf, f)) {
+        std::string line(buf);
+        while (!line.empty() && (line.back() == '\n' || line.back() == '\r')) line.pop_back();
+        if (line.find('#') != std::string::npos) continue;
+        const size_t cut = line.find('|');
+        if (cut == std::string::npos) continue;
+        if (cut > 64) {
+            std::fclose(f);
+            return -1;
+        }
+        const TYPE u = (TYPE)std::strtoull(line.substr(0, cut).c_str(), nullptr, 2);
+        const TYPE d = (TYPE)std::strtoull(line.substr(cut + 1).c_str(), nullptr, 2);
+        if ((uint64_t)n < cap) {
+            up[n] = u;
+            down[n] = d;
+        }
+        ++n;
+    }
+    std::fclose(f);
+    return n;
+}
+
+// FCI_File::construct_CI_matrix (CIMethod.cpp:890-912): row i in file order,
+// columns j = i..n-1 with get_ham_element(det_i, det_j), pushed in order.
+void file_construct(const Ham& H, double zero, CRS& mat, const TYPE* up, const TYPE* down, uint64_t n,
+                    uint64_t start_l, uint64_t end_l) {
+    SC sc(H);
+    mat.zero = zero;
+    for (uint64_t i = start_l; i < end_l; i++) {
+        mat.NewRow();
+        for (uint64_t j = i; j < n; j++) mat.PushToRow((unsigned)j, sc.element(up[i], down[i], up[j], down[j]));
+    }
+}
+
+// build_parallel + add_from_list for a determinant list (CIMethod.cpp:489-561).
+extern "C" void* or_build_crs_dets(int L, int nup, int ndown, const double* h, const double* V, double zero,
+                                   uint64_t n, const TYPE* up, const TYPE* down, int num_t) {
+    Ham H{L, nup, ndown, h, V};
+    if (num_t < 1) num_t = 1;
+    std::vector<unsigned int> wl = workload_split(n, num_t);
+    std::vector<std::unique_ptr<CRS>> parts(num_t);
+#pragma omp parallel for schedule(static, 1) num_threads(num_t)
+    for (int me = 0; me < num_t; ++me) {
+        const uint64_t a = wl[me], b = wl[me + 1];
+        parts[me].reset(new CRS((unsigned)(b - a)));
+        file_construct(H, zero, *parts[me], up, down, n, a, b);
+    }
+    OrCRS* out = new OrCRS();
+    for (auto& p : parts) {
+        uint64_t prev = out->data.size();
+        out->data.insert(out->data.end(), p->data.begin(), p->data.end());
+        out->col.insert(out->col.end(), p->col.begin(), p->col.end());
+        for (size_t i = 0; i < p->row.size(); ++i) out->row.push_back(prev + p->row[i]);
+        p.reset();
+    }
+    out->row.push_back(out->data.size());
+    out->nrows = out->row.size() - 1;
+    return out;
+}
 
 // Matrix element between two determinants (get_ham_element, CIMethod.cpp:459-487).
 extern "C" double or_ham_element(int L, int nup, int ndown, const double* h, const double* V,

@@ -76,6 +76,10 @@ def lib():
         L.or_dens_energy.restype = C.c_double
         L.or_num_threads.restype = C.c_int
         L.or_rotate_integrals.argtypes = [C.c_int, dp, dp, dp, dp, dp]
+        L.or_read_dets.argtypes = [C.c_char_p, u64, u64p, u64p]
+        L.or_read_dets.restype = C.c_int64
+        L.or_build_crs_dets.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp, C.c_double, u64, u64p, u64p, C.c_int]
+        L.or_build_crs_dets.restype = C.c_void_p
         _lib = L
     return _lib
 
@@ -148,6 +152,31 @@ def build_crs(kind: str, h, V, nup: int, ndown: int, start: int = 0, end: int | 
     if threads <= 0:
         threads = int(lib().or_num_threads())
     hh = lib().or_build_crs(k, L, nup, ndown, _d(h), _d(V), zero, start, end, threads)
+    return _take_crs(hh)
+
+
+def read_dets(path: str):
+    """Permutator_File::read_file: (up, down) uint64 arrays in file order."""
+    n = int(lib().or_read_dets(path.encode(), 0, None, None))
+    if n < 0:
+        raise ValueError(f"cannot read determinants from {path} ({n})")
+    up = np.zeros(n, dtype=np.uint64)
+    down = np.zeros(n, dtype=np.uint64)
+    lib().or_read_dets(path.encode(), n, up.ctypes.data_as(u64p), down.ctypes.data_as(u64p))
+    return up, down
+
+
+def build_crs_dets(h, V, nup: int, ndown: int, up, down, zero: float = 1e-14, threads: int = 0) -> CRS:
+    """FCI_File::construct_CI_matrix over a determinant list via build_parallel."""
+    L = h.shape[0]
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    up = np.ascontiguousarray(up, dtype=np.uint64)
+    down = np.ascontiguousarray(down, dtype=np.uint64)
+    if threads <= 0:
+        threads = int(lib().or_num_threads())
+    hh = lib().or_build_crs_dets(L, nup, ndown, _d(h), _d(V), zero, len(up), up.ctypes.data_as(u64p),
+                                 down.ctypes.data_as(u64p), threads)
     return _take_crs(hh)
 
 

@@ -10,11 +10,13 @@ dense physics V[L][L][L][L] as getVmat(i,j,k,l) returns it, econst, nalpha,
 nbeta and the orbital irreps.
 
 Run from the repo root:  python tests/golden/make_fixtures.py
-(needs /root/reference; the generated .npz files are committed).
+(needs /root/reference; the generated .npz files are committed).  The two
+determinant lists of the reference's selected-CI tests are copied unchanged.
 """
 from __future__ import annotations
 
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -60,10 +62,16 @@ def main():
     if not os.path.isdir(REF):
         print("reference data not found", file=sys.stderr)
         return 1
-    for name, src in (("beh2_sto3g", "beh2_sto3g_1_5.dat"), ("psi0_631g", "psi0_6-31g0.60.mout")):
+    for name, src in (("beh2_sto3g", "beh2_sto3g_1_5.dat"), ("psi0_631g", "psi0_6-31g0.60.mout"),
+                      ("be_ccpvdz", "Be_ccpVDZ.dat")):
         d = read_dump(os.path.join(REF, src))
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d)
         print(name, d["L"], d["nalpha"], d["nbeta"], d["econst"])
+    # determinant lists the reference's selected-CI tests read (Permutator_File),
+    # kept byte for byte as data
+    for src in ("GKCI_Be_cntSp_cc-pVDZ_0100.txt", "fci_beh2_sto-3gdeterminants.dat"):
+        shutil.copyfile(os.path.join(REF, src), os.path.join(OUT, src))
+        print(src)
     return 0
 
 

@@ -112,6 +112,77 @@ def test_config1_digest_matches_golden():
     assert (n, nnz_up, nnz_full) == (4900, 886900, 1768900)
 
 
+# ---- selected CI over determinant lists (FCI_File) ----------------------------
+DET_FILES = {"be_gkci100": ("be_ccpvdz.npz", "GKCI_Be_cntSp_cc-pVDZ_0100.txt"),
+             "beh2_full_list": ("beh2_sto3g.npz", "fci_beh2_sto-3gdeterminants.dat")}
+
+
+def check_det_crs(cim, ref):
+    row, col, val = cim.get_mat().export_upper()
+    assert np.array_equal(row, ref.row)
+    assert np.array_equal(col, ref.col)
+    assert np.array_equal(val, ref.data)  # bit-exact values, hence signs
+    full = O.full_from_upper(ref)
+    frow, fcol, fval = cim.get_mat().export_full()
+    assert np.array_equal(frow, full.row) and np.array_equal(fcol, full.col) and np.array_equal(fval, full.data)
+
+
+@pytest.mark.parametrize("case", list(DET_FILES))
+def test_det_file_crs_bit_exact_and_kat_energy(case):
+    """FCI_File from the reference's determinant files: the upper CRS equals the
+    oracle's FCI_File::construct_CI_matrix bit for bit, and the energy meets the
+    KAT (TestDetFile.cpp:32) or, for the full BeH2 list, the FCI energy."""
+    say("det file", case)
+    npz, dets = DET_FILES[case]
+    h, V, na, nb, ec = fixture(npz)
+    path = os.path.join(GOLD, dets)
+    cim = CA.FCI_File(CA.Hamiltonian(h, V, na, nb, ec), path)
+    up, down = O.read_dets(path)
+    assert np.array_equal(cim.up, up) and np.array_equal(cim.down, down)
+    check_det_crs(cim, O.build_crs_dets(h, V, na, nb, up, down))
+    info = cim.solve()
+    assert info.status == 0
+    expected = -14.5951673370889 if case == "be_gkci100" else golden()["beh2"]["fci_energy"]
+    assert abs(cim.get_ci_energy() - expected) < 1e-10
+
+
+@pytest.mark.parametrize("L,na,nb,count,seed", [(8, 4, 4, 3000, 1), (9, 5, 3, 2000, 2), (7, 2, 2, 441, 3)])
+def test_det_list_random_subset_bit_exact(L, na, nb, count, seed):
+    """Shuffled subsets of an FCI space (the whole space for L=7 2a2b): CRS
+    bit-exact, FP64 energy against dense eigh of the oracle matrix, FP32-value
+    solve with FP64 polish within 1e-8."""
+    say("det subset", L, na, nb, count)
+    h, V = CA.gen_integrals(L, seed)
+    A = O.enumerate_strings(L, na)
+    B = O.enumerate_strings(L, nb)
+    rng = np.random.default_rng(seed)
+    pick = rng.permutation(len(A) * len(B))[:count]
+    up = A[pick // len(B)]
+    down = B[pick % len(B)]
+    cim = CA.FCI_File(CA.Hamiltonian(h, V, na, nb), (up, down), value_prec=CA.CF_F32)
+    ref = O.build_crs_dets(h, V, na, nb, up, down)
+    check_det_crs(cim, ref)
+    w = np.linalg.eigvalsh(O.dense_from_upper(ref))[0]
+    cim.solve()
+    assert abs(cim.get_ci_energy() - w) < 1e-10
+    cim.solve(precision=CA.CF_F32)
+    assert abs(cim.get_ci_energy() - w) < 1e-8
+
+
+def test_det_list_rejects_bad_input():
+    h, V = CA.gen_integrals(6, 1)
+    ham = CA.Hamiltonian(h, V, 3, 3)
+    with pytest.raises(CA.CIFlowError):  # two beta electrons instead of three
+        CA.FCI_File(ham, (np.array([7], dtype=np.uint64), np.array([3], dtype=np.uint64)))
+    with pytest.raises(CA.CIFlowError):  # orbital 6 outside L = 6
+        CA.FCI_File(ham, (np.array([7 << 4], dtype=np.uint64), np.array([7], dtype=np.uint64)))
+    cim = CA.FCI_File(ham, (np.array([7, 11], dtype=np.uint64), np.array([7, 7], dtype=np.uint64)))
+    assert cim.get_mat().info()[0] == 2
+    cim._eigvec = np.array([0.6, 0.8])
+    with pytest.raises(CA.CIFlowError):  # no RDMs for determinant lists
+        cim.construct_density()
+
+
 # ---- (c) SpMV ----------------------------------------------------------------
 @pytest.mark.parametrize("case", ["fci_L8_4a4b_config1", "doci_L10_5", "fci_beh2"])
 def test_mvprod_matches_oracle(case):

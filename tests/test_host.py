@@ -6,6 +6,8 @@
     numpy operators, including tiny spaces (n < ncv) and clustered spectra,
   * argument errors come back as status codes, not crashes.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -129,3 +131,22 @@ def test_request_footprint_and_argument_checks():
     assert fp(CA.CF_FCI, 11, 5, 5, CA.CF_F64) < big
     with pytest.raises(CA.CIFlowError):
         fp(CA.CF_FCI, 8, 9, 4)
+
+
+def test_determinant_file_parser_matches_oracle(tmp_path):
+    """Permutator_File::read_file semantics: '#' lines are comments, strings are
+    binary with the highest orbital first, CR line ends are tolerated, lines
+    without '|' are skipped, strings beyond 64 orbitals are rejected."""
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    for name in ("GKCI_Be_cntSp_cc-pVDZ_0100.txt", "fci_beh2_sto-3gdeterminants.dat"):
+        path = os.path.join(gold, name)
+        up, down = CA.read_determinants(path)
+        ou, od = O.read_dets(path)
+        assert np.array_equal(up, ou) and np.array_equal(down, od)
+    p = tmp_path / "dets.txt"
+    p.write_bytes(b"# header | with a bar\n0011|0101\r\n\n1001|0011\n#0110|0110\n")
+    up, down = CA.read_determinants(str(p))
+    assert up.tolist() == [3, 9] and down.tolist() == [5, 3]
+    p.write_text("1" * 65 + "|" + "1" * 65 + "\n")
+    with pytest.raises(CA.CIFlowError):
+        CA.read_determinants(str(p))

@@ -246,3 +246,57 @@ def test_rotation_oracle_matches_einsum_and_round_trips():
     e0 = np.linalg.eigvalsh(O.dense_from_upper(O.build_crs("FCI", h, V, 3, 3)))[0]
     e1 = np.linalg.eigvalsh(O.dense_from_upper(O.build_crs("FCI", ho, Vo, 3, 3)))[0]
     assert abs(e0 - e1) < 1e-11
+
+
+# --- selected CI from determinant lists (FCI_File, Permutator_File) ----------
+def random_det_list(L, na, nb, count, seed):
+    dets = BF.basis(L, na, nb)
+    rng = np.random.default_rng(seed)
+    pick = rng.permutation(len(dets))[:count]  # a subset in shuffled order
+    up = np.array([dets[i][0] for i in pick], dtype=np.uint64)
+    down = np.array([dets[i][1] for i in pick], dtype=np.uint64)
+    return up, down
+
+
+@pytest.mark.parametrize("L,na,nb,count,seed", [(5, 2, 2, 60, 1), (6, 3, 2, 90, 2), (6, 3, 3, 400, 3)])
+def test_det_list_matrix_matches_second_quantisation(L, na, nb, count, seed):
+    """FCI_File::construct_CI_matrix on an arbitrary ordered subset: the dense
+    matrix equals <I|H|J> from explicit creation/annihilation operators."""
+    h, V = O.gen_integrals(L, seed + 10)
+    up, down = random_det_list(L, na, nb, count, seed)
+    m = O.build_crs_dets(h, V, na, nb, up, down)
+    assert m.n == count
+    H = O.dense_from_upper(m)
+    Hb = BF.hamiltonian(h, V, na, nb, dets=[(int(a), int(b)) for a, b in zip(up, down)])
+    assert np.abs(H - Hb).max() < 1e-13
+
+
+def test_det_file_be_kat():
+    """TestDetFile.cpp:32: Be cc-pVDZ over the 100-determinant GKCI list,
+    E = -14.5951673370889 (BOOST_CHECK_CLOSE 1e-10 %, i.e. 1e-12 relative)."""
+    h, V, L, na, nb, ec = load_fixture("be_ccpvdz.npz")
+    up, down = O.read_dets(os.path.join(GOLD, "GKCI_Be_cntSp_cc-pVDZ_0100.txt"))
+    assert len(up) == 100
+    m = O.build_crs_dets(h, V, na, nb, up, down)
+    E, _, info = O.lanczos(O.full_from_upper(m))
+    assert info["status"] == 0
+    assert abs((E + ec) - (-14.5951673370889)) <= 1e-12 * 14.6
+
+
+def test_det_file_full_beh2_list_is_the_permuted_fci_matrix():
+    """The BeH2 file lists all 35 x 35 determinants in its own order: its matrix is
+    the FCI matrix permuted (same pattern, elements to 1e-12) and its ground
+    state is the FCI energy."""
+    h, V, L, na, nb, ec = load_fixture("beh2_sto3g.npz")
+    up, down = O.read_dets(os.path.join(GOLD, "fci_beh2_sto-3gdeterminants.dat"))
+    m = O.build_crs_dets(h, V, na, nb, up, down)
+    Hd = O.dense_from_upper(m)
+    A = O.enumerate_strings(L, na)
+    B = O.enumerate_strings(L, nb)
+    pos = np.array([O.rank(L, na, int(a)) * len(B) + O.rank(L, nb, int(b)) for a, b in zip(up, down)])
+    assert sorted(pos.tolist()) == list(range(len(A) * len(B)))
+    Hf = O.dense_from_upper(O.build_crs("FCI", h, V, na, nb))[np.ix_(pos, pos)]
+    assert np.array_equal(Hd != 0, Hf != 0)
+    assert np.abs(Hd - Hf).max() <= 1e-12 * np.abs(Hf).max()
+    E, _, _ = O.lanczos(O.full_from_upper(m))
+    assert abs((E + ec) - golden()["beh2"]["fci_energy"]) < 1e-10
