@@ -611,7 +611,8 @@ def read_determinants(path: str):
 class FCI_File(CIMethod):
     """Selected CI over a determinant list (CIMethod.h:164-174,
     CIMethod.cpp:890-917): FCI_File(ham, "dets.txt") or FCI_File(ham, (up, down)).
-    Rows follow the list order; the RDMs are not available."""
+    Rows follow the list order; construct_density gives DensFILE's RDMs
+    (CIDens.cpp:1468-1526) for L <= 32 and distinct determinants."""
 
     kind = CF_DETS
 

@@ -1772,13 +1772,15 @@ int cf_ham_get(const cf_ham* ham, double* h, double* V) {
 int cf_rdm(cf_space* sp, const double* cvec, double* one, double* two) {
     API_BEGIN
     CF_REQUIRE(sp && cvec && one, "NULL argument");
-    CF_REQUIRE(sp->kind != CF_DETS, "RDMs are not available for determinant-list spaces");
     cf_ctx* c = sp->ham->ctx;
     c->use();
     const int L = sp->ham->L;
     std::vector<double> g(2 * L * L), G;
     if (two) G.resize((size_t)3 * L * L * L * L);
-    {
+    if (sp->kind == CF_DETS) {
+        KTimer t(c, "rdm");
+        rdm_dets(c->stream, L, sp->n, sp->dup.p, sp->ddown.p, cvec, g.data(), two ? G.data() : nullptr);
+    } else {
         KTimer t(c, "rdm");
         rdm_dense(c->stream, sp->kind, L, sp->ham->nup, sp->ham->ndown, sp->da, sp->db, sp->ta->s.p, sp->tb->s.p,
                   sp->ta->sub.p, sp->tb->sub.p, sp->ta->S, sp->tb->S, c->binom.p, cvec, sp->n, g.data(),
@@ -1791,13 +1793,15 @@ int cf_rdm(cf_space* sp, const double* cvec, double* one, double* two) {
 int cf_rdm_dense(cf_space* sp, const double* cvec, double* gamma, double* Gamma) {
     API_BEGIN
     CF_REQUIRE(sp && cvec && gamma, "NULL argument");
-    CF_REQUIRE(sp->kind != CF_DETS, "RDMs are not available for determinant-list spaces");
     cf_ctx* c = sp->ham->ctx;
     c->use();
     const int L = sp->ham->L;
     KTimer t(c, "rdm");
-    rdm_dense(c->stream, sp->kind, L, sp->ham->nup, sp->ham->ndown, sp->da, sp->db, sp->ta->s.p, sp->tb->s.p,
-              sp->ta->sub.p, sp->tb->sub.p, sp->ta->S, sp->tb->S, c->binom.p, cvec, sp->n, gamma, Gamma);
+    if (sp->kind == CF_DETS)
+        rdm_dets(c->stream, L, sp->n, sp->dup.p, sp->ddown.p, cvec, gamma, Gamma);
+    else
+        rdm_dense(c->stream, sp->kind, L, sp->ham->nup, sp->ham->ndown, sp->da, sp->db, sp->ta->s.p, sp->tb->s.p,
+                  sp->ta->sub.p, sp->tb->sub.p, sp->ta->S, sp->tb->S, c->binom.p, cvec, sp->n, gamma, Gamma);
     API_END
 }
 

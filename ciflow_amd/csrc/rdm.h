@@ -14,6 +14,12 @@ void rdm_dense(hipStream_t stream, int kind, int L, int nup, int ndown, uint64_t
                const str_t* sb, const Exc* suba, const Exc* subb, int Sa, int Sb, const uint64_t* binom,
                const double* c_host, uint64_t n, double* gamma, double* Gamma);
 
+// Dense RDMs of a determinant-list space (DensFILE, CIDens.cpp:1468-1526):
+// up/down are the device strings of the n list determinants (list order),
+// c_host the CI vector.  L <= 32; the determinants must be distinct.
+void rdm_dets(hipStream_t stream, int L, uint64_t n, const str_t* up, const str_t* down, const double* c_host,
+              double* gamma, double* Gamma);
+
 // Pack dense RDMs into the reference storage (CIDens.cpp:166-288,
 // matrix.cpp:674-689): one[2][L(L+1)/2], two[3][L^2(L^2+1)/2] (two may be null).
 void pack_rdm(int L, const double* gamma, const double* Gamma, double* one, double* two);

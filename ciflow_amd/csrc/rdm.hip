@@ -20,6 +20,10 @@
 // P_jk = sum_{J: k in J, j not in J} c_{J-k+j} c_J; each thread owns one (j,k)
 // bin and reduces over a chunk of determinants, chunks summed in order.
 #include "rdm.h"
+#include "string_ops.h"
+
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
 
 #include <vector>
 
@@ -256,6 +260,55 @@ __global__ void k_doci_rdm(uint64_t d, int L, const str_t* __restrict__ s, const
     }
 }
 
+// gamma and Gamma from the transition densities X^a, X^b (n x L^2 rows) and
+// the coefficients c_K of the same n determinants (0 for a determinant
+// outside the CI space): gamma^s = sum_K c_K X^s(K), Gamma = X^T X Gram
+// products minus the contraction term, chunks reduced in fixed order.
+static void rdm_from_x(hipStream_t stream, int L, uint64_t n, const double* c, const double* Xa, const double* Xb,
+                       double* gamma, double* Gamma) {
+    const int P = L * L;
+    uint64_t nchunk = (n + 1023) / 1024;
+    if (nchunk > 512) nchunk = 512;
+    if (nchunk < 1) nchunk = 1;
+    const uint64_t rows = (n + nchunk - 1) / nchunk;
+    {
+        DBuf<double> part(nchunk * (uint64_t)P), tot(2 * (uint64_t)P);
+        for (int s = 0; s < 2; ++s) {
+            k_gamma_part<<<(unsigned)nchunk, 256, 0, stream>>>(n, P, c, s == 0 ? Xa : Xb, rows, part.p);
+            CF_HIP(hipGetLastError());
+            k_sum_chunks<<<(unsigned)((P + 255) / 256), 256, 0, stream>>>((int)nchunk, P, part.p, tot.p + s * P);
+            CF_HIP(hipGetLastError());
+        }
+        CF_HIP(hipMemcpyAsync(gamma, tot.p, 2 * P * sizeof(double), hipMemcpyDeviceToHost, stream));
+        CF_HIP(hipStreamSynchronize(stream));
+    }
+    if (!Gamma) return;
+    const uint64_t PP = (uint64_t)P * P;
+    DBuf<double> part(nchunk * PP), G(3 * PP);
+    const double* ops[3][2] = {{Xa, Xa}, {Xa, Xb}, {Xb, Xb}};
+    for (int b = 0; b < 3; ++b) {
+        launch_gram(n, P, ops[b][0], ops[b][1], rows, nchunk, part.p, stream);
+        k_sum_chunks<<<(unsigned)((PP + 255) / 256), 256, 0, stream>>>((int)nchunk, PP, part.p, G.p + b * PP);
+        CF_HIP(hipGetLastError());
+    }
+    std::vector<double> hG(3 * PP);
+    CF_HIP(hipMemcpyAsync(hG.data(), G.p, 3 * PP * sizeof(double), hipMemcpyDeviceToHost, stream));
+    CF_HIP(hipStreamSynchronize(stream));
+    const size_t L4 = PP;
+    for (int b = 0; b < 3; ++b) {
+        const double* Gb = hG.data() + b * PP;
+        const int s = b == 2 ? 1 : 0;
+        for (int p = 0; p < L; ++p)
+            for (int q = 0; q < L; ++q)
+                for (int r = 0; r < L; ++r)
+                    for (int t = 0; t < L; ++t) {
+                        double v = Gb[(size_t)(r * L + p) * P + (q * L + t)];
+                        if (b != 1 && q == r) v -= gamma[s * P + p * L + t];
+                        Gamma[b * L4 + (((size_t)p * L + q) * L + r) * L + t] = v;
+                    }
+    }
+}
+
 void rdm_dense(hipStream_t stream, int kind, int L, int nup, int ndown, uint64_t da, uint64_t db, const str_t* sa,
                const str_t* sb, const Exc* suba, const Exc* subb, int Sa, int Sb, const uint64_t* binom,
                const double* c_host, uint64_t n, double* gamma, double* Gamma) {
@@ -314,46 +367,148 @@ void rdm_dense(hipStream_t stream, int kind, int L, int nup, int ndown, uint64_t
     CF_HIP(hipMemsetAsync(Xb.p, 0, Xb.bytes(), stream));
     k_build_x<<<(unsigned)((n + 127) / 128), 128, 0, stream>>>(da, db, L, sa, sb, suba, subb, Sa, Sb, c.p, Xa.p, Xb.p);
     CF_HIP(hipGetLastError());
-    uint64_t nchunk = (n + 1023) / 1024;
-    if (nchunk > 512) nchunk = 512;
-    if (nchunk < 1) nchunk = 1;
-    const uint64_t rows = (n + nchunk - 1) / nchunk;
-    {
-        DBuf<double> part(nchunk * (uint64_t)P), tot(2 * (uint64_t)P);
-        for (int s = 0; s < 2; ++s) {
-            k_gamma_part<<<(unsigned)nchunk, 256, 0, stream>>>(n, P, c.p, s == 0 ? Xa.p : Xb.p, rows, part.p);
-            CF_HIP(hipGetLastError());
-            k_sum_chunks<<<(unsigned)((P + 255) / 256), 256, 0, stream>>>((int)nchunk, P, part.p, tot.p + s * P);
-            CF_HIP(hipGetLastError());
+    rdm_from_x(stream, L, n, c.p, Xa.p, Xb.p, gamma, Gamma);
+}
+
+// ---------------------------------------------------------------------------
+// Determinant lists (DensFILE, CIDens.cpp:1468-1526).  The same transition
+// densities and Gram products as for FCI, over the determinants K that one
+// single excitation reaches from the list (the list included): K is keyed
+// up | down << L, the keys are sorted and made unique (hipCUB), and every
+// list determinant J scatters c_J into X^s[K][a*L + b] for K = J - b + a
+// (b occupied in J's s-string, a = b or empty), with the single's phase.
+// (K, s, a, b) fixes J, so the scatter has no conflicts and is deterministic.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t det_key(str_t u, str_t d, int L) { return (uint64_t)u | ((uint64_t)d << L); }
+
+// keys[J*E + e]: J itself, then its alpha singles, then its beta singles.
+__global__ void k_det_closure_keys(uint64_t n, int L, int E, const str_t* __restrict__ up,
+                                   const str_t* __restrict__ down, uint64_t* __restrict__ keys) {
+    const uint64_t J = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (J >= n) return;
+    const str_t u = up[J], d = down[J];
+    const str_t full = L >= 64 ? ~(str_t)0 : (((str_t)1 << L) - 1);
+    uint64_t* o = keys + J * (uint64_t)E;
+    int e = 0;
+    o[e++] = det_key(u, d, L);
+    for (str_t t = u; t; t &= t - 1) {
+        const str_t bb = t & (~t + 1);
+        for (str_t v = full & ~u; v; v &= v - 1) o[e++] = det_key((u ^ bb) | (v & (~v + 1)), d, L);
+    }
+    for (str_t t = d; t; t &= t - 1) {
+        const str_t bb = t & (~t + 1);
+        for (str_t v = full & ~d; v; v &= v - 1) o[e++] = det_key(u, (d ^ bb) | (v & (~v + 1)), L);
+    }
+}
+
+__device__ __forceinline__ uint64_t find_key(const uint64_t* __restrict__ ks, uint64_t nk, uint64_t key) {
+    uint64_t lo = 0, hi = nk;  // first index with ks[i] >= key
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (ks[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_det_scatter_x(uint64_t n, int L, const str_t* __restrict__ up, const str_t* __restrict__ down,
+                                const double* __restrict__ c, const uint64_t* __restrict__ ks, uint64_t nk,
+                                double* __restrict__ cK, double* __restrict__ Xa, double* __restrict__ Xb) {
+    const uint64_t J = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (J >= n) return;
+    const str_t u = up[J], d = down[J];
+    const str_t full = L >= 64 ? ~(str_t)0 : (((str_t)1 << L) - 1);
+    const size_t P = (size_t)L * L;
+    const double cj = c[J];
+    const uint64_t self = find_key(ks, nk, det_key(u, d, L));
+    cK[self] = cj;
+    for (str_t t = u; t; t &= t - 1) {  // diagonal: X^a[J][b*L + b] = c_J
+        const int b = __builtin_ctzll(t);
+        Xa[self * P + b * L + b] = cj;
+    }
+    for (str_t t = d; t; t &= t - 1) {
+        const int b = __builtin_ctzll(t);
+        Xb[self * P + b * L + b] = cj;
+    }
+    for (int s = 0; s < 2; ++s) {
+        const str_t own = s == 0 ? u : d;
+        double* X = s == 0 ? Xa : Xb;
+        for (str_t t = own; t; t &= t - 1) {
+            const int b = __builtin_ctzll(t);
+            for (str_t v = full & ~own; v; v &= v - 1) {
+                const int a = __builtin_ctzll(v);
+                const str_t ks_ = (own ^ ((str_t)1 << b)) | ((str_t)1 << a);  // K's s-string: J - b + a
+                const uint64_t key = s == 0 ? det_key(ks_, d, L) : det_key(u, ks_, L);
+                const uint64_t k = find_key(ks, nk, key);
+                const int ph = stacked(own & ks_, a, b) & 1;  // <K| a+_a a_b |J> phase
+                X[k * P + a * L + b] = ph ? -cj : cj;
+            }
         }
-        CF_HIP(hipMemcpyAsync(gamma, tot.p, 2 * P * sizeof(double), hipMemcpyDeviceToHost, stream));
+    }
+}
+
+__global__ void k_count_equal_neighbours(uint64_t n, const uint64_t* __restrict__ k, unsigned int* __restrict__ dup) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1 < n && k[i] == k[i + 1]) atomicAdd(dup, 1u);
+}
+
+void rdm_dets(hipStream_t stream, int L, uint64_t n, const str_t* up, const str_t* down, const double* c_host,
+              double* gamma, double* Gamma) {
+    if (2 * L > 64) throw Error(ERR_UNSUPPORTED, "determinant-list RDMs need L <= 32");
+    int na = 0, nb = 0;
+    {
+        str_t u0 = 0, d0 = 0;
+        CF_HIP(hipMemcpyAsync(&u0, up, sizeof(str_t), hipMemcpyDeviceToHost, stream));
+        CF_HIP(hipMemcpyAsync(&d0, down, sizeof(str_t), hipMemcpyDeviceToHost, stream));
         CF_HIP(hipStreamSynchronize(stream));
+        na = __builtin_popcountll(u0);
+        nb = __builtin_popcountll(d0);
     }
-    if (!Gamma) return;
-    const uint64_t PP = (uint64_t)P * P;
-    DBuf<double> part(nchunk * PP), G(3 * PP);
-    const double* ops[3][2] = {{Xa.p, Xa.p}, {Xa.p, Xb.p}, {Xb.p, Xb.p}};
-    for (int b = 0; b < 3; ++b) {
-        launch_gram(n, P, ops[b][0], ops[b][1], rows, nchunk, part.p, stream);
-        k_sum_chunks<<<(unsigned)((PP + 255) / 256), 256, 0, stream>>>((int)nchunk, PP, part.p, G.p + b * PP);
+    const int E = 1 + na * (L - na) + nb * (L - nb);
+    const uint64_t nkeys = n * (uint64_t)E;
+    const unsigned grid = (unsigned)((n + 127) / 128);
+    DBuf<uint64_t> keys(nkeys), sorted(nkeys), uniq(nkeys);
+    DBuf<int> nsel(1);
+    k_det_closure_keys<<<grid, 128, 0, stream>>>(n, L, E, up, down, keys.p);
+    CF_HIP(hipGetLastError());
+    // the list itself must not repeat a determinant (its key sits at keys[J*E])
+    {
+        DBuf<uint64_t> lk(n), lks(n);
+        DBuf<unsigned int> dup(1);
+        CF_HIP(hipMemcpy2DAsync(lk.p, sizeof(uint64_t), keys.p, E * sizeof(uint64_t), sizeof(uint64_t), n,
+                                hipMemcpyDeviceToDevice, stream));
+        size_t tb = 0;
+        CF_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, lk.p, lks.p, (int)n, 0, 2 * L, stream));
+        DBuf<char> tmp(tb ? tb : 1);
+        CF_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, lk.p, lks.p, (int)n, 0, 2 * L, stream));
+        CF_HIP(hipMemsetAsync(dup.p, 0, sizeof(unsigned int), stream));
+        k_count_equal_neighbours<<<grid, 128, 0, stream>>>(n, lks.p, dup.p);
         CF_HIP(hipGetLastError());
+        unsigned int hd = 0;
+        CF_HIP(hipMemcpyAsync(&hd, dup.p, sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
+        CF_HIP(hipStreamSynchronize(stream));
+        if (hd) throw Error(ERR_ARG, "determinant list repeats a determinant; RDMs need distinct determinants");
     }
-    std::vector<double> hG(3 * PP);
-    CF_HIP(hipMemcpyAsync(hG.data(), G.p, 3 * PP * sizeof(double), hipMemcpyDeviceToHost, stream));
+    if (nkeys > 0x7fffffffull) throw Error(ERR_UNSUPPORTED, "determinant list too long for the RDM closure");
+    size_t tb = 0, tb2 = 0;
+    CF_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, keys.p, sorted.p, (int)nkeys, 0, 2 * L, stream));
+    CF_HIP(hipcub::DeviceSelect::Unique(nullptr, tb2, sorted.p, uniq.p, nsel.p, (int)nkeys, stream));
+    DBuf<char> tmp(std::max(tb, tb2) ? std::max(tb, tb2) : 1);
+    CF_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, keys.p, sorted.p, (int)nkeys, 0, 2 * L, stream));
+    CF_HIP(hipcub::DeviceSelect::Unique(tmp.p, tb2, sorted.p, uniq.p, nsel.p, (int)nkeys, stream));
+    int hk = 0;
+    CF_HIP(hipMemcpyAsync(&hk, nsel.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     CF_HIP(hipStreamSynchronize(stream));
-    const size_t L4 = PP;
-    for (int b = 0; b < 3; ++b) {
-        const double* Gb = hG.data() + b * PP;
-        const int s = b == 2 ? 1 : 0;
-        for (int p = 0; p < L; ++p)
-            for (int q = 0; q < L; ++q)
-                for (int r = 0; r < L; ++r)
-                    for (int t = 0; t < L; ++t) {
-                        double v = Gb[(size_t)(r * L + p) * P + (q * L + t)];
-                        if (b != 1 && q == r) v -= gamma[s * P + p * L + t];
-                        Gamma[b * L4 + (((size_t)p * L + q) * L + r) * L + t] = v;
-                    }
-    }
+    const uint64_t nk = (uint64_t)hk;
+    const size_t P = (size_t)L * L;
+    DBuf<double> c(n), cK(nk), Xa(nk * P), Xb(nk * P);
+    CF_HIP(hipMemcpyAsync(c.p, c_host, n * sizeof(double), hipMemcpyHostToDevice, stream));
+    CF_HIP(hipMemsetAsync(cK.p, 0, cK.bytes(), stream));
+    CF_HIP(hipMemsetAsync(Xa.p, 0, Xa.bytes(), stream));
+    CF_HIP(hipMemsetAsync(Xb.p, 0, Xb.bytes(), stream));
+    k_det_scatter_x<<<grid, 128, 0, stream>>>(n, L, up, down, c.p, uniq.p, nk, cK.p, Xa.p, Xb.p);
+    CF_HIP(hipGetLastError());
+    rdm_from_x(stream, L, nk, cK.p, Xa.p, Xb.p, gamma, Gamma);
 }
 
 void pack_rdm(int L, const double* gamma, const double* Gamma, double* one, double* two) {

@@ -114,7 +114,8 @@ int cf_read_determinants(const char* path, uint64_t cap, uint64_t* up, uint64_t*
 /* Space of n determinants (alpha string up[i], beta string down[i], in list
  * order; each with ham's nup/ndown electrons in its L orbitals).  cf_crs_build
  * then builds the reference's FCI_File matrix (rows in list order, full rows
- * here); dims report (n, n, 1).  RDMs are not available for these spaces. */
+ * here); dims report (n, n, 1).  cf_rdm / cf_rdm_dense give DensFILE's RDMs
+ * (CIDens.cpp:1468-1526) for L <= 32 and distinct determinants. */
 int cf_space_create_dets(cf_ham* ham, uint64_t n, const uint64_t* up, const uint64_t* down, cf_space** out);
 /* Unrank strings [first, first+count) of the n-of-L space on the GPU (host out). */
 int cf_strings_unrank(cf_ctx* ctx, int L, int nel, uint64_t first, uint64_t count, uint64_t* out);

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import ciflow_amd as CA
+from oracle import bruteforce as BF
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -176,11 +177,47 @@ def test_det_list_rejects_bad_input():
         CA.FCI_File(ham, (np.array([7], dtype=np.uint64), np.array([3], dtype=np.uint64)))
     with pytest.raises(CA.CIFlowError):  # orbital 6 outside L = 6
         CA.FCI_File(ham, (np.array([7 << 4], dtype=np.uint64), np.array([7], dtype=np.uint64)))
-    cim = CA.FCI_File(ham, (np.array([7, 11], dtype=np.uint64), np.array([7, 7], dtype=np.uint64)))
-    assert cim.get_mat().info()[0] == 2
-    cim._eigvec = np.array([0.6, 0.8])
-    with pytest.raises(CA.CIFlowError):  # no RDMs for determinant lists
+    cim = CA.FCI_File(ham, (np.array([7, 11, 7], dtype=np.uint64), np.array([7, 7, 7], dtype=np.uint64)))
+    assert cim.get_mat().info()[0] == 3
+    cim._eigvec = np.array([0.6, 0.8, 0.0])
+    with pytest.raises(CA.CIFlowError):  # a repeated determinant has no RDM
         cim.construct_density()
+
+
+def test_det_file_rdms_match_fci_rdms():
+    """TestDensity.cpp: DensFILE on the full BeH2 list equals DensFCI, and the
+    density energy equals the CI energy."""
+    h, V, na, nb, ec = fixture("beh2_sto3g.npz")
+    ham = CA.Hamiltonian(h, V, na, nb, ec)
+    f = CA.FCI_File(ham, os.path.join(GOLD, "fci_beh2_sto-3gdeterminants.dat"))
+    f.solve()
+    one_f, two_f = f.construct_density()
+    fci = CA.FCI(ham)
+    fci.solve()
+    one, two = fci.construct_density()
+    _rdm_compare(h.shape[0], one_f, two_f, one, two)
+    assert abs(f.get_dens_energy() - f.get_ci_energy()) < 1e-10
+
+
+@pytest.mark.parametrize("L,na,nb,count,seed", [(6, 3, 2, 150, 4), (7, 2, 2, 200, 5), (5, 2, 2, 100, 6)])
+def test_det_list_rdms_match_second_quantisation(L, na, nb, count, seed):
+    """Shuffled subsets of a space (the whole space for L=5 2a2b): the dense RDMs
+    of the determinant-list path equal <a+a> and <a+a+aa> evaluated with
+    explicit operators on the same vector."""
+    h, V = CA.gen_integrals(L, seed)
+    A = O.enumerate_strings(L, na)
+    B = O.enumerate_strings(L, nb)
+    pick = np.random.default_rng(seed).permutation(len(A) * len(B))[:count]
+    up = A[pick // len(B)]
+    down = B[pick % len(B)]
+    cim = CA.FCI_File(CA.Hamiltonian(h, V, na, nb), (up, down))
+    cim.solve()
+    c = cim.get_eigvec()
+    g, G = cim.density_dense(c)
+    gb, Gb = BF.rdms(c, L, na, nb, dets=[(int(a), int(b)) for a, b in zip(up, down)])
+    assert np.max(np.abs(g - gb)) < 1e-12
+    assert np.max(np.abs(G - Gb)) < 1e-12
+
 
 
 # ---- (c) SpMV ----------------------------------------------------------------
