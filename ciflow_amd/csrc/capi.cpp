@@ -1488,10 +1488,10 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
         S->y32.alloc(sp->n);
     }
     S->blas = c->blas();
-    // measured on MI355X: 512 beta strings per workgroup up to L = 14 (d_beta <= 3432),
-    // 768 above (L = 16: 771 ms against 1458 ms at 1024); small beta spaces use the
-    // smallest workgroup that holds them in one chunk
-    S->shape.threads = sp->db <= 128 ? 128 : sp->db <= 256 ? 256 : sp->db <= 4096 ? 512 : 768;
+    // measured on MI355X: 256 beta strings per workgroup up to d_beta = 512 (L = 11:
+    // 176 us against 187 us at 512), 512 up to L = 14 (d_beta <= 3432), 768 above
+    // (L = 16: 771 ms against 1458 ms at 1024); d_beta <= 128 in one 128-thread chunk
+    S->shape.threads = sp->db <= 128 ? 128 : sp->db <= 512 ? 256 : sp->db <= 4096 ? 512 : 768;
     SigmaArgs& a = S->args;
     a.da = sp->da;
     a.db = sp->db;

@@ -435,9 +435,13 @@ void launch_sigma_ab2(const SigmaArgs& a0, SigmaShape sh, const CT* C, double* s
         switch (NTr) {
             case 128:
                 if (pt <= 1) ok = launch_ab2_pv<CT, 128, 1>(a, grid, C, sig, s);
+                else if (pt <= 2) ok = launch_ab2_pv<CT, 128, 2>(a, grid, C, sig, s);
+                else if (pt <= 4) ok = launch_ab2_pv<CT, 128, 4>(a, grid, C, sig, s);
                 break;
             case 256:
                 if (pt <= 1) ok = launch_ab2_pv<CT, 256, 1>(a, grid, C, sig, s);
+                else if (pt <= 2) ok = launch_ab2_pv<CT, 256, 2>(a, grid, C, sig, s);
+                else if (pt <= 4) ok = launch_ab2_pv<CT, 256, 4>(a, grid, C, sig, s);
                 break;
             case 512:
                 if (pt <= 1) ok = launch_ab2_pv<CT, 512, 1>(a, grid, C, sig, s);
