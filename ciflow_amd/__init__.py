@@ -142,6 +142,8 @@ def lib():
         "cf_space_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
         "cf_space_create_dets": (C.c_int, [vp, C.c_uint64, u64p, u64p, C.POINTER(vp)]),
         "cf_read_determinants": (C.c_int, [C.c_char_p, C.c_uint64, u64p, u64p, C.POINTER(C.c_uint64)]),
+        "cf_read_integrals": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_double), dp, dp, C.POINTER(C.c_int)]),
         "cf_space_destroy": (C.c_int, [vp]),
         "cf_space_dims": (C.c_int, [vp, u64p, u64p, u64p]),
         "cf_strings_unrank": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, u64p]),
@@ -330,6 +332,14 @@ class Hamiltonian:
         hd = _Handle("cf_ham_destroy", self.ctx.h)
         _check(lib().cf_ham_rotate(self.ptr, _d(U), C.byref(hd.ptr)))
         return Hamiltonian._adopt(hd, self.L, self.nup, self.ndown, self.econst, self.ctx)
+
+    @classmethod
+    def from_file(cls, path: str, ctx: Context | None = None) -> "Hamiltonian":
+        """Hamiltonian(filename) of the reference: read a psi4 integral dump."""
+        d = read_integrals(path)
+        ham = cls(d["h"], d["V"], d["nalpha"], d["nbeta"], d["econst"], ctx)
+        ham.irreps = d["irreps"]
+        return ham
 
     @classmethod
     def synthetic(cls, L: int, nup: int, ndown: int, seed: int, ctx: Context | None = None):
@@ -594,6 +604,23 @@ class DOCI(CIMethod):
 
     def get_name(self):
         return "DOCI"
+
+
+def read_integrals(path: str) -> dict:
+    """Hamiltonian::read_file (Hamiltonian.cpp:707-885): the psi4 integral dump
+    as dense h[L,L], physics V[L,L,L,L], econst, nalpha, nbeta and the orbital
+    irreps."""
+    L, na, nb, ec = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+    _check(lib().cf_read_integrals(path.encode(), C.byref(L), C.byref(na), C.byref(nb), C.byref(ec), None, None,
+                                   None))
+    n = L.value
+    h = np.zeros((n, n))
+    V = np.zeros((n, n, n, n))
+    ir = (C.c_int * n)()
+    _check(lib().cf_read_integrals(path.encode(), C.byref(L), C.byref(na), C.byref(nb), C.byref(ec), _d(h), _d(V),
+                                   ir))
+    return {"L": n, "nalpha": na.value, "nbeta": nb.value, "econst": ec.value, "h": h, "V": V,
+            "irreps": np.array(list(ir), dtype=int)}
 
 
 def read_determinants(path: str):

@@ -8,6 +8,7 @@
 #include <functional>
 #include <map>
 #include <fstream>
+#include <sstream>
 #include <memory>
 #include <string>
 #include <algorithm>
@@ -823,6 +824,83 @@ int cf_space_create(cf_ham* ham, int kind, cf_space** out) {
     }
     CF_HIP(hipStreamSynchronize(c->stream));
     *out = sp.release();
+    API_END
+}
+
+int cf_read_integrals(const char* path, int* L, int* nup, int* ndown, double* econst, double* h, double* V,
+                      int* irreps) {
+    API_BEGIN
+    CF_REQUIRE(path && L && nup && ndown && econst, "NULL argument");
+    std::ifstream f(path);
+    if (!f.is_open()) throw Error(ERR_ARG, std::string("cannot open ") + path);
+    std::string line;
+    auto next = [&](const char* what) {
+        if (!std::getline(f, line)) throw Error(ERR_ARG, std::string("integral file ends before ") + what);
+        while (!line.empty() && line.back() == '\r') line.pop_back();
+    };
+    auto value_of = [&](const char* what) -> std::string {
+        next(what);
+        const size_t eq = line.find('=');
+        if (eq == std::string::npos) throw Error(ERR_ARG, std::string("integral file: no '=' on the ") + what + " line");
+        return line.substr(eq + 1);
+    };
+    do next("the \"****  Molecular Integrals\" header");
+    while (line.find("****  Molecular Integrals") == std::string::npos);
+    const int na = std::atoi(value_of("Nalpha").c_str());
+    const int nb = std::atoi(value_of("Nbeta").c_str());
+    value_of("Symmetry Label");
+    value_of("Nirreps");
+    const double ec = std::strtod(value_of("Nuclear Repulsion Energy").c_str(), nullptr);
+    const int nL = std::atoi(value_of("Number Of Molecular Orbitals").c_str());
+    CF_REQUIRE(nL >= 1 && nL <= kMaxL, "integral file: number of orbitals out of range");
+    next("the irrep header");
+    next("the orbital irreps");
+    {
+        std::istringstream is(line);
+        for (int i = 0; i < nL; ++i) {
+            int ir = 0;
+            if (!(is >> ir)) throw Error(ERR_ARG, "integral file: fewer orbital irreps than orbitals");
+            if (irreps) irreps[i] = ir;
+        }
+    }
+    *L = nL;
+    *nup = na;
+    *ndown = nb;
+    *econst = ec;
+    if (!h && !V) return CF_OK;
+    CF_REQUIRE(h && V, "h and V must both be given");
+    const size_t L2 = (size_t)nL * nL, L4 = L2 * L2;
+    std::fill(h, h + L2, 0.0);
+    std::fill(V, V + L4, 0.0);
+    do next("the one-electron integrals");
+    while (line.rfind("**", 0) != 0);  // lines up to the first starting with "**"
+    auto bad = [&](int x) { return x < 0 || x >= nL; };
+    for (;;) {  // one-electron lines up to the next line starting with '*'
+        next("the two-electron integrals");
+        if (line.empty()) continue;
+        if (line[0] == '*') break;
+        std::istringstream is(line);
+        int i, j;
+        double v;
+        if (!(is >> i >> j >> v) || bad(i) || bad(j)) throw Error(ERR_ARG, "integral file: bad one-electron line: " + line);
+        h[(size_t)i * nL + j] = v;
+        h[(size_t)j * nL + i] = v;
+    }
+    auto at = [&](int p, int q, int r, int t) -> double& { return V[(((size_t)p * nL + q) * nL + r) * nL + t]; };
+    for (;;) {  // two-electron lines up to the next line starting with '*'
+        next("the end of the two-electron integrals");
+        if (line.empty()) continue;
+        if (line[0] == '*') break;
+        std::istringstream is(line);
+        int a, b, c, d;
+        double v;
+        if (!(is >> a >> b >> c >> d >> v) || bad(a) || bad(b) || bad(c) || bad(d))
+            throw Error(ERR_ARG, "integral file: bad two-electron line: " + line);
+        // chemist (ab|cd) and its 8-fold symmetry, stored as physics <ac|bd>
+        const int P[8][4] = {{a, b, c, d}, {b, a, c, d}, {a, b, d, c}, {b, a, d, c},
+                             {c, d, a, b}, {d, c, a, b}, {c, d, b, a}, {d, c, b, a}};
+        for (const auto& x : P) at(x[0], x[2], x[1], x[3]) = v;
+    }
     API_END
 }
 

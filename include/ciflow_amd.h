@@ -104,6 +104,15 @@ int cf_gen_rotation(int L, uint64_t seed, double sigma, double* K);
 int cf_space_create(cf_ham* ham, int kind, cf_space** out);
 int cf_space_destroy(cf_space* sp);
 int cf_space_dims(const cf_space* sp, uint64_t* n, uint64_t* dim_up, uint64_t* dim_down);
+/* ---- integral files: Hamiltonian::read_file (Hamiltonian.cpp:707-885), the
+ *      psi4 dump the reference reads ("****  Molecular Integrals" header with
+ *      Nalpha, Nbeta, symmetry, nuclear repulsion, L and the orbital irreps,
+ *      then "i j value" one-electron and "i j k l value" chemist-notation
+ *      two-electron lines).  Host only.  Call with h = V = irreps = NULL to
+ *      learn L, then with h[L*L], V[L^4] (physics, V[((i*L+j)*L+k)*L+l] =
+ *      getVmat(i,j,k,l), all eight permutations filled) and irreps[L]. */
+int cf_read_integrals(const char* path, int* L, int* nup, int* ndown, double* econst, double* h, double* V,
+                      int* irreps);
 /* ---- selected CI over a determinant list: Permutator_File::read_file
  *      (Permutator.cpp:179-219) and FCI_File (CIMethod.cpp:890-917).
  * Reads the "UP|DOWN" lines of a determinant file (binary strings, highest

@@ -11,7 +11,8 @@ nbeta and the orbital irreps.
 
 Run from the repo root:  python tests/golden/make_fixtures.py
 (needs /root/reference; the generated .npz files are committed).  The two
-determinant lists of the reference's selected-CI tests are copied unchanged.
+determinant lists of the reference's selected-CI tests and the BeH2 dump are
+copied unchanged.
 """
 from __future__ import annotations
 
@@ -67,9 +68,9 @@ def main():
         d = read_dump(os.path.join(REF, src))
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d)
         print(name, d["L"], d["nalpha"], d["nbeta"], d["econst"])
-    # determinant lists the reference's selected-CI tests read (Permutator_File),
-    # kept byte for byte as data
-    for src in ("GKCI_Be_cntSp_cc-pVDZ_0100.txt", "fci_beh2_sto-3gdeterminants.dat"):
+    # determinant lists the reference's selected-CI tests read (Permutator_File)
+    # and the BeH2 integral dump (for the library's own file reader), byte for byte
+    for src in ("GKCI_Be_cntSp_cc-pVDZ_0100.txt", "fci_beh2_sto-3gdeterminants.dat", "beh2_sto3g_1_5.dat"):
         shutil.copyfile(os.path.join(REF, src), os.path.join(OUT, src))
         print(src)
     return 0

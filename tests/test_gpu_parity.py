@@ -113,6 +113,18 @@ def test_config1_digest_matches_golden():
     assert (n, nnz_up, nnz_full) == (4900, 886900, 1768900)
 
 
+def test_kat_energies_from_the_integral_file():
+    """Hamiltonian(filename): the reference's BeH2 dump read by the library,
+    DOCI KAT -15.225362047972723 (TestDOCI.cpp:31) and the FCI energy."""
+    ham = CA.Hamiltonian.from_file(os.path.join(GOLD, "beh2_sto3g_1_5.dat"))
+    d = CA.DOCI(ham)
+    d.solve()
+    assert abs(d.get_ci_energy() - (-15.225362047972723)) < 1e-10
+    f = CA.FCI(ham)
+    f.solve()
+    assert abs(f.get_ci_energy() - golden()["beh2"]["fci_energy"]) < 1e-10
+
+
 # ---- selected CI over determinant lists (FCI_File) ----------------------------
 DET_FILES = {"be_gkci100": ("be_ccpvdz.npz", "GKCI_Be_cntSp_cc-pVDZ_0100.txt"),
              "beh2_full_list": ("beh2_sto3g.npz", "fci_beh2_sto-3gdeterminants.dat")}

@@ -150,3 +150,21 @@ def test_determinant_file_parser_matches_oracle(tmp_path):
     p.write_text("1" * 65 + "|" + "1" * 65 + "\n")
     with pytest.raises(CA.CIFlowError):
         CA.read_determinants(str(p))
+
+
+def test_integral_file_reader_matches_fixture(tmp_path):
+    """Hamiltonian::read_file on the reference's BeH2 dump gives, bit for bit,
+    the arrays of the committed fixture (converted by an independent reader)."""
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    d = CA.read_integrals(os.path.join(gold, "beh2_sto3g_1_5.dat"))
+    f = np.load(os.path.join(gold, "beh2_sto3g.npz"))
+    assert (d["L"], d["nalpha"], d["nbeta"]) == (int(f["L"]), int(f["nalpha"]), int(f["nbeta"]))
+    assert d["econst"] == float(f["econst"])
+    assert np.array_equal(d["h"], f["h"]) and np.array_equal(d["V"], f["V"])
+    assert np.array_equal(d["irreps"], f["irreps"])
+    with pytest.raises(CA.CIFlowError):
+        CA.read_integrals(str(tmp_path / "missing.dat"))
+    p = tmp_path / "cut.dat"
+    p.write_text("header\n****  Molecular Integrals For DOCI Start Here\nNalpha = 1\n")
+    with pytest.raises(CA.CIFlowError):
+        CA.read_integrals(str(p))
