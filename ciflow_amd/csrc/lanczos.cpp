@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <utility>
 
 namespace cf {
 
@@ -11,18 +12,27 @@ namespace cf {
 // algorithm with Wilkinson-type shifts (the classical symmetric eigensolver,
 // e.g. Golub & Van Loan, Matrix Computations, 4th ed., sections 8.3.1 and
 // 8.3.3).  O(m^3) with a small constant; the Krylov matrices here have m <= 64.
-void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& Z) {
-    w.assign(m, 0.0);
-    std::vector<double> e(m, 0.0);
+namespace {
+
+// A := Q^T A Q tridiagonal, reducing rows m-1 .. 1 in turn, so index m-1 is
+// never touched by a reflection (row m-1 of Q is e_{m-1}).  On return d holds
+// h of each reflection (0 when the row needed none), e the sub-diagonal
+// (e[i] couples i-1 and i) and A the reflectors, ready for accumulation.
+void householder(int m, std::vector<double>& A, std::vector<double>& d, std::vector<double>& e) {
+    d.assign(m, 0.0);
+    e.assign(m, 0.0);
     auto a = [&](int i, int j) -> double& { return A[(size_t)i * m + j]; };
-    // Householder: A := Q^T A Q tridiagonal, Q accumulated into A at the end
     for (int i = m - 1; i > 0; --i) {
         const int l = i - 1;
         double h = 0.0;
         if (l > 0) {
-            double scale = 0.0;
-            for (int k = 0; k <= l; ++k) scale += std::fabs(a(i, k));
-            if (scale == 0.0) {
+            // a row that is already tridiagonal needs no reflection: the
+            // Krylov matrices are tridiagonal apart from the arrow block a
+            // thick restart leaves, so this skips most rows
+            double below = 0.0;
+            for (int k = 0; k < l; ++k) below += std::fabs(a(i, k));
+            const double scale = below + std::fabs(a(i, l));
+            if (below == 0.0) {
                 e[i] = a(i, l);
             } else {
                 for (int k = 0; k <= l; ++k) {
@@ -54,24 +64,16 @@ void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<d
         } else {
             e[i] = a(i, l);
         }
-        w[i] = h;
+        d[i] = h;
     }
-    w[0] = 0.0;
     e[0] = 0.0;
-    for (int i = 0; i < m; ++i) {
-        const int l = i - 1;
-        if (w[i] != 0.0) {
-            for (int j = 0; j <= l; ++j) {
-                double g = 0.0;
-                for (int k = 0; k <= l; ++k) g += a(i, k) * a(k, j);
-                for (int k = 0; k <= l; ++k) a(k, j) -= g * a(k, i);
-            }
-        }
-        w[i] = a(i, i);
-        a(i, i) = 1.0;
-        for (int j = 0; j <= l; ++j) a(j, i) = a(i, j) = 0.0;
-    }
-    // implicit QL on (w, e), rotations applied to the columns of A
+}
+
+// Implicit QL on the tridiagonal (w diagonal, e sub-diagonal with e[i]
+// coupling i-1 and i).  Every plane rotation of columns (i, i+1) is passed to
+// rot(i, s, c) for the caller to apply to its vectors.
+template <class Rot>
+void implicit_ql(int m, std::vector<double>& w, std::vector<double>& e, Rot rot) {
     for (int i = 1; i < m; ++i) e[i - 1] = e[i];
     if (m > 0) e[m - 1] = 0.0;
     for (int l = 0; l < m; ++l) {
@@ -85,14 +87,16 @@ void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<d
             if (mm == l) break;
             if (++iter > 60) break;  // not reached for symmetric input
             double g = (w[l + 1] - w[l]) / (2.0 * e[l]);
-            double r = std::hypot(g, 1.0);
+            // plain square roots: the entries are bounded by the operator
+            // norm, far from overflow, and std::hypot costs several times more
+            double r = std::sqrt(g * g + 1.0);
             g = w[mm] - w[l] + e[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
             double s = 1.0, c = 1.0, p = 0.0;
             int i = mm - 1;
             for (; i >= l; --i) {
-                double f = s * e[i];
+                const double f = s * e[i];
                 const double b = c * e[i];
-                r = std::hypot(f, g);
+                r = std::sqrt(f * f + g * g);
                 e[i + 1] = r;
                 if (r == 0.0) {
                     w[i + 1] -= p;
@@ -106,11 +110,7 @@ void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<d
                 p = s * r;
                 w[i + 1] = g + p;
                 g = c * r - b;
-                for (int k = 0; k < m; ++k) {
-                    f = a(k, i + 1);
-                    a(k, i + 1) = s * a(k, i) + c * f;
-                    a(k, i) = c * a(k, i) - s * f;
-                }
+                rot(i, s, c);
             }
             if (r == 0.0 && i >= l) continue;
             w[l] -= p;
@@ -118,15 +118,71 @@ void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<d
             e[mm] = 0.0;
         }
     }
-    // ascending order, vectors as columns of Z
-    std::vector<int> idx(m);
-    for (int i = 0; i < m; ++i) idx[i] = i;
+}
+
+std::vector<int> ascending(const std::vector<double>& w) {
+    std::vector<int> idx(w.size());
+    for (size_t i = 0; i < w.size(); ++i) idx[i] = (int)i;
     std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return w[x] < w[y]; });
+    return idx;
+}
+
+}  // namespace
+
+void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& Z) {
+    std::vector<double> e;
+    householder(m, A, w, e);
+    auto a = [&](int i, int j) -> double& { return A[(size_t)i * m + j]; };
+    // accumulate Q into A, leaving the diagonal in w
+    for (int i = 0; i < m; ++i) {
+        const int l = i - 1;
+        if (w[i] != 0.0) {
+            for (int j = 0; j <= l; ++j) {
+                double g = 0.0;
+                for (int k = 0; k <= l; ++k) g += a(i, k) * a(k, j);
+                for (int k = 0; k <= l; ++k) a(k, j) -= g * a(k, i);
+            }
+        }
+        w[i] = a(i, i);
+        a(i, i) = 1.0;
+        for (int j = 0; j <= l; ++j) a(j, i) = a(i, j) = 0.0;
+    }
+    implicit_ql(m, w, e, [&](int i, double s, double c) {
+        for (int k = 0; k < m; ++k) {
+            const double f = a(k, i + 1);
+            a(k, i + 1) = s * a(k, i) + c * f;
+            a(k, i) = c * a(k, i) - s * f;
+        }
+    });
+    // ascending order, vectors as columns of Z
+    const std::vector<int> idx = ascending(w);
     std::vector<double> ws(m);
     Z.assign((size_t)m * m, 0.0);
     for (int c = 0; c < m; ++c) {
         ws[c] = w[idx[c]];
         for (int r = 0; r < m; ++r) Z[(size_t)r * m + c] = a(r, idx[c]);
+    }
+    w.swap(ws);
+}
+
+void sym_eig_last_row(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& zlast) {
+    std::vector<double> e;
+    householder(m, A, w, e);
+    for (int i = 0; i < m; ++i) w[i] = A[(size_t)i * m + i];
+    // row m-1 of Q is e_{m-1}; only that row of Q R is followed through QL
+    std::vector<double> z(m, 0.0);
+    if (m > 0) z[m - 1] = 1.0;
+    implicit_ql(m, w, e, [&](int i, double s, double c) {
+        const double f = z[i + 1];
+        z[i + 1] = s * z[i] + c * f;
+        z[i] = c * z[i] - s * f;
+    });
+    const std::vector<int> idx = ascending(w);
+    std::vector<double> ws(m);
+    zlast.assign(m, 0.0);
+    for (int c = 0; c < m; ++c) {
+        ws[c] = w[idx[c]];
+        zlast[c] = z[idx[c]];
     }
     w.swap(ws);
 }
@@ -252,10 +308,14 @@ bool ThickRestartLanczos::converged_early(int m) {
     std::vector<double> A((size_t)m * m);
     for (int a = 0; a < m; ++a)
         for (int c = 0; c < m; ++c) A[(size_t)a * m + c] = T_[(size_t)a * M_ + c];
-    std::vector<double> th, Y;
-    sym_eig(m, A, th, Y);
-    const double res = std::fabs(beta_last_ * Y[(size_t)(m - 1) * m + 0]);
+    // eigenvalues and the last row of the eigenvectors decide; the full
+    // decomposition is formed only once the test passes
+    std::vector<double> th, zlast;
+    sym_eig_last_row(m, A, th, zlast);
+    const double res = std::fabs(beta_last_ * zlast[0]);
     if (!accept(res, th)) return false;
+    std::vector<double> Y;
+    sym_eig(m, std::move(A), th, Y);
     m_ = m;
     status_ = 0;
     b_->rotate(m, 1, Y.data());

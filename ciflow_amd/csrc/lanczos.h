@@ -45,8 +45,12 @@ struct KrylovBackend {
     virtual void fetch_coeffs(int /*j0*/, int /*j1*/, double* /*alpha*/, double* /*beta*/) {}
 };
 
-// Symmetric eigen-decomposition (cyclic Jacobi), ascending; Z columns = vectors.
+// Symmetric eigen-decomposition (Householder + implicit QL), ascending;
+// Z columns = vectors.
 void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& Z);
+// The same eigenvalues with only the last row of Z (zlast[c] = Z[m-1][c]),
+// O(m^2) when A is tridiagonal apart from a leading arrow block.
+void sym_eig_last_row(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& zlast);
 
 class ThickRestartLanczos {
 public:
