@@ -205,7 +205,8 @@ def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
     kern, algo = m.spmv_bytes(CA.CF_F64)
     ach = algo / (avg_ms * 1e-3) / 1e9
     names = {1: "k_spmv (CRS row gather)", 3: "k_spmv_v4 (CRS row gather, vectorised streams)",
-             4: "k_spmv_ell2 (FCI position-major complete rows, double-buffered LDS x blocks, FP64 values)"}
+             4: "k_spmv_ell2 (FCI position-major complete rows, double-buffered LDS x blocks, FP64 values)",
+             5: "k_spmv_doci (DOCI position-major pair-hop columns, pair elements in LDS)"}
     crs_bytes = nnz_full * 12 + (n + 1) * 8 + 16 * n
     out = {"bound": "hbm", "kernel": names.get(kern, str(kern)), "crs_equivalent_bytes": crs_bytes,
            "crs_equivalent_GBps": crs_bytes / (avg_ms * 1e-3) / 1e9,
@@ -360,6 +361,7 @@ def bench_fci12(a, CA):
     nl32, ms32 = ctx.kernel_time("spmv_f32")
     ncount, ms_count = ctx.kernel_time("crs_count")
     nfill, ms_fill = ctx.kernel_time("crs_fill")
+    nhop, ms_hop = ctx.kernel_time("doci_ell_fill")
     if ms32 > ms64:
         fam, nl, ms, sval = "spmv_f32", nl32, ms32, 4
     else:
@@ -402,7 +404,8 @@ def bench_fci12(a, CA):
         "n_matvec": infos[-1]["n_matvec"],
         "n_matvec_f32": infos[-1]["n_matvec_f32"],
         "resid": infos[-1]["resid"],
-        "build_ms": {"count": ms_count / max(ncount, 1), "fill": ms_fill / max(nfill, 1)},
+        "build_ms": {"count": ms_count / max(ncount, 1), "fill": ms_fill / max(nfill, 1),
+                     "doci_hop_columns": ms_hop / max(nhop, 1) if nhop else None},
         "rdm_ms": rdm_ms,
         "rdm_trace": tr,
         "roofline": {"bound": "hbm", "kernel": f"{fam} (kernel {kern})", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -251,17 +251,6 @@ __device__ __forceinline__ int select_bit(str_t v, int m) {  // position of the 
     return __builtin_ctzll(v);
 }
 
-__device__ __forceinline__ uint32_t rank_str(str_t s, const uint64_t* binom) {
-    uint32_t r = 0;
-    int m = 1;
-    while (s) {
-        r += (uint32_t)binom[__builtin_ctzll(s) * kBinomCols + m];
-        ++m;
-        s &= s - 1;
-    }
-    return r;
-}
-
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_doci_rows(DociRowsArgs a) {
     const int lane = threadIdx.x & 63;
@@ -311,7 +300,7 @@ __global__ void __launch_bounds__(256) k_doci_rows(DociRowsArgs a) {
                     }
                 }
                 const str_t tgt = s ^ ((str_t)1 << j) ^ ((str_t)1 << k);
-                col = rank_str(tgt, a.binom);
+                col = rank_bits(tgt, a.binom);
                 const size_t lo = j < k ? j : k, hi = j < k ? k : j;
                 val = a.H.V[((lo * L + lo) * L + hi) * L + hi];
             }

@@ -18,6 +18,7 @@
 #include "lanczos.h"
 #include "rdm.h"
 #include "sigma.h"
+#include "spmv_doci.h"
 #include "spmv_ell.h"
 #include "spmv_fci.h"
 
@@ -207,18 +208,20 @@ struct SpmvChoice {
     int lanes = 0;
     int unroll = 0;
     EllShape ell;
+    DociShape doci;
 };
 
 struct cf_crs {
     cf_space* sp = nullptr;
     uint64_t n = 0, nnz = 0;
     int64_t nnz_upper = -1;
+    double zero = 0.0;  // build threshold: elements with |v| <= zero are not stored
     DBuf<int64_t> rowptr;
     DBuf<uint32_t> col;
     DBuf<double> val;
     DBuf<float> val32;
     int lpr = 64;     // lanes per row of the scalar row gather
-    int kernel = 0;   // 0 auto, 1 row gather, 2 FCI staged, 3 vectorised row gather, 4 FCI ELL
+    int kernel = 0;   // 0 auto, 1 row gather, 2 FCI staged, 3 vectorised row gather, 4 FCI ELL, 5 DOCI hops
     // automatic choice per value precision (row-length heuristic, or the autotuner's pick)
     SpmvChoice auto64, auto32;
     // FCI position-major complete rows (kernel 4), built on demand
@@ -230,6 +233,12 @@ struct cf_crs {
     DBuf<uint2> ent;
     EllArgs eargs;
     bool ell_ready = false;
+    // DOCI position-major hop columns + pair tables (kernel 5)
+    DBuf<uint32_t> dcol;
+    DBuf<double> dvk;
+    DBuf<float> dvk32;
+    DociEllArgs dargs;
+    bool doci_ready = false;
     std::unique_ptr<FciSpmvPlan> plan;
 };
 
@@ -272,6 +281,7 @@ SpmvChoice resolve_spmv(const cf_crs* m, int prec) {
     if (m->kernel) r.kernel = m->kernel;
     if (r.kernel == 2 && !m->plan) r.kernel = 1;
     if (r.kernel == 4 && !m->ell_ready) r.kernel = 1;
+    if (r.kernel == 5 && !m->doci_ready) r.kernel = 1;
     if (r.kernel == 1) r.lanes = a.kernel == 1 && a.lanes && !m->kernel ? a.lanes : m->lpr;
     if (r.kernel == 3 && a.kernel != 3) {
         r.lanes = 16;
@@ -286,6 +296,7 @@ void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, doub
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
         switch (ch.kernel) {
+            case 5: launch_spmv_doci_f32(m->dargs, ch.doci, m->dvk32.p, dx, dy, c->stream); break;
             case 4: launch_spmv_ell_f32(m->eargs, ch.ell, m->ell32.p, dx, dy, c->stream); break;
             case 2: launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream); break;
             case 3:
@@ -296,6 +307,7 @@ void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, doub
     } else {
         KTimer t(c, "spmv_f64");
         switch (ch.kernel) {
+            case 5: launch_spmv_doci_f64(m->dargs, ch.doci, m->dvk.p, dx, dy, c->stream); break;
             case 4: launch_spmv_ell_f64(m->eargs, ch.ell, m->ell.p, dx, dy, c->stream); break;
             case 2: launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream); break;
             case 3:
@@ -307,6 +319,32 @@ void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, doub
 }
 
 void spmv(cf_crs* m, int prec, const double* dx, double* dy) { spmv_with(m, prec, resolve_spmv(m, prec), dx, dy); }
+
+// Position-major DOCI hop columns + pair tables (kernel 5).
+void make_doci_ell(cf_crs* m) {
+    cf_space* sp = m->sp;
+    cf_ham* H = sp->ham;
+    cf_ctx* c = H->ctx;
+    CF_REQUIRE(sp->kind == CF_DOCI, "the DOCI hop SpMV needs a DOCI matrix");
+    DociEllArgs& a = m->dargs;
+    a.n = m->n;
+    a.L = H->L;
+    a.np = H->nup;
+    a.nv = H->L - H->nup;
+    a.s = sp->ta->s.p;
+    a.diag = sp->diag.p;
+    a.zero = m->zero;
+    m->dcol.alloc(std::max<uint64_t>(1, m->n * (uint64_t)a.np * (uint64_t)a.nv));
+    m->dvk.alloc((uint64_t)a.L * a.L);
+    m->dvk32.alloc((uint64_t)a.L * a.L);
+    a.col = m->dcol.p;
+    {
+        KTimer t(c, "doci_ell_fill");
+        launch_doci_ell_fill(a, c->binom.p, m->dcol.p, c->stream);
+        launch_doci_pair_table(a, H->dev(), m->dvk.p, m->dvk32.p, c->stream);
+    }
+    m->doci_ready = true;
+}
 
 // Position-major complete FCI rows + beta index tables (kernel 4).
 void make_ell(cf_crs* m) {
@@ -622,8 +660,14 @@ void check_shape(const cf_crs* m, const SpmvChoice& ch) {
         else CF_REQUIRE(t % 64 == 0 && t >= 64 && t <= 1024, "kernel 4: workgroup size must be a multiple of 64 <= 1024");
         CF_REQUIRE(v == 0 || m->sp->db % v == 0, "kernel 4: d_beta must be a multiple of the strings per thread");
         CF_REQUIRE(v == 0 || (m->sp->db + t - 1) / t <= 16, "kernel 4: workgroup too small for d_beta");
+    } else if (ch.kernel == 5) {
+        CF_REQUIRE(m->doci_ready, "kernel 5 needs a DOCI matrix");
+        const int t = ch.doci.threads;
+        CF_REQUIRE(t % 64 == 0 && t >= 64 && t <= 1024, "kernel 5: workgroup size must be a multiple of 64 <= 1024");
+        const int u = ch.doci.sub;
+        CF_REQUIRE(u == 1 || u == 2 || u == 4 || u == 8, "kernel 5: lanes per row must be 1, 2, 4 or 8");
     } else {
-        CF_REQUIRE(false, "shape kernel must be 1, 3 or 4");
+        CF_REQUIRE(false, "shape kernel must be 1, 3, 4 or 5");
     }
 }
 
@@ -633,6 +677,9 @@ SpmvChoice make_choice(int kernel, int p1, int p2, uint64_t db, int prec) {
     if (kernel == 4) {
         ch.ell.threads = p1 > 0 ? p1 : ell_default_shape(db, prec == CF_F32 ? 4 : 8).threads;
         ch.ell.vec = p2;
+    } else if (kernel == 5) {
+        ch.doci.threads = p1 > 0 ? p1 : DociShape{}.threads;
+        ch.doci.sub = p2;
     } else {
         ch.lanes = p1;
         ch.unroll = p2;
@@ -1018,6 +1065,7 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     std::unique_ptr<cf_crs> m(new cf_crs());
     m->sp = sp;
     m->n = sp->n;
+    m->zero = zero;
     CF_REQUIRE(m->n < (1ull << 32), "matrix dimension must be < 2^32 (u32 columns)");
     DBuf<uint32_t> cnt(m->n);
     m->rowptr.alloc(m->n + 1);
@@ -1111,17 +1159,29 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     const double avg = m->n ? (double)m->nnz / (double)m->n : 0.0;
     m->auto64.kernel = m->auto32.kernel = 1;
     m->auto64.lanes = m->auto32.lanes = m->lpr;
-    if (avg >= 512) m->auto64 = SpmvChoice{3, 16, 2, {}};
-    if (avg >= 128) m->auto32 = SpmvChoice{3, 8, 2, {}};
-    else if (avg >= 24) m->auto32 = SpmvChoice{3, 16, 1, {}};
+    if (avg >= 512) m->auto64 = SpmvChoice{3, 16, 2, {}, {}};
+    if (avg >= 128) m->auto32 = SpmvChoice{3, 8, 2, {}, {}};
+    else if (avg >= 24) m->auto32 = SpmvChoice{3, 16, 1, {}, {}};
     // FCI matrices with enough alpha strings to fill the GPU also get the
     // position-major copy (kernel 4): measured 2.38 ms against 5.15 ms (FP64) and
     // 1.53 ms against 3.45 ms (FP32) at L = 12, 0.52 / 0.90 ms at L = 11; at
     // L = 10 (252 alpha strings) the row gather stays faster
     if (sp->kind == CF_FCI && sp->da >= 400 && sp->db <= kEllMaxDb) {
         make_ell(m.get());
-        m->auto64 = SpmvChoice{4, 0, 0, ell_default_shape(sp->db, 8)};
-        m->auto32 = SpmvChoice{4, 0, 0, ell_default_shape(sp->db, 4)};
+        m->auto64 = SpmvChoice{4, 0, 0, ell_default_shape(sp->db, 8), {}};
+        m->auto32 = SpmvChoice{4, 0, 0, ell_default_shape(sp->db, 4), {}};
+    }
+    // DOCI matrices: hop columns (4 bytes per element) + pair tables in LDS
+    // replace the 8/12-byte element stream of the row gather.  Measured on
+    // MI355X: L = 24 0.47 ms against 1.47 (FP64) / 1.08 ms (FP32), L = 20
+    // 31 against 72 / 53 us, L = 18 (48,620 rows) 13.2 against 15.7 us FP64
+    // but 13.3 against 11.6 us FP32; below that the launch dominates and the
+    // row gather is as fast, so smaller matrices build the copy on request
+    if (sp->kind == CF_DOCI && m->n >= 40000) {
+        make_doci_ell(m.get());
+        const SpmvChoice k5{5, 0, 0, {}, doci_default_shape(m->n)};
+        m->auto64 = k5;
+        if (m->n >= 100000) m->auto32 = k5;
     }
     *out = m.release();
     API_END
@@ -1245,6 +1305,9 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
         const cf_space* sp = m->sp;
         const int Sb = sp->tb->S, Eb = sp->tb->E;
         *bytes = m->n * m->eargs.R * sval + 16 * m->n + (uint64_t)(Sb + Eb) * sp->db * 2;
+    } else if (ch.kernel == 5) {
+        // hop columns + string + diagonal + x once + y
+        *bytes = m->n * ((uint64_t)m->dargs.np * m->dargs.nv * 4 + 32);
     } else {
         *bytes = m->nnz * (sval + 4) + (m->n + 1) * 8 + 16 * m->n;
     }
@@ -1258,6 +1321,10 @@ int cf_crs_set_spmv_shape(cf_crs* m, int value_prec, int kernel, int p1, int p2)
     CF_REQUIRE(m, "matrix is NULL");
     CF_REQUIRE(value_prec == CF_F64 || value_prec == CF_F32, "value_prec must be CF_F64 or CF_F32");
     const SpmvChoice ch = make_choice(kernel, p1, p2, m->sp->db, value_prec);
+    if (kernel == 5 && m->sp->kind == CF_DOCI && !m->doci_ready) {
+        m->sp->ham->ctx->use();
+        make_doci_ell(m);
+    }
     check_shape(m, ch);
     (value_prec == CF_F32 ? m->auto32 : m->auto64) = ch;
     m->kernel = 0;
@@ -1273,6 +1340,7 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
     cf_ctx* c = m->sp->ham->ctx;
     c->use();
     const uint64_t db = m->sp->db;
+    if (m->sp->kind == CF_DOCI && !m->doci_ready) make_doci_ell(m);
     std::vector<SpmvChoice> cand;
     for (int l : {8, 16, 32, 64}) cand.push_back(make_choice(1, l, 0, db, value_prec));
     for (int l : {4, 8, 16, 32})
@@ -1291,6 +1359,9 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
         }
         for (int t : {512, 1024}) cand.push_back(make_choice(4, t, 0, db, value_prec));
     }
+    if (m->doci_ready)
+        for (int t : {128, 256, 512})
+            for (int u : {1, 2, 4, 8}) cand.push_back(make_choice(5, t, u, db, value_prec));
     ensure_staging(c, m->n);
     CF_HIP(hipMemsetAsync(c->hx.p, 0, m->n * sizeof(double), c->stream));
     hipEvent_t e0, e1;
@@ -1324,8 +1395,8 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
         for (size_t i = 0; i < cand.size() && (int)i < cap; ++i) {
             const SpmvChoice& ch = cand[i];
             table[i].kernel = ch.kernel;
-            table[i].p1 = ch.kernel == 4 ? ch.ell.threads : ch.lanes;
-            table[i].p2 = ch.kernel == 4 ? ch.ell.vec : ch.unroll;
+            table[i].p1 = ch.kernel == 4 ? ch.ell.threads : ch.kernel == 5 ? ch.doci.threads : ch.lanes;
+            table[i].p2 = ch.kernel == 4 ? ch.ell.vec : ch.kernel == 5 ? ch.doci.sub : ch.unroll;
             table[i].ms = ms[i];
         }
     if (count) *count = (int)cand.size();
@@ -1349,8 +1420,18 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y) {
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
     API_BEGIN
     CF_REQUIRE(m, "matrix is NULL");
-    CF_REQUIRE(kind >= 0 && kind <= 4, "kind must be 0, 1, 2, 3 or 4");
+    CF_REQUIRE(kind >= 0 && kind <= 5, "kind must be 0, 1, 2, 3, 4 or 5");
     m->sp->ham->ctx->use();
+    if (kind == 5) {
+        CF_REQUIRE(m->sp->kind == CF_DOCI, "kernel 5 needs a DOCI matrix");
+        if (tile_rows > 0)
+            CF_REQUIRE(tile_rows % 64 == 0 && tile_rows <= 1024, "DOCI workgroup size must be a multiple of 64 <= 1024");
+        if (!m->doci_ready) make_doci_ell(m);
+        for (SpmvChoice* a : {&m->auto64, &m->auto32}) {
+            if (a->kernel != 5) a->doci = doci_default_shape(m->n);
+            if (tile_rows > 0) a->doci.threads = tile_rows;
+        }
+    }
     if (kind == 4) {
         if (tile_rows > 0)
             CF_REQUIRE(tile_rows % 64 == 0 && tile_rows <= 1024 && (m->sp->db + tile_rows - 1) / tile_rows <= 16,

@@ -29,17 +29,6 @@
 
 namespace cf {
 
-__device__ __forceinline__ uint32_t rdm_rank(str_t s, const uint64_t* binom) {
-    uint32_t r = 0;
-    int m = 1;
-    while (s) {
-        r += (uint32_t)binom[__builtin_ctzll(s) * kBinomCols + m];
-        ++m;
-        s &= s - 1;
-    }
-    return r;
-}
-
 // X^a and X^b rows of determinant K (row-major n x L^2).  Rows must be zeroed.
 __global__ void k_build_x(uint64_t da, uint64_t db, int L, const str_t* __restrict__ sa, const str_t* __restrict__ sb,
                           const Exc* __restrict__ suba, const Exc* __restrict__ subb, int Sa, int Sb,
@@ -251,7 +240,7 @@ __global__ void k_doci_rdm(uint64_t d, int L, const str_t* __restrict__ s, const
             if (two && j != k && hk && !hj) {
                 // J = x holds k, the partner I' = x - k + j holds j
                 const str_t y = x ^ ((str_t)1 << k) ^ ((str_t)1 << j);
-                Pj += c[rdm_rank(y, binom)] * cI;
+                Pj += c[rank_bits(y, binom)] * cI;
             }
         }
         o[bin] = D;

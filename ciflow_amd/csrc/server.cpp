@@ -74,7 +74,8 @@ uint64_t estimate_footprint(const cf_request& r) {
         row = 1 + sa + sb + sa * sb + dda + ddb;
     }
     const uint64_t bytes_nnz = r.precision == CF_F32 ? 16 : 12;
-    const uint64_t crs = n * row * bytes_nnz + (n + 1) * 8 + n * 4;
+    uint64_t crs = n * row * bytes_nnz + (n + 1) * 8 + n * 4;
+    if (r.kind == CF_DOCI) crs += n * (row - 1) * 4;  // hop columns of the DOCI SpMV
     const uint64_t krylov = (uint64_t)(ncv + 3) * n * 8;
     return crs + krylov + ints + (64ull << 20);
 }

@@ -1,0 +1,40 @@
+// DOCI SpMV on position-major pair-hop columns (see spmv_doci.hip).
+#pragma once
+
+#include "kernels.h"
+
+namespace cf {
+
+// Every DOCI row has the same np * nv pair hops (np pairs, nv = L - np empty
+// orbitals), and the element of a hop j -> k depends on (j, k) only.  Slot
+// t = a * nv + b of a row is the hop of its a-th occupied orbital to its b-th
+// empty orbital, both counted from the top; col[t * n + i] is the rank of the
+// target string of row i.  Values come from an L x L pair table in LDS.
+struct DociEllArgs {
+    uint64_t n = 0;
+    int L = 0, np = 0, nv = 0;
+    const str_t* s = nullptr;        // row strings (rank order)
+    const uint32_t* col = nullptr;   // [np * nv][n]
+    const double* diag = nullptr;    // diagonal elements
+    double zero = 0.0;               // elements with |value| <= zero count as absent
+};
+
+// Workgroup size (64..1024, multiple of 64) and lanes per row (1, 2, 4, 8).
+struct DociShape {
+    int threads = 256;
+    int sub = 1;
+};
+// Lanes per row that fill the GPU for n rows (256-thread workgroups).
+DociShape doci_default_shape(uint64_t n);
+
+// col of DociEllArgs from the row strings (binom: the rank table).
+void launch_doci_ell_fill(const DociEllArgs& a, const uint64_t* binom, uint32_t* col, hipStream_t s);
+// Pair table: vk[j * L + k] = V(m, m, M, M) with m = min(j, k), M = max(j, k),
+// zeroed where |value| <= zero, as the CRS build drops such elements.
+void launch_doci_pair_table(const DociEllArgs& a, HamDev H, double* vk64, float* vk32, hipStream_t s);
+void launch_spmv_doci_f64(const DociEllArgs& a, DociShape sh, const double* vk, const double* x, double* y,
+                          hipStream_t s);
+void launch_spmv_doci_f32(const DociEllArgs& a, DociShape sh, const float* vk, const double* x, double* y,
+                          hipStream_t s);
+
+}  // namespace cf

@@ -6,6 +6,20 @@
 
 namespace cf {
 
+// Rank of a string in increasing-integer (Gosper) order: the combinatorial
+// number system, rank(s) = sum_m C(k_m, m) with k_m the position of the m-th
+// set bit (Permutator.cpp:113-153 enumerates, CIMethod.cpp:663-747 indexes).
+__device__ __forceinline__ uint32_t rank_bits(str_t s, const uint64_t* binom) {
+    uint32_t r = 0;
+    int m = 1;
+    while (s) {
+        r += (uint32_t)binom[__builtin_ctzll(s) * kBinomCols + m];
+        ++m;
+        s &= s - 1;
+    }
+    return r;
+}
+
 // One-string part of CIMethod::diagonal: get_diag_cont_one_bitstring
 // (CIMethod.cpp:329-341), same summation order (h terms, then pairs k<l).
 __device__ __forceinline__ double diag_one_string(str_t s, const HamDev& H) {

@@ -16,19 +16,7 @@ namespace cf {
 
 __device__ __forceinline__ uint64_t dbinom(const uint64_t* binom, int k, int m) { return binom[k * kBinomCols + m]; }
 
-__device__ __forceinline__ uint32_t rank_string(str_t s, const uint64_t* binom) {
-    uint32_t r = 0;
-    int m = 1;
-    while (s) {
-        int k = __builtin_ctzll(s);
-        r += (uint32_t)dbinom(binom, k, m);
-        ++m;
-        s &= s - 1;
-    }
-    return r;
-}
-
-__device__ __forceinline__ str_t unrank_string(uint64_t r, int L, int nel, const uint64_t* binom) {
+__device__ __forceinline__ str_t unrank_bits(uint64_t r, int L, int nel, const uint64_t* binom) {
     str_t s = 0;
     int hi = L;  // bits of lower m lie strictly below the previous choice
     for (int m = nel; m >= 1; --m) {
@@ -44,13 +32,13 @@ __device__ __forceinline__ str_t unrank_string(uint64_t r, int L, int nel, const
 __global__ void k_unrank(int L, int nel, uint64_t first, uint64_t count, const uint64_t* __restrict__ binom,
                          str_t* __restrict__ out) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) out[i] = unrank_string(first + i, L, nel, binom);
+    if (i < count) out[i] = unrank_bits(first + i, L, nel, binom);
 }
 
 __global__ void k_rank(const str_t* __restrict__ s, uint64_t count, const uint64_t* __restrict__ binom,
                        uint32_t* __restrict__ out) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) out[i] = rank_string(s[i], binom);
+    if (i < count) out[i] = rank_bits(s[i], binom);
 }
 
 void launch_unrank(int L, int nel, uint64_t first, uint64_t count, const uint64_t* binom, str_t* out,
@@ -167,7 +155,7 @@ __global__ void __launch_bounds__(64) k_exc_lists(int L, int nel, const str_t* _
         const str_t J = key[e];
         const uint32_t c = code[e];
         Exc x;
-        x.J = rank_string(J, binom);
+        x.J = rank_bits(J, binom);
         x.type = (uint8_t)(c & 3u);
         x.sign = 1;
         x.f = 0;

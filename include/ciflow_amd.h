@@ -162,9 +162,11 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * with vectorised column/value streams, 4 = FCI position-major complete rows
  * with LDS-staged x blocks (FCI, d_beta <= 8192; builds an extra value copy;
  * tile_rows > 0 = workgroup size, a multiple of 64 <= 1024, for both value
- * precisions). */
+ * precisions), 5 = DOCI position-major pair-hop columns with the pair
+ * elements in LDS (DOCI matrices, built with the matrix; 4 bytes per
+ * element; tile_rows > 0 = workgroup size). */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
-/* Kernel the next product with value_prec would run (1-4) and its algorithmic
+/* Kernel the next product with value_prec would run (1-5) and its algorithmic
  * HBM bytes (matrix stream + x + y), the roofline numerator. */
 int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* kernel);
 /* Launch shape of one SpMV kernel for one value precision (no reference
@@ -172,7 +174,9 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
  * lanes per row (8, 16, 32, 64); kernel 3: p1 = lanes per row (4, 8, 16,
  * 32), p2 = unroll (1, 2); kernel 4: p1 = workgroup size (multiple of 64,
  * <= 1024, 0 = default), p2 = beta strings per thread (1, 2 or 4, dividing
- * d_beta; 0 = the single-buffered kernel with p1 in 256/512/1024).  The shape
+ * d_beta; 0 = the single-buffered kernel with p1 in 256/512/1024); kernel 5:
+ * p1 = workgroup size (multiple of 64, <= 1024, 0 = 256), p2 = lanes per row
+ * (1, 2, 4 or 8).  The shape
  * becomes the automatic choice for value_prec and the matrix returns to
  * automatic selection (kernel 0). */
 int cf_crs_set_spmv_shape(cf_crs* m, int value_prec, int kernel, int p1, int p2);

@@ -506,6 +506,71 @@ def test_ell_spmv_matches_scalar_gather(case, nt):
         assert abs(cim.get_ci_energy() + econst - golden()["beh2"]["fci_energy"]) < 1e-10
 
 
+DOCI_HOP_CASES = {
+    "doci_L9_3": (9, 3, 4),
+    "doci_L6_1": (6, 1, 5),
+    "doci_L7_6": (7, 6, 6),
+    "doci_L14_7_dropped": (14, 7, 7),
+    "doci_L20_10": (20, 10, 8),
+}
+
+
+@pytest.mark.parametrize("case", ["doci_beh2", "doci_L10_5", "doci_redbcs_L8"] + list(DOCI_HOP_CASES))
+def test_doci_hop_spmv_matches_scalar_gather(case):
+    """Kernel 5 (position-major pair-hop columns, pair elements from an LDS
+    table) against the scalar row gather of the stored CRS, FP64 and FP32
+    values, 1-8 lanes per row (more lanes than pairs included): BeH2 (D2h
+    zeros dropped by the build), one pair, one hole, and a threshold that
+    drops half of the pair elements."""
+    say("doci hop spmv", case)
+    zero = 1e-14
+    if case in CASES:
+        _, make = CASES[case]
+        h, V, na, nb = make()
+    else:
+        L, na, seed = DOCI_HOP_CASES[case]
+        nb = na
+        h, V = CA.gen_integrals(L, seed)
+        if case.endswith("dropped"):
+            m0 = CA.DOCI(CA.Hamiltonian(h, V, na, nb)).get_mat()
+            row, col, val = m0.export_full(0, 64)
+            off = np.abs(val[col != np.repeat(np.arange(64), np.diff(row.astype(np.int64)))])
+            zero = float(np.median(off))
+            del m0
+    cim = CA.DOCI(CA.Hamiltonian(h, V, na, nb), value_prec=CA.CF_F32, zero=zero)
+    m = cim.get_mat()
+    n = cim.get_dim()
+    if case.endswith("dropped"):
+        assert m.info()[1] < n * (1 + na * (len(h) - na)) * 0.75  # elements really dropped
+    # automatic choice: the hop kernel from 40,000 rows (FP64) / 100,000 (FP32)
+    assert (m.spmv_bytes(CA.CF_F64)[0] == 5) == (n >= 40000)
+    assert (m.spmv_bytes(CA.CF_F32)[0] == 5) == (n >= 100000)
+    x = np.random.default_rng(9).standard_normal(n)
+    for prec in (CA.CF_F64, CA.CF_F32):
+        m.set_spmv_kernel(1)
+        y1 = m.mvprod(x, prec)
+        m.set_spmv_kernel(5)  # builds the hop columns on request
+        y2 = m.mvprod(x, prec)
+        assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec)
+        for nt, sub in ((64, 1), (256, 2), (1024, 4), (128, 8), (256, 8)):
+            m.set_spmv_shape(prec, 5, nt, sub)
+            y2 = m.mvprod(x, prec)
+            assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec, nt, sub)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 5, 100, 1)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 5, 256, 3)
+
+
+def test_doci_hop_kernel_rejected_for_fci():
+    h, V = CA.gen_integrals(6, 7)
+    m = CA.FCI(CA.Hamiltonian(h, V, 3, 3)).get_mat()
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_kernel(5)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 5, 256, 1)
+
+
 @pytest.mark.parametrize("L,na,nb", [(10, 5, 5), (7, 3, 2)])
 def test_autotune_shapes_agree_and_fastest_is_installed(L, na, nb):
     """Every launch shape the autotuner times gives the scalar gather's product
