@@ -145,6 +145,7 @@ def lib():
         "cf_read_integrals": (C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                         C.POINTER(C.c_double), dp, dp, C.POINTER(C.c_int)]),
         "cf_space_destroy": (C.c_int, [vp]),
+        "cf_space_set_ham": (C.c_int, [vp, vp]),
         "cf_space_dims": (C.c_int, [vp, u64p, u64p, u64p]),
         "cf_strings_unrank": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, u64p]),
         "cf_strings_rank": (C.c_int, [vp, C.c_int, C.c_int, u64p, C.c_uint64, u32p]),
@@ -509,6 +510,30 @@ class CIMethod:
     def set_sparsity(self, zero: float):
         self._zero = zero
 
+    # CIMethod::set_ham (CIMethod.cpp:117-126)
+    def set_ham(self, ham: Hamiltonian, build: bool = True):
+        """New integrals for this method; the solution and densities are reset.
+        A Hamiltonian with the same L, nalpha and nbeta on the same context
+        keeps the space and its matrix resident: cf_space_set_ham refills the
+        values in place (and the CRS arrays, pattern included, when they are
+        read).  Any other Hamiltonian gets a new space and matrix."""
+        if ham.ctx is self.ctx and (ham.L, ham.nup, ham.ndown) == (self.ham.L, self.ham.nup, self.ham.ndown):
+            _check(lib().cf_space_set_ham(self.ptr, ham.ptr))
+            self.space.parent = ham.handle
+            self.ham = ham
+            self._reset_solution()
+            if build and self._mat is None:
+                self.build_ci()
+        else:
+            self._new_space(ham, build)
+
+    def _new_space(self, ham: Hamiltonian, build: bool):
+        CIMethod.__init__(self, ham, value_prec=self._value_prec, build=build, zero=self._zero)
+
+    def _reset_solution(self):
+        self._eigval = self._eigvec = self._info = None
+        self._one = self._two = None
+
     def get_dim(self):
         return self.n
 
@@ -677,6 +702,17 @@ class FCI_Direct(CIMethod):
 
     def get_name(self):
         return "FCI_Direct"
+
+    def set_ham(self, ham: Hamiltonian, build: bool = True):
+        """CIMethod::set_ham: the space stays resident, the sigma's integral
+        tables are made again for the new Hamiltonian."""
+        self.sig = None
+        super().set_ham(ham, build=False)
+        self.sig = _Handle("cf_sigma_destroy", self.space)
+        _check(lib().cf_sigma_create(self.ptr, self._value_prec, C.byref(self.sig.ptr)))
+
+    def _new_space(self, ham: Hamiltonian, build: bool):
+        FCI_Direct.__init__(self, ham, value_prec=self._value_prec)
 
     def set_threads(self, threads: int):
         """Workgroup size of the alpha-beta sigma kernel (128, 256, 512, 768, 1024)."""

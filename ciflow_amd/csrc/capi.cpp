@@ -198,6 +198,7 @@ struct cf_space {
     DBuf<double> diag;
     DBuf<double> vec;  // resident eigenvector after cf_solve
     bool vec_valid = false;
+    uint64_t version = 0;  // incremented by cf_space_set_ham
     DBuf<str_t> dup, ddown;  // CF_DETS: the determinant list, list order
 };
 
@@ -216,6 +217,12 @@ struct cf_crs {
     uint64_t n = 0, nnz = 0;
     int64_t nnz_upper = -1;
     double zero = 0.0;  // build threshold: elements with |v| <= zero are not stored
+    bool want32 = false;  // FP32 value copy requested at build
+    // Hamiltonian version of the space the values belong to; a newer space
+    // version refills the derived copies and marks the CRS arrays stale (they
+    // are rebuilt, pattern included, when something reads them)
+    uint64_t version = 0;
+    bool crs_stale = false;
     DBuf<int64_t> rowptr;
     DBuf<uint32_t> col;
     DBuf<double> val;
@@ -243,6 +250,26 @@ struct cf_crs {
 };
 
 namespace {
+// One-string diagonal parts and excitation lists (values included) of a
+// string table for the Hamiltonian H; the strings themselves do not depend on H.
+void fill_strtab(cf_ctx* ctx, const cf_ham* H, StrTab* t) {
+    launch_diag1(t->nel, t->s.p, t->d, H->dev(), t->d1.p, ctx->stream);
+    if (t->E)
+        launch_exc_lists(H->L, t->nel, t->s.p, t->d, ctx->binom.p, H->dev(), t->d1.p, t->E, t->S, t->full.p, t->sub.p,
+                         ctx->stream);
+}
+
+// Diagonal of the space for its current Hamiltonian.
+void fill_space_diag(cf_space* sp) {
+    const cf_ham* ham = sp->ham;
+    cf_ctx* c = ham->ctx;
+    if (sp->kind == CF_DOCI)
+        launch_doci_diag(sp->n, sp->ta->s.p, sp->ta->d1.p, ham->dev(), sp->diag.p, c->stream);
+    else if (sp->kind == CF_FCI)
+        launch_fci_diag(sp->da, sp->db, sp->ta->s.p, sp->tb->s.p, sp->ta->d1.p, sp->tb->d1.p, ham->dev(),
+                        sp->diag.p, c->stream);
+}
+
 StrTab* make_strtab(cf_ctx* ctx, const cf_ham* H, int nel, bool lists) {
     std::unique_ptr<StrTab> t(new StrTab());
     t->L = H->L;
@@ -252,7 +279,6 @@ StrTab* make_strtab(cf_ctx* ctx, const cf_ham* H, int nel, bool lists) {
     t->s.alloc(t->d);
     t->d1.alloc(t->d);
     launch_unrank(H->L, nel, 0, t->d, ctx->binom.p, t->s.p, ctx->stream);
-    launch_diag1(nel, t->s.p, t->d, H->dev(), t->d1.p, ctx->stream);
     if (lists) {
         const int nv = H->L - nel;
         const uint64_t ns = (uint64_t)nel * nv;
@@ -262,9 +288,8 @@ StrTab* make_strtab(cf_ctx* ctx, const cf_ham* H, int nel, bool lists) {
         t->S = (int)(1 + ns);
         t->full.alloc(t->d * t->E);
         t->sub.alloc(t->d * t->S);
-        launch_exc_lists(H->L, nel, t->s.p, t->d, ctx->binom.p, H->dev(), t->d1.p, t->E, t->S, t->full.p, t->sub.p,
-                         ctx->stream);
     }
+    fill_strtab(ctx, H, t.get());
     return t.release();
 }
 
@@ -272,6 +297,9 @@ void ensure_staging(cf_ctx* c, uint64_t n) {
     if (c->hx.n < n) c->hx.alloc(n);
     if (c->hy.n < n) c->hy.alloc(n);
 }
+
+void sync_ham(cf_crs* m);
+void ensure_crs(cf_crs* m);
 
 // The configuration spmv() runs for a value precision: the explicit kernel if
 // one was set, else the automatic choice.
@@ -292,6 +320,8 @@ SpmvChoice resolve_spmv(const cf_crs* m, int prec) {
 
 void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, double* dy) {
     cf_ctx* c = m->sp->ham->ctx;
+    sync_ham(m);
+    if (m->crs_stale && ch.kernel != 4 && ch.kernel != 5) ensure_crs(m);
     if (prec == CF_F32) {
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
@@ -405,6 +435,134 @@ int pick_lanes_per_row(uint64_t n, uint64_t nnz) {
     if (avg >= 128) return 32;
     if (avg >= 24) return 16;
     return 8;
+}
+
+// Count -> scan -> fill of the CRS arrays for the space's current
+// Hamiltonian (CIMethod::build_parallel, CIMethod.cpp:489-561); reallocates
+// col/val when the count changes.
+void build_crs_arrays(cf_crs* m) {
+    cf_space* sp = m->sp;
+    cf_ctx* c = sp->ham->ctx;
+    const double zero = m->zero;
+    DBuf<uint32_t> cnt(m->n);
+    if (!m->rowptr.p) m->rowptr.alloc(m->n + 1);
+    const cf_ham* H = sp->ham;
+    FciRowsArgs fa{};
+    DociRowsArgs dargs{};
+    DetRowsArgs targs{};
+    if (sp->kind == CF_DETS) {
+        targs.n = sp->n;
+        targs.nup = H->nup;
+        targs.ndown = H->ndown;
+        targs.up = sp->dup.p;
+        targs.down = sp->ddown.p;
+        targs.H = H->dev();
+        targs.zero = zero;
+        targs.rowcnt = cnt.p;
+        KTimer t(c, "crs_count");
+        launch_det_rows(targs, false, c->stream);
+    } else if (sp->kind == CF_FCI) {
+        fa.da = sp->da;
+        fa.db = sp->db;
+        fa.Ea = sp->ta->E;
+        fa.Sa = sp->ta->S;
+        fa.Eb = sp->tb->E;
+        fa.Sb = sp->tb->S;
+        fa.sa = sp->ta->s.p;
+        fa.sb = sp->tb->s.p;
+        fa.fa = sp->ta->full.p;
+        fa.fb = sp->tb->full.p;
+        fa.subb = sp->tb->sub.p;
+        fa.segoff = sp->segoff.p;
+        fa.diag = sp->diag.p;
+        fa.H = H->dev();
+        fa.zero = zero;
+        fa.row0 = 0;
+        fa.nrows = m->n;
+        fa.rowcnt = cnt.p;
+        m->fargs = fa;
+        m->fargs.rowcnt = nullptr;
+        m->has_fargs = true;
+        KTimer t(c, "crs_count");
+        launch_fci_rows(fa, false, c->stream);
+    } else {
+        dargs.d = sp->n;
+        dargs.L = H->L;
+        dargs.np = H->nup;
+        dargs.s = sp->ta->s.p;
+        dargs.binom = c->binom.p;
+        dargs.diag = sp->diag.p;
+        dargs.H = H->dev();
+        dargs.zero = zero;
+        dargs.row0 = 0;
+        dargs.nrows = m->n;
+        dargs.rowcnt = cnt.p;
+        KTimer t(c, "crs_count");
+        launch_doci_rows(dargs, false, c->stream);
+    }
+    exclusive_scan_u32_to_i64(cnt.p, m->rowptr.p, m->n, c->stream);
+    int64_t nnz = 0;
+    copy_sync(&nnz, m->rowptr.p + m->n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
+    m->nnz = (uint64_t)nnz;
+    m->col.alloc(m->nnz);
+    m->val.alloc(m->nnz);
+    if (m->want32) m->val32.alloc(m->nnz);
+    if (sp->kind == CF_DETS) {
+        targs.rowptr = m->rowptr.p;
+        targs.col = m->col.p;
+        targs.val = m->val.p;
+        targs.val32 = m->val32.p;
+        KTimer t(c, "crs_fill");
+        launch_det_rows(targs, true, c->stream);
+    } else if (sp->kind == CF_FCI) {
+        fa.rowptr = m->rowptr.p;
+        fa.col = m->col.p;
+        fa.val = m->val.p;
+        fa.val32 = m->val32.p;
+        KTimer t(c, "crs_fill");
+        launch_fci_rows(fa, true, c->stream);
+    } else {
+        dargs.rowptr = m->rowptr.p;
+        dargs.col = m->col.p;
+        dargs.val = m->val.p;
+        dargs.val32 = m->val32.p;
+        KTimer t(c, "crs_fill");
+        launch_doci_rows(dargs, true, c->stream);
+    }
+    CF_HIP(hipStreamSynchronize(c->stream));
+    m->lpr = pick_lanes_per_row(m->n, m->nnz);
+    m->nnz_upper = -1;
+    m->crs_stale = false;
+}
+
+// Brings a matrix up to its space's Hamiltonian after cf_space_set_ham: the
+// pair table (kernel 5) and the position-major values (kernel 4) are refilled
+// in place -- their layouts do not depend on the integrals -- and the CRS
+// arrays are marked stale, to be rebuilt (pattern included, since elements
+// can cross the drop threshold) when something reads them.
+void sync_ham(cf_crs* m) {
+    cf_space* sp = m->sp;
+    if (m->version == sp->version) return;
+    cf_ctx* c = sp->ham->ctx;
+    const HamDev H = sp->ham->dev();
+    if (m->doci_ready) launch_doci_pair_table(m->dargs, H, m->dvk.p, m->dvk32.p, c->stream);
+    if (m->has_fargs) m->fargs.H = H;
+    if (m->ell_ready) {
+        KTimer t(c, "crs_fill");
+        launch_fci_ell(m->fargs, m->ell.p, m->ell32.p, c->stream);
+    }
+    m->plan.reset();  // the staged kernel reads the CRS arrays; rebuilt on request
+    m->crs_stale = true;
+    m->version = sp->version;
+}
+
+// The CRS arrays for the current Hamiltonian.
+void ensure_crs(cf_crs* m) {
+    sync_ham(m);
+    if (m->crs_stale) {
+        build_crs_arrays(m);
+        m->lpr = pick_lanes_per_row(m->n, m->nnz);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -850,7 +1008,7 @@ int cf_space_create(cf_ham* ham, int kind, cf_space** out) {
         sp->da = sp->db = sp->ta->d;
         sp->n = sp->da;
         sp->diag.alloc(sp->n);
-        launch_doci_diag(sp->n, sp->ta->s.p, sp->ta->d1.p, ham->dev(), sp->diag.p, c->stream);
+        fill_space_diag(sp.get());
     } else {
         sp->ta.reset(make_strtab(c, ham, ham->nup, true));
         if (ham->ndown == ham->nup) {
@@ -866,8 +1024,7 @@ int cf_space_create(cf_ham* ham, int kind, cf_space** out) {
         sp->segoff.alloc(sp->da * (uint64_t)(sp->ta->E + 1));
         launch_fci_segoff(sp->da, sp->ta->E, sp->ta->full.p, sp->tb->E, sp->tb->S, sp->segoff.p, c->stream);
         sp->diag.alloc(sp->n);
-        launch_fci_diag(sp->da, sp->db, sp->ta->s.p, sp->tb->s.p, sp->ta->d1.p, sp->tb->d1.p, ham->dev(),
-                        sp->diag.p, c->stream);
+        fill_space_diag(sp.get());
     }
     CF_HIP(hipStreamSynchronize(c->stream));
     *out = sp.release();
@@ -1015,6 +1172,26 @@ int cf_space_destroy(cf_space* sp) {
     API_END
 }
 
+int cf_space_set_ham(cf_space* sp, cf_ham* ham) {
+    API_BEGIN
+    CF_REQUIRE(sp && ham, "NULL argument");
+    CF_REQUIRE(ham->ctx == sp->ham->ctx, "the Hamiltonian belongs to another context");
+    CF_REQUIRE(ham->L == sp->ham->L && ham->nup == sp->ham->nup && ham->ndown == sp->ham->ndown,
+               "the Hamiltonian must have the space's L, nup and ndown");
+    cf_ctx* c = ham->ctx;
+    c->use();
+    sp->ham = ham;
+    if (sp->kind != CF_DETS) {  // determinant-list rows read the integrals directly
+        fill_strtab(c, ham, sp->ta.get());
+        if (sp->tb_own) fill_strtab(c, ham, sp->tb_own.get());
+        fill_space_diag(sp);
+    }
+    sp->vec_valid = false;
+    ++sp->version;
+    CF_HIP(hipStreamSynchronize(c->stream));
+    API_END
+}
+
 int cf_space_dims(const cf_space* sp, uint64_t* n, uint64_t* dim_up, uint64_t* dim_down) {
     API_BEGIN
     CF_REQUIRE(sp, "space is NULL");
@@ -1067,93 +1244,9 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     m->n = sp->n;
     m->zero = zero;
     CF_REQUIRE(m->n < (1ull << 32), "matrix dimension must be < 2^32 (u32 columns)");
-    DBuf<uint32_t> cnt(m->n);
-    m->rowptr.alloc(m->n + 1);
-    const cf_ham* H = sp->ham;
-    FciRowsArgs fa{};
-    DociRowsArgs dargs{};
-    DetRowsArgs targs{};
-    if (sp->kind == CF_DETS) {
-        targs.n = sp->n;
-        targs.nup = H->nup;
-        targs.ndown = H->ndown;
-        targs.up = sp->dup.p;
-        targs.down = sp->ddown.p;
-        targs.H = H->dev();
-        targs.zero = zero;
-        targs.rowcnt = cnt.p;
-        KTimer t(c, "crs_count");
-        launch_det_rows(targs, false, c->stream);
-    } else if (sp->kind == CF_FCI) {
-        fa.da = sp->da;
-        fa.db = sp->db;
-        fa.Ea = sp->ta->E;
-        fa.Sa = sp->ta->S;
-        fa.Eb = sp->tb->E;
-        fa.Sb = sp->tb->S;
-        fa.sa = sp->ta->s.p;
-        fa.sb = sp->tb->s.p;
-        fa.fa = sp->ta->full.p;
-        fa.fb = sp->tb->full.p;
-        fa.subb = sp->tb->sub.p;
-        fa.segoff = sp->segoff.p;
-        fa.diag = sp->diag.p;
-        fa.H = H->dev();
-        fa.zero = zero;
-        fa.row0 = 0;
-        fa.nrows = m->n;
-        fa.rowcnt = cnt.p;
-        m->fargs = fa;
-        m->fargs.rowcnt = nullptr;
-        m->has_fargs = true;
-        KTimer t(c, "crs_count");
-        launch_fci_rows(fa, false, c->stream);
-    } else {
-        dargs.d = sp->n;
-        dargs.L = H->L;
-        dargs.np = H->nup;
-        dargs.s = sp->ta->s.p;
-        dargs.binom = c->binom.p;
-        dargs.diag = sp->diag.p;
-        dargs.H = H->dev();
-        dargs.zero = zero;
-        dargs.row0 = 0;
-        dargs.nrows = m->n;
-        dargs.rowcnt = cnt.p;
-        KTimer t(c, "crs_count");
-        launch_doci_rows(dargs, false, c->stream);
-    }
-    exclusive_scan_u32_to_i64(cnt.p, m->rowptr.p, m->n, c->stream);
-    int64_t nnz = 0;
-    copy_sync(&nnz, m->rowptr.p + m->n, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream);
-    m->nnz = (uint64_t)nnz;
-    m->col.alloc(m->nnz);
-    m->val.alloc(m->nnz);
-    if (value_prec == CF_F32) m->val32.alloc(m->nnz);
-    if (sp->kind == CF_DETS) {
-        targs.rowptr = m->rowptr.p;
-        targs.col = m->col.p;
-        targs.val = m->val.p;
-        targs.val32 = m->val32.p;
-        KTimer t(c, "crs_fill");
-        launch_det_rows(targs, true, c->stream);
-    } else if (sp->kind == CF_FCI) {
-        fa.rowptr = m->rowptr.p;
-        fa.col = m->col.p;
-        fa.val = m->val.p;
-        fa.val32 = m->val32.p;
-        KTimer t(c, "crs_fill");
-        launch_fci_rows(fa, true, c->stream);
-    } else {
-        dargs.rowptr = m->rowptr.p;
-        dargs.col = m->col.p;
-        dargs.val = m->val.p;
-        dargs.val32 = m->val32.p;
-        KTimer t(c, "crs_fill");
-        launch_doci_rows(dargs, true, c->stream);
-    }
-    CF_HIP(hipStreamSynchronize(c->stream));
-    m->lpr = pick_lanes_per_row(m->n, m->nnz);
+    m->want32 = value_prec == CF_F32;
+    m->version = sp->version;
+    build_crs_arrays(m.get());
     // measured on MI355X (FCI L=11/12, DOCI L=20/24): vectorised streams win for
     // FP32 values at every row length and for FP64 rows of >= 512 entries
     const double avg = m->n ? (double)m->nnz / (double)m->n : 0.0;
@@ -1198,6 +1291,7 @@ int cf_crs_destroy(cf_crs* m) {
 
 static uint64_t count_rows(cf_crs* m, uint64_t r0, uint64_t r1, bool upper, std::vector<uint64_t>* per_row) {
     cf_ctx* c = m->sp->ham->ctx;
+    ensure_crs(m);
     const uint64_t nr = r1 - r0;
     std::vector<uint64_t> cnt(nr);
     if (upper) {
@@ -1242,10 +1336,12 @@ static int export_rows(cf_crs* m, uint64_t r0, uint64_t r1, bool upper, uint64_t
 int cf_crs_info(const cf_crs* m, uint64_t* n, uint64_t* nnz_full, uint64_t* nnz_upper) {
     API_BEGIN
     CF_REQUIRE(m, "matrix is NULL");
+    cf_crs* mm = const_cast<cf_crs*>(m);  // a stale CRS is rebuilt first
+    mm->sp->ham->ctx->use();
+    ensure_crs(mm);
     if (n) *n = m->n;
     if (nnz_full) *nnz_full = m->nnz;
     if (nnz_upper) {
-        cf_crs* mm = const_cast<cf_crs*>(m);
         if (mm->nnz_upper < 0) {
             mm->sp->ham->ctx->use();
             mm->nnz_upper = (int64_t)count_rows(mm, 0, mm->n, true, nullptr);
@@ -1301,6 +1397,11 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
     CF_REQUIRE(m && bytes && kernel, "NULL argument");
     const uint64_t sval = value_prec == CF_F32 ? 4 : 8;
     const SpmvChoice ch = resolve_spmv(m, value_prec);
+    if (ch.kernel != 4 && ch.kernel != 5) {
+        cf_crs* mm = const_cast<cf_crs*>(m);  // bytes of the CRS for the current Hamiltonian
+        mm->sp->ham->ctx->use();
+        ensure_crs(mm);
+    }
     if (ch.kernel == 4) {
         const cf_space* sp = m->sp;
         const int Sb = sp->tb->S, Eb = sp->tb->E;
@@ -1321,10 +1422,9 @@ int cf_crs_set_spmv_shape(cf_crs* m, int value_prec, int kernel, int p1, int p2)
     CF_REQUIRE(m, "matrix is NULL");
     CF_REQUIRE(value_prec == CF_F64 || value_prec == CF_F32, "value_prec must be CF_F64 or CF_F32");
     const SpmvChoice ch = make_choice(kernel, p1, p2, m->sp->db, value_prec);
-    if (kernel == 5 && m->sp->kind == CF_DOCI && !m->doci_ready) {
-        m->sp->ham->ctx->use();
-        make_doci_ell(m);
-    }
+    m->sp->ham->ctx->use();
+    sync_ham(m);
+    if (kernel == 5 && m->sp->kind == CF_DOCI && !m->doci_ready) make_doci_ell(m);
     check_shape(m, ch);
     (value_prec == CF_F32 ? m->auto32 : m->auto64) = ch;
     m->kernel = 0;
@@ -1340,6 +1440,7 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
     cf_ctx* c = m->sp->ham->ctx;
     c->use();
     const uint64_t db = m->sp->db;
+    ensure_crs(m);
     if (m->sp->kind == CF_DOCI && !m->doci_ready) make_doci_ell(m);
     std::vector<SpmvChoice> cand;
     for (int l : {8, 16, 32, 64}) cand.push_back(make_choice(1, l, 0, db, value_prec));
@@ -1422,6 +1523,7 @@ int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
     CF_REQUIRE(m, "matrix is NULL");
     CF_REQUIRE(kind >= 0 && kind <= 5, "kind must be 0, 1, 2, 3, 4 or 5");
     m->sp->ham->ctx->use();
+    sync_ham(m);
     if (kind == 5) {
         CF_REQUIRE(m->sp->kind == CF_DOCI, "kernel 5 needs a DOCI matrix");
         if (tile_rows > 0)
@@ -1448,6 +1550,7 @@ int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
     }
     if (kind == 2) {
         CF_REQUIRE(m->sp->kind == CF_FCI, "the staged SpMV needs an FCI matrix");
+        ensure_crs(m);
         const int W = tile_rows > 0 ? tile_rows : 16;
         make_fci_plan(m, W);
         CF_HIP(hipStreamSynchronize(m->sp->ham->ctx->stream));
@@ -1551,6 +1654,7 @@ struct cf_sigma {
     DBuf<double> vpair64, vself64;  // slice tables of the prefetching alpha-beta kernel
     DBuf<float> vpair32, vself32;
     SigmaShape shape;
+    uint64_t version = 0;  // space Hamiltonian version the tables were made for
 };
 
 namespace {
@@ -1566,6 +1670,8 @@ namespace {
 // db x da matrix Cc, so y_c = Cc Ha + Hb Cc (Ha, Hb symmetric).
 void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
     cf_space* sp = S->sp;
+    CF_REQUIRE(S->version == sp->version,
+               "the space's Hamiltonian changed after this sigma was created: create the sigma again");
     cf_ctx* c = sp->ham->ctx;
     const int m = (int)sp->db, nn = (int)sp->da;
     KTimer t(c, prec == CF_F32 ? "sigma_f32" : "sigma_f64");
@@ -1623,6 +1729,7 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     c->use();
     std::unique_ptr<cf_sigma> S(new cf_sigma());
     S->sp = sp;
+    S->version = sp->version;
     S->f32 = prec == CF_F32;
     S->Ha.alloc(sp->da * sp->da);
     launch_dense_string_ham<double>(sp->da, sp->ta->E, sp->ta->full.p, S->Ha.p, c->stream);

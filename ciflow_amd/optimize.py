@@ -164,10 +164,15 @@ class OrbitalOptimization:
         self.update_cim()
 
     def update_cim(self) -> float:
-        """set_ham + build_ci + solve for the current U; returns the CI energy."""
+        """set_ham + build_ci + solve for the current U; returns the CI energy.
+        The CI method keeps its space and matrix resident across updates
+        (CIMethod.set_ham refills the values in place)."""
         self.ham = self.ham0.rotate(self.U)
-        cls = DOCI if self.kind == CF_DOCI else FCI
-        self.cim = cls(self.ham)
+        if self.cim is None:
+            cls = DOCI if self.kind == CF_DOCI else FCI
+            self.cim = cls(self.ham, value_prec=self.precision)
+        else:
+            self.cim.set_ham(self.ham)
         self.cim.solve(precision=self.precision)
         return self.energy()
 

@@ -103,6 +103,16 @@ int cf_gen_rotation(int L, uint64_t seed, double sigma, double* K);
  *      rank fast (CIMethod.cpp:771). */
 int cf_space_create(cf_ham* ham, int kind, cf_space** out);
 int cf_space_destroy(cf_space* sp);
+/* CIMethod::set_ham (CIMethod.cpp:117-126) for a space that stays the same:
+ * the space and every matrix built on it take the new Hamiltonian (same
+ * context, L, nup, ndown) instead of being rebuilt.  Position-major values
+ * (FCI kernel 4) and the DOCI pair table (kernel 5) are refilled in place on
+ * the next use -- their layouts do not depend on the integrals -- and the CRS
+ * arrays are rebuilt, pattern included, only when something reads them
+ * (export, counts, the CRS kernels).  The resident eigenvector is dropped; an
+ * on-the-fly sigma of the space must be created again.  The space keeps a
+ * pointer to ham: ham must outlive it (or the next set_ham). */
+int cf_space_set_ham(cf_space* sp, cf_ham* ham);
 int cf_space_dims(const cf_space* sp, uint64_t* n, uint64_t* dim_up, uint64_t* dim_down);
 /* ---- integral files: Hamiltonian::read_file (Hamiltonian.cpp:707-885), the
  *      psi4 dump the reference reads ("****  Molecular Integrals" header with
