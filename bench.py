@@ -39,7 +39,7 @@ METRIC = "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of pe
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "rotate"],
+    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "spmv_doci", "rotate"],
                    default="stream")
     p.add_argument("--steps", type=int, default=None, help="stream: requests (4096); fci12: solves (4)")
     p.add_argument("--warmup", type=int, default=None, help="stream: warm-up requests (64); fci12: solves (1)")
@@ -475,14 +475,19 @@ def bench_direct16(a, CA):
     }
 
 
-def bench_spmv(a, CA):
-    """Stored-CRS SpMV alone on FCI L=12 6a6b (config 2): 3 FP64 and 3 FP32-value
-    products with the automatic kernel, then three 2 GiB device-to-device copies
-    (k_copy) -- the reference kernel for scaling rocprofv3 FETCH_SIZE /
-    WRITE_SIZE readings to bytes."""
+def bench_spmv(a, CA, doci=False):
+    """Stored-matrix SpMV alone on FCI L=12 6a6b (config 2), or with doci=True on
+    DOCI L=24 12 pairs seed 2 (config 3): 3 FP64 and 3 FP32-value products with
+    the automatic kernel, then three 2 GiB device-to-device copies (k_copy) --
+    the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings
+    to bytes."""
     ctx = CA.Context(0)
-    h, V = CA.gen_integrals(12, 1)
-    cim = CA.FCI(CA.Hamiltonian(h, V, 6, 6, 0.0, ctx), value_prec=CA.CF_F32)
+    if doci:
+        h, V = CA.gen_integrals(24, 2)
+        cim = CA.DOCI(CA.Hamiltonian(h, V, 12, 12, 0.0, ctx), value_prec=CA.CF_F32)
+    else:
+        h, V = CA.gen_integrals(12, 1)
+        cim = CA.FCI(CA.Hamiltonian(h, V, 6, 6, 0.0, ctx), value_prec=CA.CF_F32)
     m = cim.get_mat()
     n, nnz, _ = m.info()
     x = np.random.default_rng(0).standard_normal(n)
@@ -502,7 +507,8 @@ def bench_spmv(a, CA):
     copy_bytes = 2 << 30
     CA._check(CA.lib().cf_ctx_stream_copy(ctx.ptr, copy_bytes, 3))
     out["copy_bytes_each_way"] = copy_bytes
-    return {"metric": METRIC, "workload": "FCI L=12 6a6b stored-CRS SpMV", "n": n, "nnz": nnz, **out}
+    wl = "DOCI L=24 12 pairs SpMV" if doci else "FCI L=12 6a6b stored-CRS SpMV"
+    return {"metric": METRIC, "workload": wl, "n": n, "nnz": nnz, **out}
 
 
 def bench_rotate(a, CA):
@@ -539,7 +545,8 @@ def main():
     import ciflow_amd as CA
 
     run = {"stream": bench_stream, "fci12": bench_fci12, "doci24": bench_fci12, "direct16": bench_direct16,
-           "spmv": bench_spmv, "rotate": bench_rotate}[a.workload]
+           "spmv": bench_spmv, "spmv_doci": lambda a, CA: bench_spmv(a, CA, doci=True),
+           "rotate": bench_rotate}[a.workload]
     out = run(a, CA)
     print(json.dumps(out))
 
