@@ -246,6 +246,12 @@ struct cf_crs {
     DBuf<float> dvk32;
     DociEllArgs dargs;
     bool doci_ready = false;
+    // DOCI block tables (kernel 6)
+    DBuf<DociBlock> dblk;
+    DBuf<uint16_t> drank_lo, dlo_of;
+    DBuf<uint32_t> dlo_off;
+    DociBlockArgs bargs;
+    bool blk_ready = false;
     std::unique_ptr<FciSpmvPlan> plan;
 };
 
@@ -310,6 +316,7 @@ SpmvChoice resolve_spmv(const cf_crs* m, int prec) {
     if (r.kernel == 2 && !m->plan) r.kernel = 1;
     if (r.kernel == 4 && !m->ell_ready) r.kernel = 1;
     if (r.kernel == 5 && !m->doci_ready) r.kernel = 1;
+    if (r.kernel == 6 && !m->blk_ready) r.kernel = 1;
     if (r.kernel == 1) r.lanes = a.kernel == 1 && a.lanes && !m->kernel ? a.lanes : m->lpr;
     if (r.kernel == 3 && a.kernel != 3) {
         r.lanes = 16;
@@ -321,12 +328,13 @@ SpmvChoice resolve_spmv(const cf_crs* m, int prec) {
 void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, double* dy) {
     cf_ctx* c = m->sp->ham->ctx;
     sync_ham(m);
-    if (m->crs_stale && ch.kernel != 4 && ch.kernel != 5) ensure_crs(m);
+    if (m->crs_stale && ch.kernel < 4) ensure_crs(m);
     if (prec == CF_F32) {
         CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
         switch (ch.kernel) {
             case 5: launch_spmv_doci_f32(m->dargs, ch.doci, m->dvk32.p, dx, dy, c->stream); break;
+            case 6: launch_spmv_doci_blk_f32(m->bargs, m->dvk32.p, dx, dy, c->stream); break;
             case 4: launch_spmv_ell_f32(m->eargs, ch.ell, m->ell32.p, dx, dy, c->stream); break;
             case 2: launch_spmv_fci_f32(m->plan->a, m->val32.p, dx, dy, c->stream); break;
             case 3:
@@ -338,6 +346,7 @@ void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, doub
         KTimer t(c, "spmv_f64");
         switch (ch.kernel) {
             case 5: launch_spmv_doci_f64(m->dargs, ch.doci, m->dvk.p, dx, dy, c->stream); break;
+            case 6: launch_spmv_doci_blk_f64(m->bargs, m->dvk.p, dx, dy, c->stream); break;
             case 4: launch_spmv_ell_f64(m->eargs, ch.ell, m->ell.p, dx, dy, c->stream); break;
             case 2: launch_spmv_fci_f64(m->plan->a, m->val.p, dx, dy, c->stream); break;
             case 3:
@@ -351,11 +360,12 @@ void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, doub
 void spmv(cf_crs* m, int prec, const double* dx, double* dy) { spmv_with(m, prec, resolve_spmv(m, prec), dx, dy); }
 
 // Position-major DOCI hop columns + pair tables (kernel 5).
-void make_doci_ell(cf_crs* m) {
+// DOCI pair table (kernels 5 and 6): L x L elements, FP64 and FP32.
+void make_doci_table(cf_crs* m) {
+    if (m->dvk.p) return;
     cf_space* sp = m->sp;
     cf_ham* H = sp->ham;
-    cf_ctx* c = H->ctx;
-    CF_REQUIRE(sp->kind == CF_DOCI, "the DOCI hop SpMV needs a DOCI matrix");
+    CF_REQUIRE(sp->kind == CF_DOCI, "the DOCI pair-hop SpMV needs a DOCI matrix");
     DociEllArgs& a = m->dargs;
     a.n = m->n;
     a.L = H->L;
@@ -364,16 +374,81 @@ void make_doci_ell(cf_crs* m) {
     a.s = sp->ta->s.p;
     a.diag = sp->diag.p;
     a.zero = m->zero;
-    m->dcol.alloc(std::max<uint64_t>(1, m->n * (uint64_t)a.np * (uint64_t)a.nv));
     m->dvk.alloc((uint64_t)a.L * a.L);
     m->dvk32.alloc((uint64_t)a.L * a.L);
+    launch_doci_pair_table(a, H->dev(), m->dvk.p, m->dvk32.p, H->ctx->stream);
+}
+
+void make_doci_ell(cf_crs* m) {
+    make_doci_table(m);
+    cf_ctx* c = m->sp->ham->ctx;
+    DociEllArgs& a = m->dargs;
+    m->dcol.alloc(std::max<uint64_t>(1, m->n * (uint64_t)a.np * (uint64_t)a.nv));
     a.col = m->dcol.p;
     {
         KTimer t(c, "doci_ell_fill");
         launch_doci_ell_fill(a, c->binom.p, m->dcol.p, c->stream);
-        launch_doci_pair_table(a, H->dev(), m->dvk.p, m->dvk32.p, c->stream);
     }
     m->doci_ready = true;
+}
+
+// Block tables of the block-staged DOCI SpMV (kernel 6) for w low orbitals.
+void make_doci_blocks(cf_crs* m, int w) {
+    make_doci_table(m);
+    cf_space* sp = m->sp;
+    cf_ctx* c = sp->ham->ctx;
+    const int L = sp->ham->L, np = sp->ham->nup, nh = L - w;
+    CF_REQUIRE(w >= 1 && w <= kDociMaxW && nh >= 1 && nh <= 20,
+               "kernel 6: 1 <= w <= 12 low orbitals and 1..20 high orbitals");
+    std::vector<DociBlock> blocks;
+    for (uint32_t hi = 0; hi < (1u << nh); ++hi) {
+        const int ch = __builtin_popcount(hi), cl = np - ch;
+        if (cl < 0 || cl > w) continue;
+        DociBlock b;
+        b.hi = hi;
+        b.c = (uint32_t)cl;
+        b.R = (uint32_t)binom_host(w, cl);
+        uint64_t base = 0;
+        int mm = cl + 1;
+        for (uint32_t t = hi; t; t &= t - 1) base += binom_host(w + __builtin_ctz(t), mm++);
+        b.base = (uint32_t)base;
+        blocks.push_back(b);
+    }
+    std::sort(blocks.begin(), blocks.end(), [](const DociBlock& x, const DociBlock& y) { return x.base < y.base; });
+    const uint32_t W = 1u << w;
+    std::vector<uint16_t> rank_lo(W), lo_of;
+    std::vector<uint32_t> lo_off(w + 2, 0);
+    for (int cl = 0; cl <= w; ++cl) {
+        lo_off[cl] = (uint32_t)lo_of.size();
+        for (uint32_t p = 0; p < W; ++p)  // increasing integers = rank order within a popcount
+            if (__builtin_popcount(p) == cl) {
+                rank_lo[p] = (uint16_t)(lo_of.size() - lo_off[cl]);
+                lo_of.push_back((uint16_t)p);
+            }
+    }
+    lo_off[w + 1] = (uint32_t)lo_of.size();
+    m->dblk.alloc(blocks.size());
+    m->drank_lo.alloc(W);
+    m->dlo_of.alloc(W);
+    m->dlo_off.alloc(w + 2);
+    copy_sync(m->dblk.p, blocks.data(), blocks.size() * sizeof(DociBlock), hipMemcpyHostToDevice, c->stream);
+    copy_sync(m->drank_lo.p, rank_lo.data(), W * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream);
+    copy_sync(m->dlo_of.p, lo_of.data(), W * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream);
+    copy_sync(m->dlo_off.p, lo_off.data(), (w + 2) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+    DociBlockArgs& b = m->bargs;
+    b.n = m->n;
+    b.L = L;
+    b.np = np;
+    b.w = w;
+    b.blocks = m->dblk.p;
+    b.nblocks = (uint32_t)blocks.size();
+    b.rank_lo = m->drank_lo.p;
+    b.lo_of = m->dlo_of.p;
+    b.lo_off = m->dlo_off.p;
+    b.binom = c->binom.p;
+    b.diag = sp->diag.p;
+    b.zero = m->zero;
+    m->blk_ready = true;
 }
 
 // Position-major complete FCI rows + beta index tables (kernel 4).
@@ -545,7 +620,7 @@ void sync_ham(cf_crs* m) {
     if (m->version == sp->version) return;
     cf_ctx* c = sp->ham->ctx;
     const HamDev H = sp->ham->dev();
-    if (m->doci_ready) launch_doci_pair_table(m->dargs, H, m->dvk.p, m->dvk32.p, c->stream);
+    if (m->dvk.p) launch_doci_pair_table(m->dargs, H, m->dvk.p, m->dvk32.p, c->stream);
     if (m->has_fargs) m->fargs.H = H;
     if (m->ell_ready) {
         KTimer t(c, "crs_fill");
@@ -824,8 +899,10 @@ void check_shape(const cf_crs* m, const SpmvChoice& ch) {
         CF_REQUIRE(t % 64 == 0 && t >= 64 && t <= 1024, "kernel 5: workgroup size must be a multiple of 64 <= 1024");
         const int u = ch.doci.sub;
         CF_REQUIRE(u == 1 || u == 2 || u == 4 || u == 8, "kernel 5: lanes per row must be 1, 2, 4 or 8");
+    } else if (ch.kernel == 6) {
+        CF_REQUIRE(m->blk_ready, "kernel 6 needs a DOCI matrix");
     } else {
-        CF_REQUIRE(false, "shape kernel must be 1, 3, 4 or 5");
+        CF_REQUIRE(false, "shape kernel must be 1, 3, 4, 5 or 6");
     }
 }
 
@@ -1264,17 +1341,25 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
         m->auto64 = SpmvChoice{4, 0, 0, ell_default_shape(sp->db, 8), {}};
         m->auto32 = SpmvChoice{4, 0, 0, ell_default_shape(sp->db, 4), {}};
     }
-    // DOCI matrices: hop columns (4 bytes per element) + pair tables in LDS
-    // replace the 8/12-byte element stream of the row gather.  Measured on
-    // MI355X: L = 24 0.47 ms against 1.47 (FP64) / 1.08 ms (FP32), L = 20
-    // 31 against 72 / 53 us, L = 18 (48,620 rows) 13.2 against 15.7 us FP64
-    // but 13.3 against 11.6 us FP32; below that the launch dominates and the
-    // row gather is as fast, so smaller matrices build the copy on request
+    // DOCI matrices: the pair elements come from an L x L table in LDS instead
+    // of the 8/12-byte element stream of the row gather.  Measured on MI355X
+    // (FP64 / FP32 values): L = 24 block-staged 0.262 / 0.262 ms, hop columns
+    // 0.472 / 0.477, row gather 1.468 / 1.076; L = 22 68 / 69 against 121 / 125
+    // (hop columns) and 312 / 226 us; L = 20 21.8 / 20.3 against 32.6 / 33.1
+    // and 73 / 52 us; L = 18 (48,620 rows, blocks of <= 126 rows) hop columns
+    // 13.3 against 15.6 / 12.1 us row gather.  Below that the launch dominates
+    // and the row gather is as fast, so smaller matrices build on request.
     if (sp->kind == CF_DOCI && m->n >= 40000) {
-        make_doci_ell(m.get());
-        const SpmvChoice k5{5, 0, 0, {}, doci_default_shape(m->n)};
-        m->auto64 = k5;
-        if (m->n >= 100000) m->auto32 = k5;
+        const int L = sp->ham->L, w = doci_block_w(L);
+        if (m->n >= 100000 && L - w <= 20) {
+            make_doci_blocks(m.get(), w);
+            m->auto64 = m->auto32 = SpmvChoice{6, 0, 0, {}, {}};
+        } else {
+            make_doci_ell(m.get());
+            const SpmvChoice k5{5, 0, 0, {}, doci_default_shape(m->n)};
+            m->auto64 = k5;
+            if (m->n >= 100000) m->auto32 = k5;
+        }
     }
     *out = m.release();
     API_END
@@ -1397,7 +1482,7 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
     CF_REQUIRE(m && bytes && kernel, "NULL argument");
     const uint64_t sval = value_prec == CF_F32 ? 4 : 8;
     const SpmvChoice ch = resolve_spmv(m, value_prec);
-    if (ch.kernel != 4 && ch.kernel != 5) {
+    if (ch.kernel < 4) {
         cf_crs* mm = const_cast<cf_crs*>(m);  // bytes of the CRS for the current Hamiltonian
         mm->sp->ham->ctx->use();
         ensure_crs(mm);
@@ -1409,6 +1494,9 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
     } else if (ch.kernel == 5) {
         // hop columns + string + diagonal + x once + y
         *bytes = m->n * ((uint64_t)m->dargs.np * m->dargs.nv * 4 + 32);
+    } else if (ch.kernel == 6) {
+        // diagonal + x once + y (the neighbour x blocks come back through L2)
+        *bytes = m->n * 24;
     } else {
         *bytes = m->nnz * (sval + 4) + (m->n + 1) * 8 + 16 * m->n;
     }
@@ -1425,6 +1513,10 @@ int cf_crs_set_spmv_shape(cf_crs* m, int value_prec, int kernel, int p1, int p2)
     m->sp->ham->ctx->use();
     sync_ham(m);
     if (kernel == 5 && m->sp->kind == CF_DOCI && !m->doci_ready) make_doci_ell(m);
+    if (kernel == 6 && m->sp->kind == CF_DOCI) {
+        const int w = p1 > 0 ? p1 : doci_block_w(m->sp->ham->L);
+        if (!m->blk_ready || m->bargs.w != w) make_doci_blocks(m, w);
+    }
     check_shape(m, ch);
     (value_prec == CF_F32 ? m->auto32 : m->auto64) = ch;
     m->kernel = 0;
@@ -1442,6 +1534,8 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
     const uint64_t db = m->sp->db;
     ensure_crs(m);
     if (m->sp->kind == CF_DOCI && !m->doci_ready) make_doci_ell(m);
+    if (m->sp->kind == CF_DOCI && !m->blk_ready && m->sp->ham->L - doci_block_w(m->sp->ham->L) <= 20)
+        make_doci_blocks(m, doci_block_w(m->sp->ham->L));
     std::vector<SpmvChoice> cand;
     for (int l : {8, 16, 32, 64}) cand.push_back(make_choice(1, l, 0, db, value_prec));
     for (int l : {4, 8, 16, 32})
@@ -1463,6 +1557,7 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
     if (m->doci_ready)
         for (int t : {128, 256, 512})
             for (int u : {1, 2, 4, 8}) cand.push_back(make_choice(5, t, u, db, value_prec));
+    if (m->blk_ready) cand.push_back(make_choice(6, 0, 0, db, value_prec));
     ensure_staging(c, m->n);
     CF_HIP(hipMemsetAsync(c->hx.p, 0, m->n * sizeof(double), c->stream));
     hipEvent_t e0, e1;
@@ -1496,7 +1591,10 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
         for (size_t i = 0; i < cand.size() && (int)i < cap; ++i) {
             const SpmvChoice& ch = cand[i];
             table[i].kernel = ch.kernel;
-            table[i].p1 = ch.kernel == 4 ? ch.ell.threads : ch.kernel == 5 ? ch.doci.threads : ch.lanes;
+            table[i].p1 = ch.kernel == 4   ? ch.ell.threads
+                          : ch.kernel == 5 ? ch.doci.threads
+                          : ch.kernel == 6 ? m->bargs.w
+                                           : ch.lanes;
             table[i].p2 = ch.kernel == 4 ? ch.ell.vec : ch.kernel == 5 ? ch.doci.sub : ch.unroll;
             table[i].ms = ms[i];
         }
@@ -1521,7 +1619,7 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y) {
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
     API_BEGIN
     CF_REQUIRE(m, "matrix is NULL");
-    CF_REQUIRE(kind >= 0 && kind <= 5, "kind must be 0, 1, 2, 3, 4 or 5");
+    CF_REQUIRE(kind >= 0 && kind <= 6, "kind must be 0, 1, 2, 3, 4, 5 or 6");
     m->sp->ham->ctx->use();
     sync_ham(m);
     if (kind == 5) {
@@ -1533,6 +1631,11 @@ int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
             if (a->kernel != 5) a->doci = doci_default_shape(m->n);
             if (tile_rows > 0) a->doci.threads = tile_rows;
         }
+    }
+    if (kind == 6) {
+        CF_REQUIRE(m->sp->kind == CF_DOCI, "kernel 6 needs a DOCI matrix");
+        const int w = tile_rows > 0 ? tile_rows : doci_block_w(m->sp->ham->L);
+        if (!m->blk_ready || m->bargs.w != w) make_doci_blocks(m, w);
     }
     if (kind == 4) {
         if (tile_rows > 0)

@@ -166,6 +166,226 @@ static void launch_doci(const DociEllArgs& a, DociShape sh, const VT* vk, const 
     CF_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Block-staged product: no stored columns.  Rows sharing the high part hi of
+// their string form the contiguous rank range base(hi, c) + rank_lo(lo), with
+// rank(s) = sum_m C(k_m, m) split at the w-th orbital:
+//   base(hi, c) = sum over the high bits (m = c+1, c+2, ...) of C(w + q, m).
+// A workgroup owns one block (R = C(w, c) rows, every row with the same c, so
+// every loop below has the same trip count in every lane):
+//   low -> low hops:   target in the own block, x staged in LDS once;
+//   high -> high hops: block (hi ^ jh ^ kh, c), same low pattern, so the same
+//                      row index: one coalesced read of the neighbour block;
+//   high -> low hops:  block (hi ^ jh, c + 1), staged in LDS per jh;
+//   low -> high hops:  block (hi | kh, c - 1), staged in LDS per kh.
+// Per product the x blocks pass through L2 about 1 + ch (L - w - ch) times
+// (ch high pairs) plus the staged blocks; HBM sees x, y and the diagonal once.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t doci_block_base(uint32_t hi, int c, int w, const uint64_t* __restrict__ binom) {
+    uint32_t b = 0;
+    int m = c + 1;
+    for (uint32_t t = hi; t; t &= t - 1) {
+        b += (uint32_t)binom[(w + __builtin_ctz(t)) * kBinomCols + m];
+        ++m;
+    }
+    return b;
+}
+
+constexpr int kBlkRows = 1024;  // LDS x block capacity (>= C(12, 6))
+constexpr int kBlkHops = 128;   // high -> high hops per block: ch (nh - ch) <= 100 for nh <= 20
+constexpr int kBlkStages = 32;  // staged blocks per block: nh <= 20
+
+__device__ __forceinline__ int nth_bit(uint32_t v, int k) {  // position of the k-th (0-based) set bit
+    for (int i = 0; i < k; ++i) v &= v - 1;
+    return __builtin_ctz(v);
+}
+
+template <typename VT, int ROWS>
+__global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT* __restrict__ vk,
+                                                      const double* __restrict__ x, double* __restrict__ y) {
+    extern __shared__ double smem_blk[];
+    __shared__ uint32_t hh_base[kBlkHops];
+    __shared__ double hh_val[kBlkHops];
+    __shared__ uint32_t sg_base[kBlkStages], sg_orb[kBlkStages];
+    const int L = a.L, w = a.w, nh = L - w;
+    const uint32_t W = 1u << w, wmask = W - 1;
+    double* xo = smem_blk;                 // own block
+    double* sbuf = smem_blk + kBlkRows;    // two staged neighbour blocks
+    VT* tab = reinterpret_cast<VT*>(smem_blk + 3 * kBlkRows);
+    uint16_t* rlo = reinterpret_cast<uint16_t*>(tab + L * L);
+    const int tid = threadIdx.x;
+    const DociBlock blk = a.blocks[blockIdx.x];
+    const uint32_t R = blk.R, base = blk.base, hi = blk.hi;
+    const int c = (int)blk.c;
+    const uint32_t hv = ~hi & ((1u << nh) - 1);
+    const int ch = __builtin_popcount(hi), cv = nh - ch;
+    const int nhh = ch * cv;
+    // stages: high -> low (block (hi ^ jh, c + 1)) for every occupied high
+    // orbital, then low -> high (block (hi | kh, c - 1)) for every empty one
+    const int n_hl = c < w ? ch : 0, n_lh = c > 0 ? cv : 0, nst = n_hl + n_lh;
+    const uint32_t R_hl = c < w ? a.lo_off[c + 2] - a.lo_off[c + 1] : 0u;
+    const uint32_t R_lh = c > 0 ? a.lo_off[c] - a.lo_off[c - 1] : 0u;
+    for (int t = tid; t < L * L; t += 256) tab[t] = vk[t];
+    for (uint32_t t = tid; t < W; t += 256) rlo[t] = a.rank_lo[t];
+    for (uint32_t t = tid; t < R; t += 256) xo[t] = x[base + t];
+    if (tid < nhh) {  // neighbour blocks of the high -> high hops, one per thread
+        const int jh = nth_bit(hi, tid / cv), kh = nth_bit(hv, tid % cv);
+        hh_base[tid] = doci_block_base(hi ^ (1u << jh) ^ (1u << kh), c, w, a.binom);
+        hh_val[tid] = (double)vk[(w + jh) * L + (w + kh)];
+    }
+    if (tid < nst) {
+        if (tid < n_hl) {
+            const int jh = nth_bit(hi, tid);
+            sg_base[tid] = doci_block_base(hi ^ (1u << jh), c + 1, w, a.binom);
+            sg_orb[tid] = (uint32_t)(w + jh);
+        } else {
+            const int kh = nth_bit(hv, tid - n_hl);
+            sg_base[tid] = doci_block_base(hi | (1u << kh), c - 1, w, a.binom);
+            sg_orb[tid] = (uint32_t)(w + kh);
+        }
+    }
+    const uint16_t* lof = a.lo_of + a.lo_off[c];
+    uint32_t lo[ROWS];
+    double acc[ROWS];
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+        const uint32_t r = tid + q * 256;
+        lo[q] = r < R ? lof[r] : 0u;
+        acc[q] = 0.0;
+    }
+    __syncthreads();
+    // first staged block into registers while the own block is worked on
+    double pre[4];
+    auto fetch = [&](int sidx) {
+        const uint32_t Rs = sidx < n_hl ? R_hl : R_lh, b = sg_base[sidx];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t t = tid + i * 256;
+            pre[i] = t < Rs ? x[b + t] : 0.0;
+        }
+    };
+    if (nst) fetch(0);
+    // diagonal and low -> low hops
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+        const uint32_t r = tid + q * 256;
+        if (r < R) {
+            double d = a.diag[base + r];
+            if (!(fabs(d) > a.zero)) d = 0.0;
+            double s = (double)(VT)d * xo[r];
+            const uint32_t l = lo[q], vl = ~l & wmask;
+            for (uint32_t o = l; o; o &= o - 1) {
+                const int j = __builtin_ctz(o);
+                const uint32_t lj = l ^ (1u << j);
+                const VT* trow = tab + j * L;
+                for (uint32_t v = vl; v; v &= v - 1) {
+                    const int k = __builtin_ctz(v);
+                    s += (double)trow[k] * xo[rlo[lj | (1u << k)]];
+                }
+            }
+            acc[q] = s;
+        }
+    }
+    // high -> high hops: the neighbour block at the same row index, four
+    // hops' loads in flight per row
+    {
+        int t = 0;
+        for (; t + 4 <= nhh; t += 4) {
+            const uint32_t b0 = hh_base[t], b1 = hh_base[t + 1], b2 = hh_base[t + 2], b3 = hh_base[t + 3];
+            const double e0 = hh_val[t], e1 = hh_val[t + 1], e2 = hh_val[t + 2], e3 = hh_val[t + 3];
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) {
+                const uint32_t r = tid + q * 256;
+                if (r < R) {
+                    const double x0 = x[b0 + r], x1 = x[b1 + r], x2 = x[b2 + r], x3 = x[b3 + r];
+                    acc[q] += e0 * x0 + e1 * x1 + e2 * x2 + e3 * x3;
+                }
+            }
+        }
+        for (; t < nhh; ++t) {
+            const uint32_t b0 = hh_base[t];
+            const double e0 = hh_val[t];
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) {
+                const uint32_t r = tid + q * 256;
+                if (r < R) acc[q] += e0 * x[b0 + r];
+            }
+        }
+    }
+    // staged hops across the split, double-buffered: stage s+1 loads into
+    // registers while stage s is read from LDS; one barrier per stage
+    for (int sidx = 0; sidx < nst; ++sidx) {
+        double* buf = sbuf + (sidx & 1) * kBlkRows;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t t = tid + i * 256;
+            if (t < kBlkRows) buf[t] = pre[i];
+        }
+        __syncthreads();
+        if (sidx + 1 < nst) fetch(sidx + 1);
+        const int orb = (int)sg_orb[sidx];
+        const VT* trow = tab + orb * L;  // pair elements with the high orbital (symmetric table)
+        if (sidx < n_hl) {  // high orbital orb -> empty low orbital k
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) {
+                const uint32_t r = tid + q * 256;
+                if (r < R) {
+                    const uint32_t l = lo[q];
+                    double s = 0.0;
+                    for (uint32_t v = ~l & wmask; v; v &= v - 1) {
+                        const int k = __builtin_ctz(v);
+                        s += (double)trow[k] * buf[rlo[l | (1u << k)]];
+                    }
+                    acc[q] += s;
+                }
+            }
+        } else {  // occupied low orbital j -> empty high orbital orb
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) {
+                const uint32_t r = tid + q * 256;
+                if (r < R) {
+                    const uint32_t l = lo[q];
+                    double s = 0.0;
+                    for (uint32_t o = l; o; o &= o - 1) {
+                        const int j = __builtin_ctz(o);
+                        s += (double)trow[j] * buf[rlo[l ^ (1u << j)]];
+                    }
+                    acc[q] += s;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < ROWS; ++q) {
+        const uint32_t r = tid + q * 256;
+        if (r < R) y[base + r] = acc[q];
+    }
+}
+
+int doci_block_w(int L) { return L / 2 < kDociMaxW ? L / 2 : kDociMaxW; }
+
+template <typename VT>
+static void launch_doci_blk(const DociBlockArgs& a, const VT* vk, const double* x, double* y, hipStream_t s) {
+    if (!a.nblocks) return;
+    CF_REQUIRE(a.w >= 1 && a.w <= kDociMaxW && a.L - a.w >= 1 && a.L - a.w <= 20,
+               "block-staged DOCI SpMV: 1 <= w <= 12 low orbitals and 1..20 high orbitals");
+    const size_t shm = 3 * kBlkRows * sizeof(double) + (size_t)a.L * a.L * sizeof(VT) +
+                       ((size_t)1 << a.w) * sizeof(uint16_t);
+    const uint64_t maxR = binom_host(a.w, a.w / 2);
+    if (maxR <= 256) k_spmv_doci_blk<VT, 1><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
+    else if (maxR <= 512) k_spmv_doci_blk<VT, 2><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
+    else k_spmv_doci_blk<VT, 4><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
+    CF_HIP(hipGetLastError());
+}
+
+void launch_spmv_doci_blk_f64(const DociBlockArgs& a, const double* vk, const double* x, double* y, hipStream_t s) {
+    launch_doci_blk<double>(a, vk, x, y, s);
+}
+
+void launch_spmv_doci_blk_f32(const DociBlockArgs& a, const float* vk, const double* x, double* y, hipStream_t s) {
+    launch_doci_blk<float>(a, vk, x, y, s);
+}
+
 void launch_spmv_doci_f64(const DociEllArgs& a, DociShape sh, const double* vk, const double* x, double* y,
                           hipStream_t s) {
     launch_doci<double>(a, sh, vk, x, y, s);

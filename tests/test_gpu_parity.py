@@ -518,10 +518,11 @@ DOCI_HOP_CASES = {
 @pytest.mark.parametrize("case", ["doci_beh2", "doci_L10_5", "doci_redbcs_L8"] + list(DOCI_HOP_CASES))
 def test_doci_hop_spmv_matches_scalar_gather(case):
     """Kernel 5 (position-major pair-hop columns, pair elements from an LDS
-    table) against the scalar row gather of the stored CRS, FP64 and FP32
-    values, 1-8 lanes per row (more lanes than pairs included): BeH2 (D2h
-    zeros dropped by the build), one pair, one hole, and a threshold that
-    drops half of the pair elements."""
+    table) and kernel 6 (block-staged, no columns; several low/high splits)
+    against the scalar row gather of the stored CRS, FP64 and FP32 values,
+    1-8 lanes per row (more lanes than pairs included): BeH2 (D2h zeros
+    dropped by the build), one pair, one hole, and a threshold that drops
+    half of the pair elements."""
     say("doci hop spmv", case)
     zero = 1e-14
     if case in CASES:
@@ -542,9 +543,11 @@ def test_doci_hop_spmv_matches_scalar_gather(case):
     n = cim.get_dim()
     if case.endswith("dropped"):
         assert m.info()[1] < n * (1 + na * (len(h) - na)) * 0.75  # elements really dropped
-    # automatic choice: the hop kernel from 40,000 rows (FP64) / 100,000 (FP32)
-    assert (m.spmv_bytes(CA.CF_F64)[0] == 5) == (n >= 40000)
-    assert (m.spmv_bytes(CA.CF_F32)[0] == 5) == (n >= 100000)
+    # automatic choice: block-staged from 100,000 rows, hop columns from
+    # 40,000 (FP64 values), the CRS row gather below
+    k64, k32 = m.spmv_bytes(CA.CF_F64)[0], m.spmv_bytes(CA.CF_F32)[0]
+    assert k64 == (6 if n >= 100000 else 5 if n >= 40000 else k64) and (n >= 40000 or k64 < 4)
+    assert k32 == (6 if n >= 100000 else k32) and (n >= 100000 or k32 < 4)
     x = np.random.default_rng(9).standard_normal(n)
     for prec in (CA.CF_F64, CA.CF_F32):
         m.set_spmv_kernel(1)
@@ -556,10 +559,16 @@ def test_doci_hop_spmv_matches_scalar_gather(case):
             m.set_spmv_shape(prec, 5, nt, sub)
             y2 = m.mvprod(x, prec)
             assert np.max(np.abs(y2 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec, nt, sub)
+        for w in sorted({max(1, len(h) // 2 - 1), len(h) // 2, min(12, len(h) - 1)}):
+            m.set_spmv_shape(prec, 6, w, 0)  # block-staged, w low orbitals
+            y3 = m.mvprod(x, prec)
+            assert np.max(np.abs(y3 - y1)) <= 1e-13 * np.max(np.abs(y1)), (case, prec, "blk", w)
     with pytest.raises(CA.CIFlowError):
         m.set_spmv_shape(CA.CF_F64, 5, 100, 1)
     with pytest.raises(CA.CIFlowError):
         m.set_spmv_shape(CA.CF_F64, 5, 256, 3)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 6, 13, 0)
 
 
 def test_doci_hop_kernel_rejected_for_fci():

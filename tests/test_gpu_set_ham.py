@@ -33,11 +33,13 @@ def same_rows(m1, m2):
     assert np.array_equal(v1, v2)
 
 
-@pytest.mark.parametrize("kind,L,na,nb", [("DOCI", 10, 5, 5), ("DOCI", 18, 9, 9), ("FCI", 9, 4, 4),
+@pytest.mark.parametrize("kind,L,na,nb", [("DOCI", 10, 5, 5), ("DOCI", 18, 9, 9), ("DOCI", 20, 10, 10),
+                                          ("FCI", 9, 4, 4),
                                           ("FCI", 11, 5, 5), ("FCI", 8, 4, 3)])
 def test_set_ham_equals_fresh_build(kind, L, na, nb):
     """Resident matrix after set_ham == a matrix built for the new integrals:
-    position-major copies (FCI L=11 kernel 4, DOCI L=18 kernel 5) refilled,
+    position-major copies (FCI L=11 kernel 4, DOCI L=18 kernel 5, DOCI L=20
+    kernel 6) refilled,
     CRS rebuilt on first read, FP64 and FP32 values."""
     say("set_ham", kind, L, na, nb)
     cls = CA.FCI if kind == "FCI" else CA.DOCI
@@ -85,7 +87,7 @@ def test_set_ham_pattern_change_rebuilds_crs():
     assert cim.get_mat().info()[1] < nnz_a
     same_rows(cim.get_mat(), fresh.get_mat())
     x = np.random.default_rng(1).standard_normal(cim.get_dim())
-    for kern in (5, 1, 3):
+    for kern in (5, 6, 1, 3):
         cim.get_mat().set_spmv_kernel(kern)
         y = cim.get_mat().mvprod(x)
         yf = fresh.get_mat().mvprod(x)
