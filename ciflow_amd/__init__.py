@@ -415,7 +415,7 @@ class SparseMatrix_CRS_Sym:
         return self._export(False, r0, self.getn() if r1 is None else r1)
 
     def spmv_bytes(self, precision: int = CF_F64):
-        """(kernel id 1-5, algorithmic HBM bytes per product) of the current choice."""
+        """(kernel id 1-6, algorithmic HBM bytes per product) of the current choice."""
         b = C.c_uint64()
         k = C.c_int()
         _check(lib().cf_crs_spmv_bytes(self.ptr, precision, C.byref(b), C.byref(k)))
@@ -424,7 +424,8 @@ class SparseMatrix_CRS_Sym:
     def set_spmv_kernel(self, kind: int, tile_rows: int = 0):
         """0 automatic, 1 plain row gather, 2 FCI alpha-block staged, 3 vectorised
         row gather, 4 FCI position-major rows, 5 DOCI pair-hop columns
-        (tile_rows = workgroup size for 4 and 5)."""
+        (tile_rows = workgroup size for 4 and 5), 6 DOCI block-staged
+        (tile_rows = low orbitals)."""
         _check(lib().cf_crs_set_spmv_kernel(self.ptr, kind, tile_rows))
 
     def set_spmv_shape(self, precision: int, kernel: int, p1: int, p2: int = 0):

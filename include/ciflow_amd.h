@@ -173,11 +173,15 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * with LDS-staged x blocks (FCI, d_beta <= 8192; builds an extra value copy;
  * tile_rows > 0 = workgroup size, a multiple of 64 <= 1024, for both value
  * precisions), 5 = DOCI position-major pair-hop columns with the pair
- * elements in LDS (DOCI matrices, built with the matrix; 4 bytes per
- * element; tile_rows > 0 = workgroup size). */
+ * elements in LDS (DOCI matrices; 4 bytes per element; tile_rows > 0 =
+ * workgroup size), 6 = DOCI block-staged (no stored columns; x blocks of the
+ * rows sharing their high orbitals and the pair elements in LDS; tile_rows >
+ * 0 = low orbitals w, 1..12, with 1..20 high orbitals).  The DOCI copies are
+ * built with matrices of >= 40,000 rows, else on request. */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
-/* Kernel the next product with value_prec would run (1-5) and its algorithmic
- * HBM bytes (matrix stream + x + y), the roofline numerator. */
+/* Kernel the next product with value_prec would run (1-6) and its algorithmic
+ * HBM bytes (matrix stream + x + y; kernel 6: diagonal + x + y), the roofline
+ * numerator. */
 int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* kernel);
 /* Launch shape of one SpMV kernel for one value precision (no reference
  * counterpart; the reference's mvprod has no launch shape).  kernel 1: p1 =
@@ -186,7 +190,7 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
  * <= 1024, 0 = default), p2 = beta strings per thread (1, 2 or 4, dividing
  * d_beta; 0 = the single-buffered kernel with p1 in 256/512/1024); kernel 5:
  * p1 = workgroup size (multiple of 64, <= 1024, 0 = 256), p2 = lanes per row
- * (1, 2, 4 or 8).  The shape
+ * (1, 2, 4 or 8); kernel 6: p1 = low orbitals w (0 = min(L/2, 12)).  The shape
  * becomes the automatic choice for value_prec and the matrix returns to
  * automatic selection (kernel 0). */
 int cf_crs_set_spmv_shape(cf_crs* m, int value_prec, int kernel, int p1, int p2);
