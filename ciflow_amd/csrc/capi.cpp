@@ -1345,20 +1345,22 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     // of the 8/12-byte element stream of the row gather.  Measured on MI355X
     // (FP64 / FP32 values): L = 24 block-staged 0.262 / 0.262 ms, hop columns
     // 0.472 / 0.477, row gather 1.468 / 1.076; L = 22 68 / 69 against 121 / 125
-    // (hop columns) and 312 / 226 us; L = 20 21.8 / 20.3 against 32.6 / 33.1
-    // and 73 / 52 us; L = 18 (48,620 rows, blocks of <= 126 rows) hop columns
-    // 13.3 against 15.6 / 12.1 us row gather.  Below that the launch dominates
-    // and the row gather is as fast, so smaller matrices build on request.
+    // (hop columns) and 312 / 226 us; L = 20 18.7 / 19.1 against 33.1 / 33.8
+    // and 73 / 52 us; L = 19 (92,378 rows) 15.8 / 15.7 against 21.1 / 21.5 and
+    // 31.7 / 23.7 us; L = 18 (48,620 rows) hop columns 13.2 against 14.5
+    // (block-staged) and 15.7 / 11.5 us (row gather).  Below that the launch
+    // dominates and the row gather is as fast, so smaller matrices build on
+    // request.
     if (sp->kind == CF_DOCI && m->n >= 40000) {
         const int L = sp->ham->L, w = doci_block_w(L);
-        if (m->n >= 100000 && L - w <= 20) {
+        if (m->n >= 60000 && L - w <= 20) {
             make_doci_blocks(m.get(), w);
             m->auto64 = m->auto32 = SpmvChoice{6, 0, 0, {}, {}};
         } else {
             make_doci_ell(m.get());
             const SpmvChoice k5{5, 0, 0, {}, doci_default_shape(m->n)};
             m->auto64 = k5;
-            if (m->n >= 100000) m->auto32 = k5;
+
         }
     }
     *out = m.release();

@@ -200,8 +200,8 @@ __device__ __forceinline__ int nth_bit(uint32_t v, int k) {  // position of the 
     return __builtin_ctz(v);
 }
 
-template <typename VT, int ROWS>
-__global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT* __restrict__ vk,
+template <typename VT, int ROWS, int T>
+__global__ void __launch_bounds__(T) k_spmv_doci_blk(DociBlockArgs a, const VT* __restrict__ vk,
                                                       const double* __restrict__ x, double* __restrict__ y) {
     extern __shared__ double smem_blk[];
     __shared__ uint32_t hh_base[kBlkHops];
@@ -225,9 +225,9 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
     const int n_hl = c < w ? ch : 0, n_lh = c > 0 ? cv : 0, nst = n_hl + n_lh;
     const uint32_t R_hl = c < w ? a.lo_off[c + 2] - a.lo_off[c + 1] : 0u;
     const uint32_t R_lh = c > 0 ? a.lo_off[c] - a.lo_off[c - 1] : 0u;
-    for (int t = tid; t < L * L; t += 256) tab[t] = vk[t];
-    for (uint32_t t = tid; t < W; t += 256) rlo[t] = a.rank_lo[t];
-    for (uint32_t t = tid; t < R; t += 256) xo[t] = x[base + t];
+    for (int t = tid; t < L * L; t += T) tab[t] = vk[t];
+    for (uint32_t t = tid; t < W; t += T) rlo[t] = a.rank_lo[t];
+    for (uint32_t t = tid; t < R; t += T) xo[t] = x[base + t];
     if (tid < nhh) {  // neighbour blocks of the high -> high hops, one per thread
         const int jh = nth_bit(hi, tid / cv), kh = nth_bit(hv, tid % cv);
         hh_base[tid] = doci_block_base(hi ^ (1u << jh) ^ (1u << kh), c, w, a.binom);
@@ -249,18 +249,18 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
     double acc[ROWS];
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-        const uint32_t r = tid + q * 256;
+        const uint32_t r = tid + q * T;
         lo[q] = r < R ? lof[r] : 0u;
         acc[q] = 0.0;
     }
     __syncthreads();
     // first staged block into registers while the own block is worked on
-    double pre[4];
+    double pre[ROWS];
     auto fetch = [&](int sidx) {
         const uint32_t Rs = sidx < n_hl ? R_hl : R_lh, b = sg_base[sidx];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t t = tid + i * 256;
+        for (int i = 0; i < ROWS; ++i) {
+            const uint32_t t = tid + i * T;
             pre[i] = t < Rs ? x[b + t] : 0.0;
         }
     };
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
     // diagonal and low -> low hops
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-        const uint32_t r = tid + q * 256;
+        const uint32_t r = tid + q * T;
         if (r < R) {
             double d = a.diag[base + r];
             if (!(fabs(d) > a.zero)) d = 0.0;
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
             const double e0 = hh_val[t], e1 = hh_val[t + 1], e2 = hh_val[t + 2], e3 = hh_val[t + 3];
 #pragma unroll
             for (int q = 0; q < ROWS; ++q) {
-                const uint32_t r = tid + q * 256;
+                const uint32_t r = tid + q * T;
                 if (r < R) {
                     const double x0 = x[b0 + r], x1 = x[b1 + r], x2 = x[b2 + r], x3 = x[b3 + r];
                     acc[q] += e0 * x0 + e1 * x1 + e2 * x2 + e3 * x3;
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
             const double e0 = hh_val[t];
 #pragma unroll
             for (int q = 0; q < ROWS; ++q) {
-                const uint32_t r = tid + q * 256;
+                const uint32_t r = tid + q * T;
                 if (r < R) acc[q] += e0 * x[b0 + r];
             }
         }
@@ -317,10 +317,7 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
     for (int sidx = 0; sidx < nst; ++sidx) {
         double* buf = sbuf + (sidx & 1) * kBlkRows;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const uint32_t t = tid + i * 256;
-            if (t < kBlkRows) buf[t] = pre[i];
-        }
+        for (int i = 0; i < ROWS; ++i) buf[tid + i * T] = pre[i];
         __syncthreads();
         if (sidx + 1 < nst) fetch(sidx + 1);
         const int orb = (int)sg_orb[sidx];
@@ -328,7 +325,7 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
         if (sidx < n_hl) {  // high orbital orb -> empty low orbital k
 #pragma unroll
             for (int q = 0; q < ROWS; ++q) {
-                const uint32_t r = tid + q * 256;
+                const uint32_t r = tid + q * T;
                 if (r < R) {
                     const uint32_t l = lo[q];
                     double s = 0.0;
@@ -342,7 +339,7 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
         } else {  // occupied low orbital j -> empty high orbital orb
 #pragma unroll
             for (int q = 0; q < ROWS; ++q) {
-                const uint32_t r = tid + q * 256;
+                const uint32_t r = tid + q * T;
                 if (r < R) {
                     const uint32_t l = lo[q];
                     double s = 0.0;
@@ -357,7 +354,7 @@ __global__ void __launch_bounds__(256) k_spmv_doci_blk(DociBlockArgs a, const VT
     }
 #pragma unroll
     for (int q = 0; q < ROWS; ++q) {
-        const uint32_t r = tid + q * 256;
+        const uint32_t r = tid + q * T;
         if (r < R) y[base + r] = acc[q];
     }
 }
@@ -371,10 +368,15 @@ static void launch_doci_blk(const DociBlockArgs& a, const VT* vk, const double* 
                "block-staged DOCI SpMV: 1 <= w <= 12 low orbitals and 1..20 high orbitals");
     const size_t shm = 3 * kBlkRows * sizeof(double) + (size_t)a.L * a.L * sizeof(VT) +
                        ((size_t)1 << a.w) * sizeof(uint16_t);
+    // rows of the largest block over the workgroup (T * ROWS >= C(w, w/2)); the
+    // neighbour-block bases are computed one per thread (T >= ch (nh - ch))
     const uint64_t maxR = binom_host(a.w, a.w / 2);
-    if (maxR <= 256) k_spmv_doci_blk<VT, 1><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
-    else if (maxR <= 512) k_spmv_doci_blk<VT, 2><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
-    else k_spmv_doci_blk<VT, 4><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
+    const int nh = a.L - a.w, maxhh = (nh / 2) * ((nh + 1) / 2);
+    if (maxR <= 64 && maxhh <= 64) k_spmv_doci_blk<VT, 1, 64><<<a.nblocks, 64, shm, s>>>(a, vk, x, y);
+    else if (maxR <= 128 && maxhh <= 128) k_spmv_doci_blk<VT, 1, 128><<<a.nblocks, 128, shm, s>>>(a, vk, x, y);
+    else if (maxR <= 256) k_spmv_doci_blk<VT, 1, 256><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
+    else if (maxR <= 512) k_spmv_doci_blk<VT, 2, 256><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
+    else k_spmv_doci_blk<VT, 4, 256><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
     CF_HIP(hipGetLastError());
 }
 

@@ -543,11 +543,11 @@ def test_doci_hop_spmv_matches_scalar_gather(case):
     n = cim.get_dim()
     if case.endswith("dropped"):
         assert m.info()[1] < n * (1 + na * (len(h) - na)) * 0.75  # elements really dropped
-    # automatic choice: block-staged from 100,000 rows, hop columns from
+    # automatic choice: block-staged from 60,000 rows, hop columns from
     # 40,000 (FP64 values), the CRS row gather below
     k64, k32 = m.spmv_bytes(CA.CF_F64)[0], m.spmv_bytes(CA.CF_F32)[0]
-    assert k64 == (6 if n >= 100000 else 5 if n >= 40000 else k64) and (n >= 40000 or k64 < 4)
-    assert k32 == (6 if n >= 100000 else k32) and (n >= 100000 or k32 < 4)
+    assert k64 == (6 if n >= 60000 else 5 if n >= 40000 else k64) and (n >= 40000 or k64 < 4)
+    assert k32 == (6 if n >= 60000 else k32) and (n >= 60000 or k32 < 4)
     x = np.random.default_rng(9).standard_normal(n)
     for prec in (CA.CF_F64, CA.CF_F32):
         m.set_spmv_kernel(1)
