@@ -56,6 +56,8 @@ def parse():
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-fp64", action="store_true", help="direct16: skip the FP64 solve")
     p.add_argument("--cpu-lines", type=int, default=6, help="alpha lines in the FCI CPU-baseline sample")
+    p.add_argument("--solver", choices=["auto", "lanczos", "davidson"], default="auto",
+                   help="stream: served solver (auto: Davidson for FCI and DOCI >= 150k determinants)")
     return p.parse_args()
 
 
@@ -230,7 +232,8 @@ def bench_stream(a, CA):
     from ciflow_amd import stream as S
     K = a.steps if a.steps is not None else 4096
     W = a.warmup if a.warmup is not None else 64
-    served = dict(precision=CA.CF_F32, check_every=2, tol_energy=1e-9, direct_fci=True)
+    method = {"auto": CA.CF_SOLVER_AUTO, "lanczos": CA.CF_LANCZOS, "davidson": CA.CF_DAVIDSON}[a.solver]
+    served = dict(precision=CA.CF_F32, check_every=2, tol_energy=1e-9, direct_fci=True, method=method)
     reqs = S.make_stream(K, rate=0.0)
     warm = S.make_stream(W, rate=0.0, first_seed=500000, mix_seed=99) if W > 0 else []
     t0 = time.perf_counter()
@@ -297,6 +300,8 @@ def bench_stream(a, CA):
         "config": {"workload": ("served stream of rotated-integral ground-state solves: 70% DOCI L~U[12,20], "
                                 "30% FCI L~U[8,11] (on-the-fly sigma), half filling, seeds 1000+k, U=exp(K) "
                                 "K~N(0,0.05^2), FP32 values + FP64 polish, Kato-Temple energy stop 1e-9 Eh"),
+                   "solver": {"auto": "Davidson for FCI and DOCI >= 150,000 determinants, Lanczos for smaller DOCI",
+                              "lanczos": "thick-restart Lanczos", "davidson": "Davidson"}[a.solver],
                    "requests": K, "doci": int((kinds == 1).sum()), "fci": int((kinds == 0).sum()),
                    "slots": a.slots, "closed_batch": True,
                    "inputs": "h, V, U copied from host memory at submission (1.3 MB per L=20 request)"},

@@ -28,6 +28,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "ciflow_amd.h")
 CF_OK, CF_ERR_ARG, CF_ERR_HIP, CF_ERR_NOMEM, CF_ERR_NOCONV, CF_ERR_UNSUPPORTED, CF_ERR_STATE = range(7)
 CF_FCI, CF_DOCI, CF_DETS = 0, 1, 2
 CF_F64, CF_F32 = 0, 1
+CF_LANCZOS, CF_DAVIDSON, CF_SOLVER_AUTO = 0, 1, 2
 
 _lib = None
 _exiting = False
@@ -83,7 +84,8 @@ class TuneEntry(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("ncv", C.c_int), ("maxrestart", C.c_int), ("tol", C.c_double), ("precision", C.c_int),
                 ("tol_f32", C.c_double), ("seed", C.c_uint64), ("check_every", C.c_int),
-                ("tol_energy", C.c_double), ("progress", PROGRESS_FN), ("progress_user", C.c_void_p)]
+                ("tol_energy", C.c_double), ("progress", PROGRESS_FN), ("progress_user", C.c_void_p),
+                ("method", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -97,7 +99,8 @@ class SolveInfo(C.Structure):
 class Request(C.Structure):
     _fields_ = [("tag", C.c_uint64), ("kind", C.c_int), ("L", C.c_int), ("nup", C.c_int), ("ndown", C.c_int),
                 ("h", dp), ("V", dp), ("U", dp), ("econst", C.c_double), ("precision", C.c_int), ("ncv", C.c_int),
-                ("tol", C.c_double), ("check_every", C.c_int), ("tol_energy", C.c_double), ("direct", C.c_int)]
+                ("tol", C.c_double), ("check_every", C.c_int), ("tol_energy", C.c_double), ("direct", C.c_int),
+                ("method", C.c_int)]
 
 
 class Result(C.Structure):
@@ -780,28 +783,28 @@ class SolveServer(_Handle):
 
     @staticmethod
     def _request(tag, kind, h, V, nup, ndown, U=None, econst=0.0, precision=CF_F64, ncv=0, tol=0.0, check_every=0,
-                 tol_energy=0.0, direct=False):
+                 tol_energy=0.0, direct=False, method=0):
         h = np.ascontiguousarray(h, dtype=np.float64)
         V = np.ascontiguousarray(V, dtype=np.float64)
         U = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
         L = h.shape[0]
         assert h.shape == (L, L) and V.shape == (L, L, L, L) and (U is None or U.shape == (L, L))
         r = Request(tag, kind, L, nup, ndown, _d(h), _d(V), _d(U) if U is not None else None, econst, precision,
-                    ncv, tol, check_every, tol_energy, int(bool(direct)))
+                    ncv, tol, check_every, tol_energy, int(bool(direct)), int(method))
         return r, (h, V, U)
 
     def submit(self, tag: int, kind: int, h, V, nup: int, ndown: int, U=None, econst: float = 0.0,
                precision: int = CF_F64, ncv: int = 0, tol: float = 0.0, check_every: int = 0,
-               tol_energy: float = 0.0, direct: bool = False):
+               tol_energy: float = 0.0, direct: bool = False, method: int = 0):
         r, keep = self._request(tag, kind, h, V, nup, ndown, U, econst, precision, ncv, tol, check_every,
-                                tol_energy, direct)
+                                tol_energy, direct, method)
         _check(lib().cf_server_submit(self.ptr, C.byref(r)))  # arrays are copied inside
 
     @staticmethod
     def footprint(kind: int, L: int, nup: int, ndown: int, precision: int = CF_F64, ncv: int = 0,
                   direct: bool = False) -> int:
         z = np.zeros(1)
-        r = Request(0, kind, L, nup, ndown, _d(z), _d(z), None, 0.0, precision, ncv, 0.0, 0, 0.0, int(bool(direct)))
+        r = Request(0, kind, L, nup, ndown, _d(z), _d(z), None, 0.0, precision, ncv, 0.0, 0, 0.0, int(bool(direct)), 0)
         b = C.c_uint64()
         _check(lib().cf_request_footprint(C.byref(r), C.byref(b)))
         return b.value

@@ -1684,6 +1684,7 @@ void cf_solve_opts_default(cf_solve_opts* o) {
     o->tol_energy = 0.0;
     o->progress = nullptr;
     o->progress_user = nullptr;
+    o->method = CF_LANCZOS;
 }
 
 }  // extern "C"
@@ -1692,6 +1693,184 @@ namespace {
 // Shared driver of cf_solve / cf_solve_direct: FP64 thick-restart Lanczos, or
 // an FP32-operator phase followed by the FP64 phase started from its Ritz
 // vector.  The eigenvector stays resident in sp->vec.
+// Preconditioned Davidson for the lowest eigenpair (E. R. Davidson, J. Comput.
+// Phys. 17 (1975) 87): basis V and its image S = H V in HBM, the projected
+// matrix G = V^T S on the host.  Each iteration: Ritz pair (theta, y) of G,
+// x = V y and H x = S y in one pass, r = H x - theta x, correction
+// t_i = r_i / (theta - H_ii), two Gram-Schmidt passes, one product.  A full
+// basis restarts to its `keep` lowest Ritz vectors (V and S rotated by the
+// same MFMA kernel as the Lanczos restart).
+struct Davidson {
+    cf_ctx* c;
+    uint64_t n;
+    int M, keep;
+    const double* diag;
+    DBuf<double> V, S, x, hx, t, partial, dg, dy, Yd;
+    int nb;
+    double* hbuf;
+    std::vector<double> G;  // M x M, row-major
+    int m = 0;              // basis size
+    int napply = 0, nrestart = 0;
+    std::function<void(const double*, double*)> op;
+
+    Davidson(cf_ctx* ctx, uint64_t n_, int maxb, const double* d) : c(ctx), n(n_), diag(d) {
+        M = (int)std::min<uint64_t>((uint64_t)maxb, n);
+        keep = std::max(1, std::min(4, M - 1));
+        V.alloc((size_t)M * n);
+        S.alloc((size_t)M * n);
+        x.alloc(n);
+        hx.alloc(n);
+        t.alloc(n);
+        nb = multi_dot_blocks(n);
+        partial.alloc((size_t)64 * nb);
+        dg.alloc(64);
+        dy.alloc(64);
+        Yd.alloc(64 * 64);
+        hbuf = c->pinned;
+        G.assign((size_t)M * M, 0.0);
+    }
+    double* v(int i) { return V.p + (size_t)i * n; }
+    double* s(int i) { return S.p + (size_t)i * n; }
+    void dots(int k, const double* base, const double* w, double* out) {
+        launch_multi_dot(n, k, base, w, partial.p, nb, c->stream);
+        launch_reduce_partials(k, nb, partial.p, out, c->stream);
+    }
+    // t <- (t - V V^T t) twice, then its squared norm into dg[63]
+    void orthogonalize_t() {
+        for (int pass = 0; pass < 2; ++pass) {
+            dots(m, V.p, t.p, dg.p);
+            launch_multi_axpy(n, m, V.p, dg.p, t.p, c->stream);
+        }
+        dots(1, t.p, t.p, dg.p + 63);
+    }
+    void add_column() {
+        op(v(m), s(m));
+        ++napply;
+        dots(m + 1, V.p, s(m), dg.p);
+        CF_HIP(hipMemcpyAsync(hbuf, dg.p, (m + 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipStreamSynchronize(c->stream));
+        for (int i = 0; i <= m; ++i) G[(size_t)i * M + m] = G[(size_t)m * M + i] = hbuf[i];
+        ++m;
+    }
+    // basis = the normalised vector in t
+    void start_from_t() {
+        m = 0;
+        dots(1, t.p, t.p, dg.p + 63);
+        CF_HIP(hipMemcpyAsync(hbuf, dg.p + 63, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipStreamSynchronize(c->stream));
+        CF_REQUIRE(hbuf[0] > 0.0, "Davidson start vector is zero");
+        launch_scale_copy(n, t.p, 1.0 / std::sqrt(hbuf[0]), v(0), c->stream);
+        add_column();
+    }
+    // Iterates to convergence; x holds the Ritz vector (|x| = 1) on return.
+    // Returns 0 converged, 1 iteration limit, 2 stagnation.  One host wait per
+    // iteration: the next basis vector and its product are formed before the
+    // residual of the current Ritz vector is read back, so a converged solve
+    // ends with one product that is not used.  The first `krylov` expansions
+    // take the plain residual (a Krylov step) instead of the preconditioned
+    // correction.
+    int run(int maxit, int krylov, double tol, double tol_energy,
+            const std::function<void(int, double, double)>& progress, double* theta_out, double* res_out) {
+        std::vector<double> A, th, Y;
+        double* ys = hbuf + 256;  // staged Ritz coefficients (<= 24)
+        double* Ys = hbuf + 320;  // staged restart rotation (<= 24 x 24)
+        for (int it = 0;; ++it) {
+            A.assign((size_t)m * m, 0.0);
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) A[(size_t)i * m + j] = G[(size_t)i * M + j];
+            sym_eig(m, A, th, Y);
+            if (m == M) {  // restart on the keep lowest Ritz vectors
+                std::memcpy(Ys, Y.data(), sizeof(double) * m * m);
+                CF_HIP(hipMemcpyAsync(Yd.p, Ys, sizeof(double) * m * m, hipMemcpyHostToDevice, c->stream));
+                launch_rotate_basis(n, m, keep, V.p, Yd.p, c->stream);
+                launch_rotate_basis(n, m, keep, S.p, Yd.p, c->stream);
+                std::fill(G.begin(), G.end(), 0.0);
+                for (int i = 0; i < keep; ++i) G[(size_t)i * M + i] = th[i];
+                m = keep;
+                ++nrestart;
+                for (int i = 0; i < m; ++i) ys[i] = i == 0 ? 1.0 : 0.0;
+            } else {
+                for (int i = 0; i < m; ++i) ys[i] = Y[(size_t)i * m + 0];
+            }
+            CF_HIP(hipMemcpyAsync(dy.p, ys, m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            launch_combine2(n, m, V.p, S.p, dy.p, x.p, hx.p, c->stream);
+            launch_dav_residual(n, th[0], it < krylov ? 0.0 : 1e-4, x.p, hx.p, diag, t.p, partial.p, nb, c->stream);
+            launch_reduce_partials(1, nb, partial.p, dg.p + 62, c->stream);
+            orthogonalize_t();
+            launch_scale_rnorm(n, t.p, dg.p + 63, v(m), c->stream);
+            op(v(m), s(m));
+            ++napply;
+            dots(m + 1, V.p, s(m), dg.p);
+            CF_HIP(hipMemcpyAsync(hbuf, dg.p, 64 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            CF_HIP(hipStreamSynchronize(c->stream));
+            const double res = std::sqrt(std::max(0.0, hbuf[62]));
+            *theta_out = th[0];
+            *res_out = res;
+            if (progress) progress(napply, th[0], res);
+            if (res <= tol * std::fabs(th[0])) return 0;
+            if (tol_energy > 0.0 && th.size() >= 2 && res * res <= tol_energy * (th[1] - th[0])) return 0;
+            if (!(hbuf[63] > 1e-28)) return 2;
+            if (it >= maxit) return 1;
+            for (int i = 0; i <= m; ++i) G[(size_t)i * M + m] = G[(size_t)m * M + i] = hbuf[i];
+            ++m;
+        }
+    }
+};
+
+void run_davidson(cf_space* sp, uint64_t n, const cf_solve_opts& o, std::function<void(const double*, double*)> op64,
+                  std::function<void(const double*, double*)> op32, double* eigvec, cf_solve_info* info) {
+    cf_ctx* c = sp->ham->ctx;
+    CF_REQUIRE(sp->diag.p != nullptr, "the Davidson solver needs a space with a diagonal (FCI or DOCI)");
+    const auto t0 = std::chrono::steady_clock::now();
+    Davidson d(c, n, std::min(o.ncv, 24), sp->diag.p);
+    const int maxit = o.maxrestart > 0 ? o.maxrestart * d.M : 20000;
+    // Start: the deterministic random vector of the Lanczos path, expanded by
+    // plain Krylov steps before the preconditioned corrections begin.  A start
+    // on the lowest-diagonal determinants converges fast but stays in their
+    // symmetry sector (H conserves S^2; a closed-shell determinant is a pure
+    // singlet), and it can settle on a singlet above a triplet ground state;
+    // the Krylov steps from a random vector, as in Lanczos, let the lowest
+    // state of any sector lead before the diagonal preconditioner sharpens it.
+    constexpr int kKrylovSteps = 8;
+    launch_fill_start(n, o.seed, d.t.p, c->stream);
+    int nmv32 = 0, status = 0;
+    double theta = 0.0, res = 0.0;
+    auto prog = [&](int nmv, double th, double r) {
+        if (o.progress) o.progress(o.progress_user, nmv, th, r);
+    };
+    if (o.precision == CF_F32) {
+        CF_REQUIRE((bool)op32, "the FP32 operator is not available (build with CF_F32)");
+        d.op = op32;
+        d.start_from_t();
+        d.run(maxit, kKrylovSteps, o.tol_f32, o.tol_energy, prog, &theta, &res);
+        nmv32 = d.napply;
+        launch_copy(n, d.x.p, d.t.p, c->stream);  // polish from the FP32 Ritz vector
+        d.napply = 0;
+    }
+    d.op = op64;
+    d.start_from_t();
+    auto prog64 = [&](int nmv, double th, double r) {
+        if (o.progress) o.progress(o.progress_user, nmv32 + nmv, th, r);
+    };
+    status = d.run(maxit, o.precision == CF_F32 ? 0 : kKrylovSteps, o.tol, o.tol_energy, prog64, &theta, &res);
+    info->eigval = theta;
+    info->energy = theta + sp->ham->econst;
+    info->resid = res;
+    info->n_matvec = d.napply;
+    info->n_matvec_f32 = nmv32;
+    info->n_restart = d.nrestart;
+    info->status = status == 0 ? 0 : 1;
+    if (sp->vec.n != n) sp->vec.alloc(n);
+    std::vector<double> hv(n);
+    CF_HIP(hipMemcpyAsync(hv.data(), d.x.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipStreamSynchronize(c->stream));
+    sign_fix(hv);
+    copy_sync(sp->vec.p, hv.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    sp->vec_valid = true;
+    if (eigvec) std::memcpy(eigvec, hv.data(), n * sizeof(double));
+    info->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::function<void(const double*, double*)> op64,
                std::function<void(const double*, double*)> op32, double* eigvec, cf_solve_info* info) {
     cf_solve_opts o;
@@ -1702,6 +1881,11 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     CF_REQUIRE(o.check_every >= 0, "check_every must be >= 0");
     CF_REQUIRE(o.tol_energy >= 0.0, "tol_energy must be >= 0");
     CF_REQUIRE(o.precision == CF_F64 || o.precision == CF_F32, "precision must be CF_F64 or CF_F32");
+    CF_REQUIRE(o.method == CF_LANCZOS || o.method == CF_DAVIDSON, "method must be CF_LANCZOS or CF_DAVIDSON");
+    if (o.method == CF_DAVIDSON) {
+        run_davidson(sp, n, o, op64, op32, eigvec, info);
+        return;
+    }
     cf_ctx* c = sp->ham->ctx;
     const auto t0 = std::chrono::steady_clock::now();
     const int maxr = o.maxrestart > 0 ? o.maxrestart : (int)std::min<uint64_t>(3 * n, 1u << 30);

@@ -129,6 +129,15 @@ void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, doub
                       hipStream_t stream);
 void launch_reduce_partials(int k, int nblocks, const double* partial, double* out, hipStream_t stream);
 int multi_dot_blocks(uint64_t n);
+// Davidson: x = V y and hx = S y (k columns, one pass); then r = hx - theta x,
+// t = r / (theta - diag) (|theta - diag| clamped to >= shift_min; t = r for
+// shift_min 0) and
+// partial[b] = block b's part of |r|^2 (nblocks = multi_dot_blocks(n)).
+void launch_combine2(uint64_t n, int k, const double* V, const double* S, const double* y, double* x, double* hx,
+                     hipStream_t stream);
+void launch_scale_rnorm(uint64_t n, const double* src, const double* norm2, double* dst, hipStream_t stream);
+void launch_dav_residual(uint64_t n, double theta, double shift_min, const double* x, const double* hx,
+                         const double* diag, double* t, double* partial, int nblocks, hipStream_t stream);
 
 void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream);
 void launch_local3(uint64_t n, const double* a, const double* v1, const double* b, const double* v0, double* w,

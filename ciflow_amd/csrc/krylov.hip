@@ -4,6 +4,8 @@
 // (vector i at Vb + i*n).
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace cf {
 
 constexpr int kDotChunk = 8;  // vectors per block row
@@ -389,6 +391,90 @@ void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream)
     uint64_t grid = (n + 255) / 256;
     if (grid > 8192) grid = 8192;
     k_fill_start<<<(unsigned)grid, 256, 0, stream>>>(n, seed, v);
+    CF_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Davidson (served solves): Ritz vector and its image in one pass over both
+// bases, then residual, diagonal-preconditioned correction and |r|^2 parts.
+// ---------------------------------------------------------------------------
+// x = V[:, 0:k] y, hx = S[:, 0:k] y
+__global__ void __launch_bounds__(256) k_combine2(uint64_t n, int k, const double* __restrict__ V,
+                                                  const double* __restrict__ S, const double* __restrict__ y,
+                                                  double* __restrict__ x, double* __restrict__ hx) {
+    __shared__ double ys[64];
+    if (threadIdx.x < 64) ys[threadIdx.x] = (int)threadIdx.x < k ? y[threadIdx.x] : 0.0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < k; ++i) {
+            a += ys[i] * V[(uint64_t)i * n + q];
+            b += ys[i] * S[(uint64_t)i * n + q];
+        }
+        x[q] = a;
+        hx[q] = b;
+    }
+}
+
+void launch_combine2(uint64_t n, int k, const double* V, const double* S, const double* y, double* x, double* hx,
+                     hipStream_t stream) {
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
+    k_combine2<<<grid, 256, 0, stream>>>(n, k, V, S, y, x, hx);
+    CF_HIP(hipGetLastError());
+}
+
+// r = hx - theta x; t = r / (theta - diag) with |theta - diag| >= shift_min
+// (t = r when shift_min == 0: a plain Krylov expansion); partial[b] = this
+// block's part of |r|^2
+__global__ void __launch_bounds__(kDotThreads) k_dav_residual(uint64_t n, double theta, double shift_min,
+                                                              const double* __restrict__ x,
+                                                              const double* __restrict__ hx,
+                                                              const double* __restrict__ diag,
+                                                              double* __restrict__ t, double* __restrict__ partial) {
+    __shared__ double red[kDotThreads / 64];
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * kDotThreads;
+    for (uint64_t q = (uint64_t)blockIdx.x * kDotThreads + threadIdx.x; q < n; q += stride) {
+        const double r = hx[q] - theta * x[q];
+        acc += r * r;
+        if (shift_min > 0.0) {
+            double den = theta - diag[q];
+            if (fabs(den) < shift_min) den = den < 0.0 ? -shift_min : shift_min;
+            t[q] = r / den;
+        } else {
+            t[q] = r;
+        }
+    }
+    const double s = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int q = 0; q < kDotThreads / 64; ++q) a += red[q];
+        partial[blockIdx.x] = a;
+    }
+}
+
+void launch_dav_residual(uint64_t n, double theta, double shift_min, const double* x, const double* hx,
+                         const double* diag, double* t, double* partial, int nblocks, hipStream_t stream) {
+    k_dav_residual<<<(unsigned)nblocks, kDotThreads, 0, stream>>>(n, theta, shift_min, x, hx, diag, t, partial);
+    CF_HIP(hipGetLastError());
+}
+
+// dst = src / sqrt(*norm2) (zeros when *norm2 <= 1e-28): normalisation by a
+// norm that stays on the device
+__global__ void k_scale_rnorm(uint64_t n, const double* __restrict__ src, const double* __restrict__ norm2,
+                              double* __restrict__ dst) {
+    const double q2 = *norm2;
+    const double s = q2 > 1e-28 ? 1.0 / sqrt(q2) : 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = src[q] * s;
+}
+
+void launch_scale_rnorm(uint64_t n, const double* src, const double* norm2, double* dst, hipStream_t stream) {
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
+    k_scale_rnorm<<<grid, 256, 0, stream>>>(n, src, norm2, dst);
     CF_HIP(hipGetLastError());
 }
 

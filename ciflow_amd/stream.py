@@ -75,7 +75,7 @@ def prepare(reqs: list[StreamRequest], threads: int = 8) -> list[tuple]:
 def run_stream(reqs: list[StreamRequest], arrays: list[tuple], slots: int = 4, precision: int = CF_F32,
                hbm_budget: float = 0.0, tol: float = 0.0, server: SolveServer | None = None,
                timeout: float = 0.0, check_every: int = 0, tol_energy: float = 0.0, direct_fci: bool = False,
-               ncv: int = 0) -> dict:
+               ncv: int = 0, method: int = 0) -> dict:
     """Submit the requests at their arrival times and collect the results.
 
     Returns {"results": [dict per request, ordered by k], "wall_s": first
@@ -93,7 +93,8 @@ def run_stream(reqs: list[StreamRequest], arrays: list[tuple], slots: int = 4, p
                 if dt > 0:
                     time.sleep(dt)
                 srv.submit(r.k, r.kind, h, V, r.n, r.n, U=U, precision=precision, ncv=ncv, tol=tol,
-                           check_every=check_every, tol_energy=tol_energy, direct=direct_fci and r.kind == CF_FCI)
+                           check_every=check_every, tol_energy=tol_energy, direct=direct_fci and r.kind == CF_FCI,
+                           method=method)
         except BaseException as e:  # surfaced by the caller
             err.append(e)
 

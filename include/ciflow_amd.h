@@ -228,7 +228,15 @@ typedef struct {
      * applied so far, the lowest Ritz value and its residual estimate; NULL: none. */
     void (*progress)(void* user, int n_matvec, double theta, double resid);
     void* progress_user;
+    /* CF_LANCZOS (default): the thick-restart Lanczos above, ARPACK's method.
+     * CF_DAVIDSON (no reference counterpart; for served solves): diagonal-
+     * preconditioned Davidson from the Lanczos start vector (8 plain Krylov
+     * steps first), basis of min(ncv, 24) vectors restarted to 4 Ritz
+     * vectors; same precision phases and stopping tests.  Needs a space with
+     * a diagonal (FCI, DOCI). */
+    int method;
 } cf_solve_opts;
+enum { CF_LANCZOS = 0, CF_DAVIDSON = 1, CF_SOLVER_AUTO = 2 };
 
 typedef struct {
     double eigval;       /* lowest eigenvalue of the CI matrix (no Econst) */
@@ -322,6 +330,10 @@ typedef struct {
     int check_every;    /* cf_solve_opts.check_every */
     double tol_energy;  /* cf_solve_opts.tol_energy */
     int direct;         /* FCI only: 1 = on-the-fly sigma (cf_solve_direct) instead of a stored CRS */
+    int method;         /* cf_solve_opts.method: CF_LANCZOS, CF_DAVIDSON, or CF_SOLVER_AUTO:
+                           Davidson for FCI and for DOCI spaces of >= 150,000
+                           determinants, Lanczos for the smaller DOCI ones (whose
+                           steps are launch-bound, where Lanczos's fused step wins) */
 } cf_request;
 typedef struct {
     uint64_t tag;
