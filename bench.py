@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--no-fp64", action="store_true", help="direct16: skip the FP64 solve")
     p.add_argument("--cpu-lines", type=int, default=6, help="alpha lines in the FCI CPU-baseline sample")
     p.add_argument("--solver", choices=["auto", "lanczos", "davidson"], default="auto",
-                   help="stream: served solver (auto: Davidson for FCI and DOCI >= 150k determinants)")
+                   help="stream: served solver (auto: Davidson for FCI and DOCI >= 2,000 determinants)")
     return p.parse_args()
 
 
@@ -300,7 +300,7 @@ def bench_stream(a, CA):
         "config": {"workload": ("served stream of rotated-integral ground-state solves: 70% DOCI L~U[12,20], "
                                 "30% FCI L~U[8,11] (on-the-fly sigma), half filling, seeds 1000+k, U=exp(K) "
                                 "K~N(0,0.05^2), FP32 values + FP64 polish, Kato-Temple energy stop 1e-9 Eh"),
-                   "solver": {"auto": "Davidson for FCI and DOCI >= 150,000 determinants, Lanczos for smaller DOCI",
+                   "solver": {"auto": "Davidson for FCI and DOCI >= 2,000 determinants, Lanczos for smaller DOCI",
                               "lanczos": "thick-restart Lanczos", "davidson": "Davidson"}[a.solver],
                    "requests": K, "doci": int((kinds == 1).sum()), "fci": int((kinds == 0).sum()),
                    "slots": a.slots, "closed_batch": True,
@@ -431,8 +431,11 @@ def bench_fci12(a, CA):
 
 def bench_direct16(a, CA):
     """FCI L=16 8a8b seed 3 (165,636,900 determinants) on the on-the-fly sigma:
-    FP64 solve and FP32-sigma + FP64 polish, at most 40 Krylov vectors, then
-    the 1-RDM of the ground state.  One step = one FP32+polish solve."""
+    FP64 solve (Lanczos, ARPACK semantics) and FP32-sigma + FP64 refinement,
+    at most 40 Krylov vectors, then the 1-RDM of the ground state.  One step =
+    one FP32+refinement solve, by Davidson (--solver auto/davidson: residual
+    corrections preconditioned by the diagonal, at most 24 basis vectors) or
+    by Lanczos (--solver lanczos)."""
     L, nel, seed = 16, 8, 3
     ctx = CA.Context(0)
     h, V = CA.gen_integrals(L, seed)
@@ -448,8 +451,9 @@ def bench_direct16(a, CA):
     def prog(nmv, th, r):
         log(f"  {nmv} products, theta {th:.10f}, residual {r:.2e}, {time.perf_counter() - t0:.1f} s")
 
+    method = CA.CF_LANCZOS if a.solver == "lanczos" else CA.CF_DAVIDSON
     t0 = time.perf_counter()
-    i32 = d.solve(precision=CA.CF_F32, ncv=39, progress=prog)
+    i32 = d.solve(precision=CA.CF_F32, ncv=39, progress=prog, method=method)
     t32 = time.perf_counter() - t0
     e32 = d.get_ci_energy()
     log(f"  E={e32:.12f} matvec {i32.n_matvec}+{i32.n_matvec_f32} {t32:.1f} s")
@@ -477,7 +481,7 @@ def bench_direct16(a, CA):
         "ms_per_step": t32 * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32-sigma/f64-accumulate+f64-polish", "data": "synthetic",
         "config": {"workload": f"direct FCI L={L} {nel}a{nel}b seed {seed}, on-the-fly sigma, ncv 39",
-                   "determinants": n},
+                   "determinants": n, "solver": "lanczos" if method == CA.CF_LANCZOS else "davidson"},
         "energy": e32, "n_matvec": i32.n_matvec, "n_matvec_f32": i32.n_matvec_f32,
         "sigma_ms": {"f32": ms32 / max(n32, 1), "f64": ms64 / max(n64, 1)},
         "setup_s": t_setup, "rdm1_s": t_rdm, "rdm1_trace": tr, "fp64": out64,

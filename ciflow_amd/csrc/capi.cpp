@@ -1705,8 +1705,10 @@ struct Davidson {
     uint64_t n;
     int M, keep;
     const double* diag;
-    DBuf<double> V, S, x, hx, t, partial, dg, dy, Yd;
+    DBuf<double> V, S, x, hx, t, partial, dg, dy, Yd, dc, pnorm, rparts;
     int nb;
+    bool fused;  // n <= kFusedMaxN: step kernels reduce their partials themselves
+    int Gd;      // workgroups of the wide dot products
     double* hbuf;
     std::vector<double> G;  // M x M, row-major
     int m = 0;              // basis size
@@ -1722,7 +1724,12 @@ struct Davidson {
         hx.alloc(n);
         t.alloc(n);
         nb = multi_dot_blocks(n);
-        partial.alloc((size_t)64 * nb);
+        fused = n <= kFusedMaxN;
+        Gd = fused_dot_blocks(n);
+        partial.alloc((size_t)64 * std::max(nb, Gd));
+        dc.alloc(64);
+        pnorm.alloc(256);
+        rparts.alloc(256);
         dg.alloc(64);
         dy.alloc(64);
         Yd.alloc(64 * 64);
@@ -1793,14 +1800,29 @@ struct Davidson {
                 for (int i = 0; i < m; ++i) ys[i] = Y[(size_t)i * m + 0];
             }
             CF_HIP(hipMemcpyAsync(dy.p, ys, m * sizeof(double), hipMemcpyHostToDevice, c->stream));
-            launch_combine2(n, m, V.p, S.p, dy.p, x.p, hx.p, c->stream);
-            launch_dav_residual(n, th[0], it < krylov ? 0.0 : 1e-4, x.p, hx.p, diag, t.p, partial.p, nb, c->stream);
-            launch_reduce_partials(1, nb, partial.p, dg.p + 62, c->stream);
-            orthogonalize_t();
-            launch_scale_rnorm(n, t.p, dg.p + 63, v(m), c->stream);
-            op(v(m), s(m));
-            ++napply;
-            dots(m + 1, V.p, s(m), dg.p);
+            const double shift = it < krylov ? 0.0 : 1e-4;
+            if (fused) {  // 8 launches besides the product
+                launch_combine_residual(n, m, V.p, S.p, dy.p, th[0], shift, diag, x.p, hx.p, t.p, rparts.p,
+                                        c->stream);
+                for (int pass = 0; pass < 2; ++pass) {
+                    launch_multi_dot_wide(n, m, V.p, t.p, partial.p, Gd, c->stream);
+                    launch_multi_axpy_r(n, m, Gd, partial.p, V.p, t.p, dc.p, pnorm.p, c->stream);
+                }
+                launch_scale_rnorm_parts(n, t.p, pnorm.p, rparts.p, v(m), dg.p + 62, c->stream);
+                op(v(m), s(m));
+                ++napply;
+                launch_multi_dot_wide(n, m + 1, V.p, s(m), partial.p, Gd, c->stream);
+                launch_reduce_partials(m + 1, Gd, partial.p, dg.p, c->stream);
+            } else {
+                launch_combine2(n, m, V.p, S.p, dy.p, x.p, hx.p, c->stream);
+                launch_dav_residual(n, th[0], shift, x.p, hx.p, diag, t.p, partial.p, nb, c->stream);
+                launch_reduce_partials(1, nb, partial.p, dg.p + 62, c->stream);
+                orthogonalize_t();
+                launch_scale_rnorm(n, t.p, dg.p + 63, v(m), c->stream);
+                op(v(m), s(m));
+                ++napply;
+                dots(m + 1, V.p, s(m), dg.p);
+            }
             CF_HIP(hipMemcpyAsync(hbuf, dg.p, 64 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
             CF_HIP(hipStreamSynchronize(c->stream));
             const double res = std::sqrt(std::max(0.0, hbuf[62]));

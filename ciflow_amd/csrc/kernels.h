@@ -136,6 +136,14 @@ int multi_dot_blocks(uint64_t n);
 void launch_combine2(uint64_t n, int k, const double* V, const double* S, const double* y, double* x, double* hx,
                      hipStream_t stream);
 void launch_scale_rnorm(uint64_t n, const double* src, const double* norm2, double* dst, hipStream_t stream);
+// Fused Davidson step for n <= kFusedMaxN: x, hx, t and per-block |r|^2 parts
+// (fused_axpy_blocks(n) of them) in one pass; then, after two multi_axpy_r
+// passes over t, dst = t / |t| with out[0] = |r|^2 and out[1] = |t|^2.
+void launch_combine_residual(uint64_t n, int k, const double* V, const double* S, const double* y, double theta,
+                             double shift_min, const double* diag, double* x, double* hx, double* t, double* rparts,
+                             hipStream_t stream);
+void launch_scale_rnorm_parts(uint64_t n, const double* src, const double* pnorm, const double* rparts, double* dst,
+                              double* out, hipStream_t stream);
 void launch_dav_residual(uint64_t n, double theta, double shift_min, const double* x, const double* hx,
                          const double* diag, double* t, double* partial, int nblocks, hipStream_t stream);
 

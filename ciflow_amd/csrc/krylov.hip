@@ -462,6 +462,86 @@ void launch_dav_residual(uint64_t n, double theta, double shift_min, const doubl
     CF_HIP(hipGetLastError());
 }
 
+// Fused forms for n <= kFusedMaxN (one workgroup per fused_axpy_blocks slot):
+// x, hx, the correction t and this block's part of |r|^2 in one pass ...
+__global__ void __launch_bounds__(256) k_combine_residual(uint64_t n, int k, const double* __restrict__ V,
+                                                          const double* __restrict__ S, const double* __restrict__ y,
+                                                          double theta, double shift_min,
+                                                          const double* __restrict__ diag, double* __restrict__ x,
+                                                          double* __restrict__ hx, double* __restrict__ t,
+                                                          double* __restrict__ rparts) {
+    __shared__ double ys[64];
+    __shared__ double red[4];
+    if (threadIdx.x < 64) ys[threadIdx.x] = (int)threadIdx.x < k ? y[threadIdx.x] : 0.0;
+    __syncthreads();
+    double acc = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
+        double a = 0.0, b = 0.0;
+        for (int i = 0; i < k; ++i) {
+            a += ys[i] * V[(uint64_t)i * n + q];
+            b += ys[i] * S[(uint64_t)i * n + q];
+        }
+        x[q] = a;
+        hx[q] = b;
+        const double r = b - theta * a;
+        acc += r * r;
+        if (shift_min > 0.0) {
+            double den = theta - diag[q];
+            if (fabs(den) < shift_min) den = den < 0.0 ? -shift_min : shift_min;
+            t[q] = r / den;
+        } else {
+            t[q] = r;
+        }
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) rparts[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void launch_combine_residual(uint64_t n, int k, const double* V, const double* S, const double* y, double theta,
+                             double shift_min, const double* diag, double* x, double* hx, double* t, double* rparts,
+                             hipStream_t stream) {
+    k_combine_residual<<<(unsigned)fused_axpy_blocks(n), 256, 0, stream>>>(n, k, V, S, y, theta, shift_min, diag,
+                                                                           x, hx, t, rparts);
+    CF_HIP(hipGetLastError());
+}
+
+// ... and dst = src / |src| from the per-block |src|^2 parts of the last
+// multi_axpy_r (np parts, fixed order); block 0 also stores the sums of
+// rparts (|r|^2) and pnorm (|src|^2) into out[0], out[1].
+__global__ void __launch_bounds__(256) k_scale_rnorm_parts(uint64_t n, const double* __restrict__ src,
+                                                           const double* __restrict__ pnorm,
+                                                           const double* __restrict__ rparts, int np,
+                                                           double* __restrict__ dst, double* __restrict__ out) {
+    __shared__ double red[2][4];
+    double a = (int)threadIdx.x < np ? pnorm[threadIdx.x] : 0.0;
+    double r = (int)threadIdx.x < np ? rparts[threadIdx.x] : 0.0;
+    a = wave_sum(a);
+    r = wave_sum(r);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = a;
+        red[1][threadIdx.x >> 6] = r;
+    }
+    __syncthreads();
+    const double q2 = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    const double s = q2 > 1e-28 ? 1.0 / sqrt(q2) : 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = src[q] * s;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out[0] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        out[1] = q2;
+    }
+}
+
+void launch_scale_rnorm_parts(uint64_t n, const double* src, const double* pnorm, const double* rparts, double* dst,
+                              double* out, hipStream_t stream) {
+    const int np = fused_axpy_blocks(n);
+    k_scale_rnorm_parts<<<(unsigned)np, 256, 0, stream>>>(n, src, pnorm, rparts, np, dst, out);
+    CF_HIP(hipGetLastError());
+}
+
 // dst = src / sqrt(*norm2) (zeros when *norm2 <= 1e-28): normalisation by a
 // norm that stays on the device
 __global__ void k_scale_rnorm(uint64_t n, const double* __restrict__ src, const double* __restrict__ norm2,

@@ -141,13 +141,13 @@ void worker_main(cf_server* S) {
             o.tol_energy = r.tol_energy > 0.0 ? r.tol_energy : 0.0;
             o.method = r.method;
             if (r.method == CF_SOLVER_AUTO) {
-                // measured per class on MI355X (8 workers): Davidson 89.8 against
-                // Lanczos 67.6 solves/s for FCI L=11, 252.8 / 237.1 for L=10,
-                // 238.1 / 218.7 for DOCI L=20; equal for DOCI L=16-19 and
-                // 1047 / 1167 for DOCI L=12
+                // measured per class on MI355X (8 workers, solves/s, Davidson
+                // against Lanczos): FCI L=8-11 609 / 568, 433 / 384, 255 / 240,
+                // 81-90 / 68; DOCI L=14-20 1004 / 966 ... 246 / 218; DOCI L=12
+                // (924 determinants, launch-bound) 1066 / 1138
                 uint64_t n = 0;
                 cf_space_dims(sp, &n, nullptr, nullptr);
-                o.method = (r.kind == CF_FCI || n >= 150000) ? CF_DAVIDSON : CF_LANCZOS;
+                o.method = (r.kind == CF_FCI || n >= 2000) ? CF_DAVIDSON : CF_LANCZOS;
             }
             cf_solve_info info;
             st = direct ? cf_solve_direct(sg, &o, nullptr, &info) : cf_solve(m, &o, nullptr, &info);

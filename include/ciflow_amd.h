@@ -331,8 +331,8 @@ typedef struct {
     double tol_energy;  /* cf_solve_opts.tol_energy */
     int direct;         /* FCI only: 1 = on-the-fly sigma (cf_solve_direct) instead of a stored CRS */
     int method;         /* cf_solve_opts.method: CF_LANCZOS, CF_DAVIDSON, or CF_SOLVER_AUTO:
-                           Davidson for FCI and for DOCI spaces of >= 150,000
-                           determinants, Lanczos for the smaller DOCI ones (whose
+                           Davidson for FCI and for DOCI spaces of >= 2,000
+                           determinants, Lanczos for the smallest DOCI ones (whose
                            steps are launch-bound, where Lanczos's fused step wins) */
 } cf_request;
 typedef struct {
