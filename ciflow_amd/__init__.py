@@ -394,7 +394,9 @@ class SparseMatrix_CRS_Sym:
         return n.value, nf.value, nu.value
 
     def getn(self):
-        return self.info()[0]
+        n = C.c_uint64()
+        _check(lib().cf_crs_info(self.ptr, C.byref(n), None, None))  # no CRS counts needed
+        return n.value
 
     def datasize(self):
         """Stored upper-triangle entries, as the reference's data.size()."""

@@ -330,7 +330,7 @@ void spmv_with(cf_crs* m, int prec, const SpmvChoice& ch, const double* dx, doub
     sync_ham(m);
     if (m->crs_stale && ch.kernel < 4) ensure_crs(m);
     if (prec == CF_F32) {
-        CF_REQUIRE(m->val32.p != nullptr, "matrix was built without FP32 values");
+        CF_REQUIRE(m->want32, "matrix was built without FP32 values");
         KTimer t(c, "spmv_f32");
         switch (ch.kernel) {
             case 5: launch_spmv_doci_f32(m->dargs, ch.doci, m->dvk32.p, dx, dy, c->stream); break;
@@ -466,7 +466,7 @@ void make_ell(cf_crs* m) {
     m->ent.alloc(sp->da * (uint64_t)Ea);
     launch_ell_entries(sp->da, Ea, nda, nsa, sp->ta->full.p, sp->segoff.p, m->ent.p, c->stream);
     m->ell.alloc(m->n * R);
-    if (m->val32.p) m->ell32.alloc(m->n * R);
+    if (m->want32) m->ell32.alloc(m->n * R);
     {
         KTimer t(c, "crs_fill");
         launch_fci_ell(m->fargs, m->ell.p, m->ell32.p, c->stream);
@@ -1323,10 +1323,22 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
     CF_REQUIRE(m->n < (1ull << 32), "matrix dimension must be < 2^32 (u32 columns)");
     m->want32 = value_prec == CF_F32;
     m->version = sp->version;
-    build_crs_arrays(m.get());
+    const cf_ham* H = sp->ham;
+    // DOCI matrices that run the block-staged kernel (below) need no CRS for
+    // products: its arrays (pattern and values) are built when something reads
+    // them (export, counts, the CRS kernels), not here
+    const int np = H->nup, nv = H->L - H->nup;
+    const bool defer = sp->kind == CF_DOCI && m->n >= 60000 && H->L - doci_block_w(H->L) <= 20;
+    const uint64_t doci_row = 1 + (uint64_t)np * (uint64_t)nv;
+    if (defer) {
+        m->crs_stale = true;
+        m->lpr = pick_lanes_per_row(m->n, m->n * doci_row);
+    } else {
+        build_crs_arrays(m.get());
+    }
     // measured on MI355X (FCI L=11/12, DOCI L=20/24): vectorised streams win for
     // FP32 values at every row length and for FP64 rows of >= 512 entries
-    const double avg = m->n ? (double)m->nnz / (double)m->n : 0.0;
+    const double avg = defer ? (double)doci_row : m->n ? (double)m->nnz / (double)m->n : 0.0;
     m->auto64.kernel = m->auto32.kernel = 1;
     m->auto64.lanes = m->auto32.lanes = m->lpr;
     if (avg >= 512) m->auto64 = SpmvChoice{3, 16, 2, {}, {}};
@@ -1423,9 +1435,11 @@ static int export_rows(cf_crs* m, uint64_t r0, uint64_t r1, bool upper, uint64_t
 int cf_crs_info(const cf_crs* m, uint64_t* n, uint64_t* nnz_full, uint64_t* nnz_upper) {
     API_BEGIN
     CF_REQUIRE(m, "matrix is NULL");
-    cf_crs* mm = const_cast<cf_crs*>(m);  // a stale CRS is rebuilt first
-    mm->sp->ham->ctx->use();
-    ensure_crs(mm);
+    cf_crs* mm = const_cast<cf_crs*>(m);
+    if (nnz_full || nnz_upper) {  // counts of a stale (or deferred) CRS: build it first
+        mm->sp->ham->ctx->use();
+        ensure_crs(mm);
+    }
     if (n) *n = m->n;
     if (nnz_full) *nnz_full = m->nnz;
     if (nnz_upper) {
@@ -1529,7 +1543,7 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
     API_BEGIN
     CF_REQUIRE(m, "matrix is NULL");
     CF_REQUIRE(value_prec == CF_F64 || value_prec == CF_F32, "value_prec must be CF_F64 or CF_F32");
-    CF_REQUIRE(value_prec == CF_F64 || m->val32.p, "matrix was built without FP32 values");
+    CF_REQUIRE(value_prec == CF_F64 || m->want32, "matrix was built without FP32 values");
     CF_REQUIRE(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
     cf_ctx* c = m->sp->ham->ctx;
     c->use();
@@ -2014,7 +2028,7 @@ int cf_solve(cf_crs* m, const cf_solve_opts* opts, double* eigvec, cf_solve_info
     CF_REQUIRE(m && info, "NULL argument");
     m->sp->ham->ctx->use();
     std::function<void(const double*, double*)> op32;
-    if (m->val32.p) op32 = [m](const double* x, double* y) { spmv(m, CF_F32, x, y); };
+    if (m->want32) op32 = [m](const double* x, double* y) { spmv(m, CF_F32, x, y); };
     run_solve(m->sp, m->n, opts, [m](const double* x, double* y) { spmv(m, CF_F64, x, y); }, op32, eigvec, info);
     API_END
 }

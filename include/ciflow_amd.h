@@ -145,9 +145,14 @@ int cf_strings_rank(cf_ctx* ctx, int L, int nel, const uint64_t* strings, uint64
  *      CIMethod::build_parallel (CIMethod.cpp:489-561, 611-660, 749-888).
  *      The GPU keeps full rows (both triangles); entries with |v| <= zero
  *      are not stored, as SparseMatrix_CRS::PushToRow does (:240-272).
- *      value_prec CF_F32 also stores an FP32 copy of the values. */
+ *      value_prec CF_F32 also stores an FP32 copy of the values.  DOCI
+ *      matrices of >= 60,000 rows, whose products run the block-staged kernel
+ *      (no CRS needed), build the CRS arrays when something first reads them
+ *      (counts, export, a CRS kernel). */
 int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out);
 int cf_crs_destroy(cf_crs* m);
+/* Dimension and nonzero counts; pass NULL for counts not needed (a deferred
+ * CRS is built only when a count is asked for). */
 int cf_crs_info(const cf_crs* m, uint64_t* n, uint64_t* nnz_full, uint64_t* nnz_upper);
 /* Export rows [row_begin, row_end) of the upper triangle in the reference
  * layout (SparseMatrix_CRS.h:109-114: row pointer, column, value; columns
