@@ -1715,12 +1715,14 @@ namespace {
 // t_i = r_i / (theta - H_ii), two Gram-Schmidt passes, one product.  A full
 // basis restarts to its `keep` lowest Ritz vectors (V and S rotated by the
 // same MFMA kernel as the Lanczos restart).
+constexpr double kDgksEta = 0.717;
+
 struct Davidson {
     cf_ctx* c;
     uint64_t n;
     int M, keep;
     const double* diag;
-    DBuf<double> V, S, x, hx, t, partial, dg, dy, Yd, dc, pnorm, rparts;
+    DBuf<double> V, S, x, hx, t, partial, dg, dy, Yd, dc, pnorm, pnorm2, rparts, tparts;
     int nb;
     bool fused;  // small n: step kernels reduce their partials themselves
     int Gd;      // workgroups of the wide dot products
@@ -1747,7 +1749,9 @@ struct Davidson {
         partial.alloc((size_t)64 * std::max(nb, Gd));
         dc.alloc(64);
         pnorm.alloc(256);
+        pnorm2.alloc(256);
         rparts.alloc(256);
+        tparts.alloc(256);
         dg.alloc(64);
         dy.alloc(64);
         Yd.alloc(64 * 64);
@@ -1825,12 +1829,16 @@ struct Davidson {
             const double shift = it < krylov ? 0.0 : 1e-4;
             if (fused) {  // 8 launches besides the product
                 launch_combine_residual(n, m, V.p, S.p, dy.p, th[0], shift, diag, x.p, hx.p, t.p, rparts.p,
-                                        c->stream);
-                for (int pass = 0; pass < 2; ++pass) {
-                    launch_multi_dot_wide(n, m, V.p, t.p, partial.p, Gd, c->stream);
-                    launch_multi_axpy_r(n, m, Gd, partial.p, V.p, t.p, dc.p, pnorm.p, c->stream);
-                }
-                launch_scale_rnorm_parts(n, t.p, pnorm.p, rparts.p, v(m), dg.p + 62, c->stream);
+                                        tparts.p, c->stream);
+                launch_multi_dot_wide(n, m, V.p, t.p, partial.p, Gd, c->stream);
+                launch_multi_axpy_r(n, m, Gd, partial.p, V.p, t.p, dc.p, pnorm.p, c->stream);
+                // second pass only when the first removed more than half of t
+                // (|t'| <= 0.717 |t|, the refinement test ARPACK's dsaitr uses)
+                launch_multi_dot_wide(n, m, V.p, t.p, partial.p, Gd, c->stream, tparts.p, pnorm.p, kDgksEta);
+                launch_multi_axpy_r(n, m, Gd, partial.p, V.p, t.p, dc.p, pnorm2.p, c->stream, tparts.p, pnorm.p,
+                                    kDgksEta);
+                launch_scale_rnorm_parts(n, t.p, pnorm.p, rparts.p, v(m), dg.p + 62, c->stream, tparts.p, pnorm2.p,
+                                         kDgksEta);
                 op(v(m), s(m));
                 ++napply;
                 launch_multi_dot_wide(n, m + 1, V.p, s(m), partial.p, Gd, c->stream);

@@ -141,9 +141,12 @@ void launch_scale_rnorm(uint64_t n, const double* src, const double* norm2, doub
 // passes over t, dst = t / |t| with out[0] = |r|^2 and out[1] = |t|^2.
 void launch_combine_residual(uint64_t n, int k, const double* V, const double* S, const double* y, double theta,
                              double shift_min, const double* diag, double* x, double* hx, double* t, double* rparts,
-                             hipStream_t stream);
+                             double* tparts, hipStream_t stream);
+// (with tparts: |t|^2 parts before the first pass; pnorm2 the parts after a
+// selective second pass, used unless the first pass sufficed)
 void launch_scale_rnorm_parts(uint64_t n, const double* src, const double* pnorm, const double* rparts, double* dst,
-                              double* out, hipStream_t stream);
+                              double* out, hipStream_t stream, const double* tparts = nullptr,
+                              const double* pnorm2 = nullptr, double eta = 0.0);
 void launch_dav_residual(uint64_t n, double theta, double shift_min, const double* x, const double* hx,
                          const double* diag, double* t, double* partial, int nblocks, hipStream_t stream);
 
@@ -161,15 +164,20 @@ void launch_scale_copy(uint64_t n, const double* src, double s, double* dst, hip
 constexpr uint64_t kFusedMaxN = 262144;
 int fused_dot_blocks(uint64_t n);
 int fused_axpy_blocks(uint64_t n);
+// With skip_before/skip_after (per-block |.|^2 parts, fused_axpy_blocks(n) of
+// each): a selective second Gram-Schmidt pass, skipped inside the kernel when
+// sum(after) > eta^2 sum(before).
 void launch_multi_dot_wide(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
-                           hipStream_t stream);
+                           hipStream_t stream, const double* skip_before = nullptr,
+                           const double* skip_after = nullptr, double eta = 0.0);
 // w -= a v1 + b v0, a = sum of the nblocks partials (written to a_out), b = *b_in
 void launch_local3_r(uint64_t n, int nblocks, const double* partial, const double* v1, const double* b_in,
                      const double* v0, double* w, double* a_out, hipStream_t stream);
 // w -= Vb[:, 0:k] c, c_i = sum of partial[i*nblocks + 0:nblocks] (written to c_out);
 // pnorm[block] = that workgroup's part of |w|^2 after the update (fused_axpy_blocks(n) workgroups)
 void launch_multi_axpy_r(uint64_t n, int k, int nblocks, const double* partial, const double* Vb, double* w,
-                         double* c_out, double* pnorm, hipStream_t stream);
+                         double* c_out, double* pnorm, hipStream_t stream, const double* skip_before = nullptr,
+                         const double* skip_after = nullptr, double eta = 0.0);
 // finish_step with |w|^2 = the sum of the fused_axpy_blocks(n) entries of pnorm
 void launch_finish_step_r(uint64_t n, int j, const double* c1, const double* c2, const double* pnorm,
                           const double* w, double* vnext, double* alpha, double* beta, hipStream_t stream);

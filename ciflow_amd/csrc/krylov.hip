@@ -24,6 +24,31 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Selective second Gram-Schmidt pass (Daniel-Gragg-Kaufman-Stewart): the pass
+// is skipped when the first one kept more than eta of the vector,
+// sum(after) > eta^2 sum(before), both given as np <= 256 per-block parts.
+// Every thread of the block gets the same answer (fixed summation order).
+__device__ bool dgks_skip(const double* __restrict__ before, const double* __restrict__ after, int np, double eta2) {
+    __shared__ double red[2][16];
+    const int t = threadIdx.x;
+    double b = t < np ? before[t] : 0.0, a = t < np ? after[t] : 0.0;
+    b = wave_sum(b);
+    a = wave_sum(a);
+    if ((t & 63) == 0) {
+        red[0][t >> 6] = b;
+        red[1][t >> 6] = a;
+    }
+    __syncthreads();
+    double sb = 0.0, sa = 0.0;
+    const int waves = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < waves; ++w) {
+        sb += red[0][w];
+        sa += red[1][w];
+    }
+    __syncthreads();
+    return sa > eta2 * sb;
+}
+
 // partial[i * nblocks + bx] = sum over this block's elements of Vb[i] . w
 __global__ void __launch_bounds__(kDotThreads) k_multi_dot(uint64_t n, int k, const double* __restrict__ Vb,
                                                            const double* __restrict__ w, double* __restrict__ partial) {
@@ -178,8 +203,10 @@ int fused_axpy_blocks(uint64_t n) {
 
 __global__ void __launch_bounds__(1024) k_multi_dot_wide(uint64_t n, int k, const double* __restrict__ Vb,
                                                          const double* __restrict__ w,
-                                                         double* __restrict__ partial) {
+                                                         double* __restrict__ partial, const double* skip_before,
+                                                         const double* skip_after, int skip_np, double eta2) {
     __shared__ double red[kDotChunk][16];
+    if (skip_before && dgks_skip(skip_before, skip_after, skip_np, eta2)) return;
     const int i0 = blockIdx.y * kDotChunk;
     double acc[kDotChunk];
 #pragma unroll
@@ -206,12 +233,13 @@ __global__ void __launch_bounds__(1024) k_multi_dot_wide(uint64_t n, int k, cons
 }
 
 void launch_multi_dot_wide(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,
-                           hipStream_t stream) {
+                           hipStream_t stream, const double* skip_before, const double* skip_after, double eta) {
     if (k <= 0) return;
     if (nblocks < 1 || nblocks > 64 || (nblocks & (nblocks - 1)))
         throw Error(ERR_ARG, "multi_dot_wide: nblocks must be a power of two <= 64");
     dim3 grid((unsigned)nblocks, (unsigned)((k + kDotChunk - 1) / kDotChunk));
-    k_multi_dot_wide<<<grid, 1024, 0, stream>>>(n, k, Vb, w, partial);
+    k_multi_dot_wide<<<grid, 1024, 0, stream>>>(n, k, Vb, w, partial, skip_before, skip_after,
+                                                fused_axpy_blocks(n), eta * eta);
     CF_HIP(hipGetLastError());
 }
 
@@ -260,9 +288,12 @@ void launch_local3_r(uint64_t n, int nblocks, const double* partial, const doubl
 
 __global__ void __launch_bounds__(256) k_multi_axpy_r(uint64_t n, int k, int G, const double* __restrict__ partial,
                                                       const double* __restrict__ Vb, double* __restrict__ w,
-                                                      double* __restrict__ c_out, double* __restrict__ pnorm) {
+                                                      double* __restrict__ c_out, double* __restrict__ pnorm,
+                                                      const double* skip_before, const double* skip_after,
+                                                      double eta2) {
     __shared__ double cs[64];
     __shared__ double red[4];
+    if (skip_before && dgks_skip(skip_before, skip_after, (int)gridDim.x, eta2)) return;
     reduce_coeffs(k, G, partial, cs);
     __syncthreads();
     if (blockIdx.x == 0 && threadIdx.x < k) c_out[threadIdx.x] = cs[threadIdx.x];
@@ -282,11 +313,13 @@ __global__ void __launch_bounds__(256) k_multi_axpy_r(uint64_t n, int k, int G, 
 }
 
 void launch_multi_axpy_r(uint64_t n, int k, int nblocks, const double* partial, const double* Vb, double* w,
-                         double* c_out, double* pnorm, hipStream_t stream) {
+                         double* c_out, double* pnorm, hipStream_t stream, const double* skip_before,
+                         const double* skip_after, double eta) {
     if (!n || k <= 0) return;
     if (k > 64 || nblocks < 1 || nblocks > 64 || (nblocks & (nblocks - 1)) || k * nblocks > 16 * 256)
         throw Error(ERR_ARG, "multi_axpy_r: k <= 64 and a power-of-two nblocks <= 64 with k*nblocks <= 4096");
-    k_multi_axpy_r<<<(unsigned)fused_axpy_blocks(n), 256, 0, stream>>>(n, k, nblocks, partial, Vb, w, c_out, pnorm);
+    k_multi_axpy_r<<<(unsigned)fused_axpy_blocks(n), 256, 0, stream>>>(n, k, nblocks, partial, Vb, w, c_out, pnorm,
+                                                                       skip_before, skip_after, eta * eta);
     CF_HIP(hipGetLastError());
 }
 
@@ -469,12 +502,12 @@ __global__ void __launch_bounds__(256) k_combine_residual(uint64_t n, int k, con
                                                           double theta, double shift_min,
                                                           const double* __restrict__ diag, double* __restrict__ x,
                                                           double* __restrict__ hx, double* __restrict__ t,
-                                                          double* __restrict__ rparts) {
+                                                          double* __restrict__ rparts, double* __restrict__ tparts) {
     __shared__ double ys[64];
-    __shared__ double red[4];
+    __shared__ double red[2][4];
     if (threadIdx.x < 64) ys[threadIdx.x] = (int)threadIdx.x < k ? y[threadIdx.x] : 0.0;
     __syncthreads();
-    double acc = 0.0;
+    double acc = 0.0, acct = 0.0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
         double a = 0.0, b = 0.0;
@@ -486,25 +519,33 @@ __global__ void __launch_bounds__(256) k_combine_residual(uint64_t n, int k, con
         hx[q] = b;
         const double r = b - theta * a;
         acc += r * r;
+        double tq = r;
         if (shift_min > 0.0) {
             double den = theta - diag[q];
             if (fabs(den) < shift_min) den = den < 0.0 ? -shift_min : shift_min;
-            t[q] = r / den;
-        } else {
-            t[q] = r;
+            tq = r / den;
         }
+        t[q] = tq;
+        acct += tq * tq;
     }
     acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    acct = wave_sum(acct);
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = acc;
+        red[1][threadIdx.x >> 6] = acct;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) rparts[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (threadIdx.x == 0) {
+        rparts[blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        tparts[blockIdx.x] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    }
 }
 
 void launch_combine_residual(uint64_t n, int k, const double* V, const double* S, const double* y, double theta,
                              double shift_min, const double* diag, double* x, double* hx, double* t, double* rparts,
-                             hipStream_t stream) {
+                             double* tparts, hipStream_t stream) {
     k_combine_residual<<<(unsigned)fused_axpy_blocks(n), 256, 0, stream>>>(n, k, V, S, y, theta, shift_min, diag,
-                                                                           x, hx, t, rparts);
+                                                                           x, hx, t, rparts, tparts);
     CF_HIP(hipGetLastError());
 }
 
@@ -514,8 +555,12 @@ void launch_combine_residual(uint64_t n, int k, const double* V, const double* S
 __global__ void __launch_bounds__(256) k_scale_rnorm_parts(uint64_t n, const double* __restrict__ src,
                                                            const double* __restrict__ pnorm,
                                                            const double* __restrict__ rparts, int np,
-                                                           double* __restrict__ dst, double* __restrict__ out) {
+                                                           double* __restrict__ dst, double* __restrict__ out,
+                                                           const double* tparts, const double* pnorm2,
+                                                           double eta2) {
     __shared__ double red[2][4];
+    // with a selective second pass: its parts, unless the first pass sufficed
+    if (tparts && !dgks_skip(tparts, pnorm, np, eta2)) pnorm = pnorm2;
     double a = (int)threadIdx.x < np ? pnorm[threadIdx.x] : 0.0;
     double r = (int)threadIdx.x < np ? rparts[threadIdx.x] : 0.0;
     a = wave_sum(a);
@@ -536,9 +581,11 @@ __global__ void __launch_bounds__(256) k_scale_rnorm_parts(uint64_t n, const dou
 }
 
 void launch_scale_rnorm_parts(uint64_t n, const double* src, const double* pnorm, const double* rparts, double* dst,
-                              double* out, hipStream_t stream) {
+                              double* out, hipStream_t stream, const double* tparts, const double* pnorm2,
+                              double eta) {
     const int np = fused_axpy_blocks(n);
-    k_scale_rnorm_parts<<<(unsigned)np, 256, 0, stream>>>(n, src, pnorm, rparts, np, dst, out);
+    k_scale_rnorm_parts<<<(unsigned)np, 256, 0, stream>>>(n, src, pnorm, rparts, np, dst, out, tparts, pnorm2,
+                                                          eta * eta);
     CF_HIP(hipGetLastError());
 }
 
