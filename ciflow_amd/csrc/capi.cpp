@@ -1722,7 +1722,7 @@ struct Davidson {
     uint64_t n;
     int M, keep;
     const double* diag;
-    DBuf<double> V, S, x, hx, t, partial, dg, dy, Yd, dc, pnorm, pnorm2, rparts, tparts;
+    DBuf<double> V, S, x, hx, t, partial, dg, Yd, dc, pnorm, pnorm2, rparts, tparts;
     int nb;
     bool fused;  // small n: step kernels reduce their partials themselves
     int Gd;      // workgroups of the wide dot products
@@ -1733,7 +1733,7 @@ struct Davidson {
     std::function<void(const double*, double*)> op;
 
     Davidson(cf_ctx* ctx, uint64_t n_, int maxb, const double* d) : c(ctx), n(n_), diag(d) {
-        M = (int)std::min<uint64_t>((uint64_t)maxb, n);
+        M = (int)std::min<uint64_t>((uint64_t)std::min(maxb, kDavMaxBasis), n);
         keep = std::max(1, std::min(4, M - 1));
         V.alloc((size_t)M * n);
         S.alloc((size_t)M * n);
@@ -1753,7 +1753,6 @@ struct Davidson {
         rparts.alloc(256);
         tparts.alloc(256);
         dg.alloc(64);
-        dy.alloc(64);
         Yd.alloc(64 * 64);
         hbuf = c->pinned;
         G.assign((size_t)M * M, 0.0);
@@ -1805,7 +1804,7 @@ struct Davidson {
         std::vector<double> A, th, Y;
         double best = std::numeric_limits<double>::infinity();
         int since_best = 0;
-        double* ys = hbuf + 256;  // staged Ritz coefficients (<= 24)
+        double* ys = hbuf + 256;  // Ritz coefficients (<= 24), passed to the kernels by value
         double* Ys = hbuf + 320;  // staged restart rotation (<= 24 x 24)
         for (int it = 0;; ++it) {
             A.assign((size_t)m * m, 0.0);
@@ -1825,11 +1824,10 @@ struct Davidson {
             } else {
                 for (int i = 0; i < m; ++i) ys[i] = Y[(size_t)i * m + 0];
             }
-            CF_HIP(hipMemcpyAsync(dy.p, ys, m * sizeof(double), hipMemcpyHostToDevice, c->stream));
             const double shift = it < krylov ? 0.0 : 1e-4;
             if (fused) {  // 8 launches besides the product
-                launch_combine_residual(n, m, V.p, S.p, dy.p, th[0], shift, diag, x.p, hx.p, t.p, rparts.p,
-                                        tparts.p, c->stream);
+                launch_combine_residual(n, m, V.p, S.p, ys, th[0], shift, diag, x.p, hx.p, t.p, rparts.p, tparts.p,
+                                        c->stream);
                 launch_multi_dot_wide(n, m, V.p, t.p, partial.p, Gd, c->stream);
                 launch_multi_axpy_r(n, m, Gd, partial.p, V.p, t.p, dc.p, pnorm.p, c->stream);
                 // second pass only when the first removed more than half of t
@@ -1844,7 +1842,7 @@ struct Davidson {
                 launch_multi_dot_wide(n, m + 1, V.p, s(m), partial.p, Gd, c->stream);
                 launch_reduce_partials(m + 1, Gd, partial.p, dg.p, c->stream);
             } else {
-                launch_combine2(n, m, V.p, S.p, dy.p, x.p, hx.p, c->stream);
+                launch_combine2(n, m, V.p, S.p, ys, x.p, hx.p, c->stream);
                 launch_dav_residual(n, th[0], shift, x.p, hx.p, diag, t.p, partial.p, nb, c->stream);
                 launch_reduce_partials(1, nb, partial.p, dg.p + 62, c->stream);
                 orthogonalize_t();

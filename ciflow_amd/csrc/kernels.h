@@ -129,6 +129,12 @@ void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, doub
                       hipStream_t stream);
 void launch_reduce_partials(int k, int nblocks, const double* partial, double* out, hipStream_t stream);
 int multi_dot_blocks(uint64_t n);
+// Davidson basis limit and the Ritz coefficients, passed to the kernels by
+// value (host y, k <= kDavMaxBasis).
+constexpr int kDavMaxBasis = 24;
+struct DavCoeffs {
+    double c[kDavMaxBasis];
+};
 // Davidson: x = V y and hx = S y (k columns, one pass); then r = hx - theta x,
 // t = r / (theta - diag) (|theta - diag| clamped to >= shift_min; t = r for
 // shift_min 0) and

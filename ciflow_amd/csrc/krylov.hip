@@ -431,12 +431,12 @@ void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream)
 // Davidson (served solves): Ritz vector and its image in one pass over both
 // bases, then residual, diagonal-preconditioned correction and |r|^2 parts.
 // ---------------------------------------------------------------------------
-// x = V[:, 0:k] y, hx = S[:, 0:k] y
+// x = V[:, 0:k] y, hx = S[:, 0:k] y (y by value)
 __global__ void __launch_bounds__(256) k_combine2(uint64_t n, int k, const double* __restrict__ V,
-                                                  const double* __restrict__ S, const double* __restrict__ y,
+                                                  const double* __restrict__ S, DavCoeffs y,
                                                   double* __restrict__ x, double* __restrict__ hx) {
-    __shared__ double ys[64];
-    if (threadIdx.x < 64) ys[threadIdx.x] = (int)threadIdx.x < k ? y[threadIdx.x] : 0.0;
+    __shared__ double ys[kDavMaxBasis];
+    if (threadIdx.x < kDavMaxBasis) ys[threadIdx.x] = (int)threadIdx.x < k ? y.c[threadIdx.x] : 0.0;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
@@ -452,8 +452,11 @@ __global__ void __launch_bounds__(256) k_combine2(uint64_t n, int k, const doubl
 
 void launch_combine2(uint64_t n, int k, const double* V, const double* S, const double* y, double* x, double* hx,
                      hipStream_t stream) {
+    if (k < 1 || k > kDavMaxBasis) throw Error(ERR_ARG, "combine2: 1 <= k <= 24");
+    DavCoeffs yc{};
+    for (int i = 0; i < k; ++i) yc.c[i] = y[i];
     const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
-    k_combine2<<<grid, 256, 0, stream>>>(n, k, V, S, y, x, hx);
+    k_combine2<<<grid, 256, 0, stream>>>(n, k, V, S, yc, x, hx);
     CF_HIP(hipGetLastError());
 }
 
@@ -498,14 +501,14 @@ void launch_dav_residual(uint64_t n, double theta, double shift_min, const doubl
 // Fused forms for n <= kFusedMaxN (one workgroup per fused_axpy_blocks slot):
 // x, hx, the correction t and this block's part of |r|^2 in one pass ...
 __global__ void __launch_bounds__(256) k_combine_residual(uint64_t n, int k, const double* __restrict__ V,
-                                                          const double* __restrict__ S, const double* __restrict__ y,
+                                                          const double* __restrict__ S, DavCoeffs y,
                                                           double theta, double shift_min,
                                                           const double* __restrict__ diag, double* __restrict__ x,
                                                           double* __restrict__ hx, double* __restrict__ t,
                                                           double* __restrict__ rparts, double* __restrict__ tparts) {
-    __shared__ double ys[64];
+    __shared__ double ys[kDavMaxBasis];
     __shared__ double red[2][4];
-    if (threadIdx.x < 64) ys[threadIdx.x] = (int)threadIdx.x < k ? y[threadIdx.x] : 0.0;
+    if (threadIdx.x < kDavMaxBasis) ys[threadIdx.x] = (int)threadIdx.x < k ? y.c[threadIdx.x] : 0.0;
     __syncthreads();
     double acc = 0.0, acct = 0.0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -544,7 +547,10 @@ __global__ void __launch_bounds__(256) k_combine_residual(uint64_t n, int k, con
 void launch_combine_residual(uint64_t n, int k, const double* V, const double* S, const double* y, double theta,
                              double shift_min, const double* diag, double* x, double* hx, double* t, double* rparts,
                              double* tparts, hipStream_t stream) {
-    k_combine_residual<<<(unsigned)fused_axpy_blocks(n), 256, 0, stream>>>(n, k, V, S, y, theta, shift_min, diag,
+    if (k < 1 || k > kDavMaxBasis) throw Error(ERR_ARG, "combine_residual: 1 <= k <= 24");
+    DavCoeffs yc{};
+    for (int i = 0; i < k; ++i) yc.c[i] = y[i];
+    k_combine_residual<<<(unsigned)fused_axpy_blocks(n), 256, 0, stream>>>(n, k, V, S, yc, theta, shift_min, diag,
                                                                            x, hx, t, rparts, tparts);
     CF_HIP(hipGetLastError());
 }
