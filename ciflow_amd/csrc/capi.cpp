@@ -199,6 +199,7 @@ struct cf_space {
     DBuf<double> diag;
     DBuf<double> vec;  // resident eigenvector after cf_solve
     bool vec_valid = false;
+    bool vec_sign_pending = false;  // sign convention applied when the vector is read
     uint64_t version = 0;  // incremented by cf_space_set_ham
     DBuf<str_t> dup, ddown;  // CF_DETS: the determinant list, list order
 };
@@ -864,6 +865,31 @@ void sign_fix(std::vector<double>& x) {
         if (std::fabs(x[q]) > std::fabs(x[im])) im = q;
     if (!x.empty() && x[im] < 0)
         for (auto& v : x) v = -v;
+}
+
+// The solve's vector becomes the space's resident eigenvector.  With a host
+// buffer it is read back, given the sign convention (largest-magnitude
+// component positive) and returned; without one it stays on the device as it
+// is and the sign is fixed when it is read (cf_space_eigvec), which saves the
+// served solves two copies of n doubles.
+void keep_eigvec(cf_space* sp, uint64_t n, const double* dvec, double* eigvec) {
+    cf_ctx* c = sp->ham->ctx;
+    if (sp->vec.n != n) sp->vec.alloc(n);
+    if (!eigvec) {
+        launch_copy(n, dvec, sp->vec.p, c->stream);
+        CF_HIP(hipStreamSynchronize(c->stream));
+        sp->vec_valid = true;
+        sp->vec_sign_pending = true;
+        return;
+    }
+    std::vector<double> hv(n);
+    CF_HIP(hipMemcpyAsync(hv.data(), dvec, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CF_HIP(hipStreamSynchronize(c->stream));
+    sign_fix(hv);
+    copy_sync(sp->vec.p, hv.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    sp->vec_valid = true;
+    sp->vec_sign_pending = false;
+    std::memcpy(eigvec, hv.data(), n * sizeof(double));
 }
 }  // namespace
 
@@ -1919,14 +1945,7 @@ void run_davidson(cf_space* sp, uint64_t n, const cf_solve_opts& o, std::functio
     info->n_matvec_f32 = nmv32;
     info->n_restart = d.nrestart;
     info->status = status == 0 ? 0 : 1;
-    if (sp->vec.n != n) sp->vec.alloc(n);
-    std::vector<double> hv(n);
-    CF_HIP(hipMemcpyAsync(hv.data(), d.x.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CF_HIP(hipStreamSynchronize(c->stream));
-    sign_fix(hv);
-    copy_sync(sp->vec.p, hv.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    sp->vec_valid = true;
-    if (eigvec) std::memcpy(eigvec, hv.data(), n * sizeof(double));
+    keep_eigvec(sp, n, d.x.p, eigvec);
     info->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -1977,14 +1996,7 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     info->n_matvec_f32 = nmv32;
     info->n_restart = nrs + sm.n_restart();
     info->status = sm.status();
-    if (sp->vec.n != n) sp->vec.alloc(n);
-    std::vector<double> hv(n);
-    CF_HIP(hipMemcpyAsync(hv.data(), b.v(0), n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CF_HIP(hipStreamSynchronize(c->stream));
-    sign_fix(hv);
-    copy_sync(sp->vec.p, hv.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    sp->vec_valid = true;
-    if (eigvec) std::memcpy(eigvec, hv.data(), n * sizeof(double));
+    keep_eigvec(sp, n, b.v(0), eigvec);
     info->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 }  // namespace
@@ -2062,6 +2074,11 @@ int cf_space_eigvec(cf_space* sp, double* eigvec) {
     CF_REQUIRE(sp->vec_valid, "no solve has run on this space");
     sp->ham->ctx->use();
     copy_sync(eigvec, sp->vec.p, sp->n * sizeof(double), hipMemcpyDeviceToHost, sp->ham->ctx->stream);
+    if (sp->vec_sign_pending) {
+        std::vector<double> hv(eigvec, eigvec + sp->n);
+        sign_fix(hv);
+        std::memcpy(eigvec, hv.data(), sp->n * sizeof(double));
+    }
     API_END
 }
 

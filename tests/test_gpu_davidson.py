@@ -117,3 +117,19 @@ def test_served_davidson_alone_equals_in_mix_and_matches_oracle(method):
         m = O.build_crs("DOCI" if r.kind == CA.CF_DOCI else "FCI", hr, Vr, r.n, r.n)
         E, _, info = O.lanczos(O.full_from_upper(m), tol=1e-12)
         assert abs(b["energy"] - E) < 1e-8, (r.kind, r.L, b["energy"], E)
+
+
+@pytest.mark.parametrize("method", [CA.CF_LANCZOS, CA.CF_DAVIDSON])
+def test_resident_eigvec_read_back_with_sign_convention(method):
+    """A solve without a host buffer leaves the vector on the device; reading
+    it gives the same vector, sign convention included (largest-magnitude
+    component positive), as a solve that returns it."""
+    ham = CA.Hamiltonian(*CA.gen_integrals(9, 2), 4, 4)
+    d = CA.FCI_Direct(ham)
+    d.solve(want_vec=False, method=method)
+    c = np.zeros(d.get_dim())
+    CA._check(CA.lib().cf_space_eigvec(d.ptr, c.ctypes.data_as(CA.dp)))
+    d.solve(want_vec=True, method=method)
+    v = d.get_eigvec()
+    assert np.array_equal(c, v)
+    assert v[np.argmax(np.abs(v))] > 0
