@@ -256,9 +256,11 @@ typedef struct {
 
 void cf_solve_opts_default(cf_solve_opts* o);
 /* eigvec: host array of n doubles or NULL.  The vector also stays on the GPU
- * for cf_rdm_resident. */
+ * as the space's resident eigenvector (NULL: it is not read back at all). */
 int cf_solve(cf_crs* m, const cf_solve_opts* opts, double* eigvec, cf_solve_info* info);
-/* The resident eigenvector of the last solve on this space (host out, n). */
+/* The resident eigenvector of the last solve on this space (host out, n), with
+ * the sign convention of the returned vectors: largest-magnitude component
+ * positive. */
 int cf_space_eigvec(cf_space* sp, double* eigvec);
 
 /* ---- on-the-fly sigma for FCI spaces too large to store (replaces the
