@@ -133,3 +133,18 @@ def test_resident_eigvec_read_back_with_sign_convention(method):
     v = d.get_eigvec()
     assert np.array_equal(c, v)
     assert v[np.argmax(np.abs(v))] > 0
+
+
+def test_davidson_fp32_phase_stops_at_its_noise_floor():
+    """An FP32-phase tolerance the FP32 operator cannot reach (1e-14) ends the
+    phase when its residual stalls, not at the iteration cap, and the FP64
+    polish still lands on the FP64 energy."""
+    ham = CA.Hamiltonian(*CA.gen_integrals(10, 3), 5, 5)
+    cim = CA.FCI_Direct(ham, value_prec=CA.CF_F32)
+    cim.solve()
+    e64 = cim.get_ci_energy()
+    info = cim.solve(precision=CA.CF_F32, method=CA.CF_DAVIDSON, tol_f32=1e-14)
+    say("stall", info.n_matvec_f32, info.n_matvec)
+    assert info.status == 0
+    assert info.n_matvec_f32 < 2000
+    assert abs(cim.get_ci_energy() - e64) < 1e-10
