@@ -209,9 +209,13 @@ __global__ void __launch_bounds__(T) k_spmv_doci_blk(DociBlockArgs a, const VT* 
     __shared__ uint32_t sg_base[kBlkStages], sg_orb[kBlkStages];
     const int L = a.L, w = a.w, nh = L - w;
     const uint32_t W = 1u << w, wmask = W - 1;
-    double* xo = smem_blk;                 // own block
-    double* sbuf = smem_blk + kBlkRows;    // two staged neighbour blocks
-    VT* tab = reinterpret_cast<VT*>(smem_blk + 3 * kBlkRows);
+    // two staged-block buffers; the own block lives in the second one, which
+    // the first staged block does not touch and the second is written only
+    // after every thread has passed the first stage's barrier (and so has
+    // finished the own-block hops)
+    double* sbuf = smem_blk;
+    double* xo = smem_blk + kBlkRows;
+    VT* tab = reinterpret_cast<VT*>(smem_blk + 2 * kBlkRows);
     uint16_t* rlo = reinterpret_cast<uint16_t*>(tab + L * L);
     const int tid = threadIdx.x;
     const DociBlock blk = a.blocks[blockIdx.x];
@@ -366,7 +370,7 @@ static void launch_doci_blk(const DociBlockArgs& a, const VT* vk, const double* 
     if (!a.nblocks) return;
     CF_REQUIRE(a.w >= 1 && a.w <= kDociMaxW && a.L - a.w >= 1 && a.L - a.w <= 20,
                "block-staged DOCI SpMV: 1 <= w <= 12 low orbitals and 1..20 high orbitals");
-    const size_t shm = 3 * kBlkRows * sizeof(double) + (size_t)a.L * a.L * sizeof(VT) +
+    const size_t shm = 2 * kBlkRows * sizeof(double) + (size_t)a.L * a.L * sizeof(VT) +
                        ((size_t)1 << a.w) * sizeof(uint16_t);
     // rows of the largest block over the workgroup (T * ROWS >= C(w, w/2)); the
     // neighbour-block bases are computed one per thread (T >= ch (nh - ch))
