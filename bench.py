@@ -334,11 +334,12 @@ def bench_fci12(a, CA):
     h, V = CA.gen_integrals(a.L, a.seed)
     ham = CA.Hamiltonian(h, V, a.nel, a.nel, 0.0, ctx)
     prec = CA.CF_F32 if a.precision == "f32" else CA.CF_F64
+    method = {"auto": CA.CF_SOLVER_AUTO, "lanczos": CA.CF_LANCZOS, "davidson": CA.CF_DAVIDSON}[a.solver]
     ctx.synchronize()
 
     def step():
         cim = cls(ham, value_prec=prec)
-        info = cim.solve(precision=prec)
+        info = cim.solve(precision=prec, method=method)
         return cim, info
 
     for i in range(W):
@@ -404,9 +405,11 @@ def bench_fci12(a, CA):
         "dtype": "f64" if prec == CA.CF_F64 else "f32-values/f64-accumulate+f64-polish",
         "data": "synthetic",
         "config": {"workload": (f"DOCI L={a.L} {a.nel} pairs" if doci else f"FCI L={a.L} {a.nel}a{a.nel}b")
-                   + f" seed {a.seed}: GPU CRS build + Lanczos ground state",
+                   + f" seed {a.seed}: GPU CRS build + ground state",
                    "determinants": n, "nnz_full": nnz_full, "nnz_upper": nnz_upper, "tol": 1e-12,
-                   "precision": a.precision},
+                   "precision": a.precision,
+                   "solver": {"auto": "Davidson (spaces of >= 2,000 determinants)", "lanczos": "thick-restart Lanczos",
+                              "davidson": "Davidson"}[a.solver]},
         "p50_ms": float(np.percentile(lat_ms, 50)),
         "p99_ms": float(np.percentile(lat_ms, 99)),
         "energy": energy,

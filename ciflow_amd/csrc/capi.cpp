@@ -1789,14 +1789,6 @@ struct Davidson {
         launch_multi_dot(n, k, base, w, partial.p, nb, c->stream);
         launch_reduce_partials(k, nb, partial.p, out, c->stream);
     }
-    // t <- (t - V V^T t) twice, then its squared norm into dg[63]
-    void orthogonalize_t() {
-        for (int pass = 0; pass < 2; ++pass) {
-            dots(m, V.p, t.p, dg.p);
-            launch_multi_axpy(n, m, V.p, dg.p, t.p, c->stream);
-        }
-        dots(1, t.p, t.p, dg.p + 63);
-    }
     void add_column() {
         op(v(m), s(m));
         ++napply;
@@ -1867,11 +1859,18 @@ struct Davidson {
                 ++napply;
                 launch_multi_dot_wide(n, m + 1, V.p, s(m), partial.p, Gd, c->stream);
                 launch_reduce_partials(m + 1, Gd, partial.p, dg.p, c->stream);
-            } else {
-                launch_combine2(n, m, V.p, S.p, ys, x.p, hx.p, c->stream);
-                launch_dav_residual(n, th[0], shift, x.p, hx.p, diag, t.p, partial.p, nb, c->stream);
-                launch_reduce_partials(1, nb, partial.p, dg.p + 62, c->stream);
-                orthogonalize_t();
+            } else {  // two passes over V (and one over S) before the product
+                // x, t, first-pass dots V^T t -> dg[0, m), |t|^2 -> dg[61], |r|^2 -> dg[62]
+                launch_combine_res_dot(n, m, V.p, S.p, ys, th[0], shift, diag, x.p, t.p, partial.p, nb, 61, 62,
+                                       c->stream);
+                launch_reduce_partials(m, nb, partial.p, dg.p, c->stream);
+                launch_reduce_partials(2, nb, partial.p + (size_t)61 * nb, dg.p + 61, c->stream);
+                // t -= V dg[0, m); second-pass dots -> dg[0, m), |t'|^2 -> dg[m]
+                launch_axpy_dot(n, m, V.p, dg.p, t.p, partial.p, nb, m, c->stream);
+                launch_reduce_partials(m + 1, nb, partial.p, dg.p, c->stream);
+                // second pass only when the first removed more than half of t
+                launch_multi_axpy(n, m, V.p, dg.p, t.p, c->stream, dg.p + 61, dg.p + m, kDgksEta);
+                dots(1, t.p, t.p, dg.p + 63);
                 launch_scale_rnorm(n, t.p, dg.p + 63, v(m), c->stream);
                 op(v(m), s(m));
                 ++napply;
@@ -1959,7 +1958,12 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     CF_REQUIRE(o.check_every >= 0, "check_every must be >= 0");
     CF_REQUIRE(o.tol_energy >= 0.0, "tol_energy must be >= 0");
     CF_REQUIRE(o.precision == CF_F64 || o.precision == CF_F32, "precision must be CF_F64 or CF_F32");
-    CF_REQUIRE(o.method == CF_LANCZOS || o.method == CF_DAVIDSON, "method must be CF_LANCZOS or CF_DAVIDSON");
+    CF_REQUIRE(o.method == CF_LANCZOS || o.method == CF_DAVIDSON || o.method == CF_SOLVER_AUTO,
+               "method must be CF_LANCZOS, CF_DAVIDSON or CF_SOLVER_AUTO");
+    // measured on MI355X, FP64 solves to tol 1e-12: FCI L=12 6a6b 476 -> 417 ms,
+    // DOCI L=24 12 pairs 136 -> 111 ms with Davidson; below a few thousand
+    // determinants the solves are launch-bound and Lanczos is as fast
+    if (o.method == CF_SOLVER_AUTO) o.method = (sp->diag.p && n >= 2000) ? CF_DAVIDSON : CF_LANCZOS;
     if (o.method == CF_DAVIDSON) {
         run_davidson(sp, n, o, op64, op32, eigvec, info);
         return;

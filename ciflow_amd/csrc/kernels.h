@@ -155,8 +155,21 @@ void launch_scale_rnorm_parts(uint64_t n, const double* src, const double* pnorm
                               const double* pnorm2 = nullptr, double eta = 0.0);
 void launch_dav_residual(uint64_t n, double theta, double shift_min, const double* x, const double* hx,
                          const double* diag, double* t, double* partial, int nblocks, hipStream_t stream);
+// Large-n Davidson step in two basis passes (nblocks = multi_dot_blocks(n),
+// partial of 64 rows x nblocks): x = V y and t from r = S y - theta x as
+// launch_dav_residual, with parts of V^T t in rows 0..k-1, |t|^2 in row_t and
+// |r|^2 in row_r; then t -= V c with parts of V^T t (rows 0..k-1) and |t|^2
+// (row_t) of the updated t.
+void launch_combine_res_dot(uint64_t n, int k, const double* V, const double* S, const double* y, double theta,
+                            double shift_min, const double* diag, double* x, double* t, double* partial, int nblocks,
+                            int row_t, int row_r, hipStream_t stream);
+void launch_axpy_dot(uint64_t n, int k, const double* V, const double* c, double* t, double* partial, int nblocks,
+                     int row_t, hipStream_t stream);
 
-void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream);
+// w -= Vb[:, 0:k] c; with gate_before/gate_after (|w|^2 before and after a
+// first Gram-Schmidt pass, device memory) skipped when after > eta^2 before.
+void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream,
+                       const double* gate_before = nullptr, const double* gate_after = nullptr, double eta = 0.0);
 void launch_local3(uint64_t n, const double* a, const double* v1, const double* b, const double* v0, double* w,
                    hipStream_t stream);
 void launch_finish_step(uint64_t n, int j, const double* c1, const double* c2, const double* wnorm2,

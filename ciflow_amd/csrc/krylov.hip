@@ -102,10 +102,15 @@ void launch_reduce_partials(int k, int nblocks, const double* partial, double* o
     CF_HIP(hipGetLastError());
 }
 
-// w -= Vb[:, 0:k] c
+// w -= Vb[:, 0:k] c; with gate = {before, after} (two |w|^2 sums in device
+// memory), nothing is done when after > eta2 before (the selective second
+// Gram-Schmidt pass; every block reads the same two values)
 __global__ void __launch_bounds__(256) k_multi_axpy(uint64_t n, int k, const double* __restrict__ Vb,
-                                                    const double* __restrict__ c, double* __restrict__ w) {
+                                                    const double* __restrict__ c, double* __restrict__ w,
+                                                    const double* __restrict__ gate_before,
+                                                    const double* __restrict__ gate_after, double eta2) {
     __shared__ double cs[64];
+    if (gate_before && *gate_after > eta2 * *gate_before) return;
     if (threadIdx.x < 64) cs[threadIdx.x] = (int)threadIdx.x < k ? c[threadIdx.x] : 0.0;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -116,12 +121,13 @@ __global__ void __launch_bounds__(256) k_multi_axpy(uint64_t n, int k, const dou
     }
 }
 
-void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream) {
+void launch_multi_axpy(uint64_t n, int k, const double* Vb, const double* c, double* w, hipStream_t stream,
+                       const double* gate_before, const double* gate_after, double eta) {
     if (k <= 0 || !n) return;
     if (k > 64) throw Error(ERR_ARG, "multi_axpy: k > 64");
     uint64_t grid = (n + 255) / 256;
     if (grid > 8192) grid = 8192;
-    k_multi_axpy<<<(unsigned)grid, 256, 0, stream>>>(n, k, Vb, c, w);
+    k_multi_axpy<<<(unsigned)grid, 256, 0, stream>>>(n, k, Vb, c, w, gate_before, gate_after, eta * eta);
     CF_HIP(hipGetLastError());
 }
 
@@ -497,6 +503,162 @@ void launch_dav_residual(uint64_t n, double theta, double shift_min, const doubl
     k_dav_residual<<<(unsigned)nblocks, kDotThreads, 0, stream>>>(n, theta, shift_min, x, hx, diag, t, partial);
     CF_HIP(hipGetLastError());
 }
+
+// Large-n Davidson step in two passes over the basis instead of four: V
+// values are kept in registers so that each pass also forms the next dot
+// products.  The basis size K is a template parameter so that every load of
+// an element's K values is issued before the first is used.  Block parts go
+// to partial[row * gridDim.x + block] (the layout of k_multi_dot, reduced by
+// k_reduce_partials): rows 0..K-1 the dots, row_a the first norm, row_b the
+// second (skipped when < 0).
+template <int K>
+__device__ __forceinline__ void block_parts(const double (&acc)[K + 2], int row_a, int row_b,
+                                            double* __restrict__ partial) {
+    __shared__ double red[K + 2][kDotThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < K + 2; ++j) {
+        const double s = wave_sum(acc[j]);
+        if (lane == 0) red[j][wid] = s;
+    }
+    __syncthreads();
+    const int i = threadIdx.x;
+    const int row = i < K ? i : (i == K ? row_a : row_b);
+    if (i < K + 2 && row >= 0) {
+        double s = 0.0;
+        for (int q = 0; q < kDotThreads / 64; ++q) s += red[i][q];
+        partial[(uint64_t)row * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// x = V y, r = S y - theta x, t as k_dav_residual; parts of V^T t (rows
+// 0..K-1), |t|^2 (row row_t) and |r|^2 (row row_r)
+template <int K>
+__global__ void __launch_bounds__(kDotThreads) k_combine_res_dot(uint64_t n, const double* __restrict__ V,
+                                                                 const double* __restrict__ S, DavCoeffs y,
+                                                                 double theta, double shift_min,
+                                                                 const double* __restrict__ diag,
+                                                                 double* __restrict__ x, double* __restrict__ t,
+                                                                 double* __restrict__ partial, int row_t, int row_r) {
+    double acc[K + 2];
+#pragma unroll
+    for (int j = 0; j < K + 2; ++j) acc[j] = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * kDotThreads;
+    for (uint64_t q = (uint64_t)blockIdx.x * kDotThreads + threadIdx.x; q < n; q += stride) {
+        double vq[K], sq[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            vq[i] = V[(uint64_t)i * n + q];
+            sq[i] = S[(uint64_t)i * n + q];
+        }
+        const double dq = shift_min > 0.0 ? diag[q] : 0.0;
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            a += y.c[i] * vq[i];
+            b += y.c[i] * sq[i];
+        }
+        x[q] = a;
+        const double r = b - theta * a;
+        double tq = r;
+        if (shift_min > 0.0) {
+            double den = theta - dq;
+            if (fabs(den) < shift_min) den = den < 0.0 ? -shift_min : shift_min;
+            tq = r / den;
+        }
+        t[q] = tq;
+#pragma unroll
+        for (int i = 0; i < K; ++i) acc[i] += vq[i] * tq;
+        acc[K] += tq * tq;
+        acc[K + 1] += r * r;
+    }
+    block_parts<K>(acc, row_t, row_r, partial);
+}
+
+// t -= V c (c: K coefficients in device memory), then parts of V^T t (rows
+// 0..K-1) and |t|^2 (row row_t)
+template <int K>
+__global__ void __launch_bounds__(kDotThreads) k_axpy_dot(uint64_t n, const double* __restrict__ V,
+                                                          const double* __restrict__ c, double* __restrict__ t,
+                                                          double* __restrict__ partial, int row_t) {
+    __shared__ double cs[K];
+    if (threadIdx.x < K) cs[threadIdx.x] = c[threadIdx.x];
+    __syncthreads();
+    double acc[K + 2];
+#pragma unroll
+    for (int j = 0; j < K + 2; ++j) acc[j] = 0.0;
+    const uint64_t stride = (uint64_t)gridDim.x * kDotThreads;
+    for (uint64_t q = (uint64_t)blockIdx.x * kDotThreads + threadIdx.x; q < n; q += stride) {
+        double vq[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) vq[i] = V[(uint64_t)i * n + q];
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) s += cs[i] * vq[i];
+        const double tq = t[q] - s;
+        t[q] = tq;
+#pragma unroll
+        for (int i = 0; i < K; ++i) acc[i] += vq[i] * tq;
+        acc[K] += tq * tq;
+    }
+    block_parts<K>(acc, row_t, -1, partial);
+}
+
+template <int K>
+static void combine_res_dot_k(uint64_t n, const double* V, const double* S, const DavCoeffs& y, double theta,
+                              double shift_min, const double* diag, double* x, double* t, double* partial,
+                              int nblocks, int row_t, int row_r, hipStream_t stream) {
+    k_combine_res_dot<K><<<(unsigned)nblocks, kDotThreads, 0, stream>>>(n, V, S, y, theta, shift_min, diag, x, t,
+                                                                        partial, row_t, row_r);
+}
+
+template <int K>
+static void axpy_dot_k(uint64_t n, const double* V, const double* c, double* t, double* partial, int nblocks,
+                       int row_t, hipStream_t stream) {
+    k_axpy_dot<K><<<(unsigned)nblocks, kDotThreads, 0, stream>>>(n, V, c, t, partial, row_t);
+}
+
+// one instance per basis size 1..kDavMaxBasis
+#define CF_DAV_SIZES(X)                                                                                         \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) \
+        X(20) X(21) X(22) X(23) X(24)
+static_assert(kDavMaxBasis == 24, "CF_DAV_SIZES lists basis sizes 1..24");
+
+void launch_combine_res_dot(uint64_t n, int k, const double* V, const double* S, const double* y, double theta,
+                            double shift_min, const double* diag, double* x, double* t, double* partial, int nblocks,
+                            int row_t, int row_r, hipStream_t stream) {
+    if (k < 1 || k > kDavMaxBasis) throw Error(ERR_ARG, "combine_res_dot: 1 <= k <= 24");
+    if (row_t < k || row_r < k || row_t == row_r || row_t >= 64 || row_r >= 64)
+        throw Error(ERR_ARG, "combine_res_dot: norm rows must lie in [k, 64) and differ");
+    DavCoeffs yc{};
+    for (int i = 0; i < k; ++i) yc.c[i] = y[i];
+    switch (k) {
+#define CF_CASE(K)                                                                                       \
+    case K:                                                                                              \
+        combine_res_dot_k<K>(n, V, S, yc, theta, shift_min, diag, x, t, partial, nblocks, row_t, row_r, \
+                             stream);                                                                    \
+        break;
+        CF_DAV_SIZES(CF_CASE)
+#undef CF_CASE
+    }
+    CF_HIP(hipGetLastError());
+}
+
+void launch_axpy_dot(uint64_t n, int k, const double* V, const double* c, double* t, double* partial, int nblocks,
+                     int row_t, hipStream_t stream) {
+    if (k < 1 || k > kDavMaxBasis) throw Error(ERR_ARG, "axpy_dot: 1 <= k <= 24");
+    if (row_t < k || row_t >= 64) throw Error(ERR_ARG, "axpy_dot: norm row must lie in [k, 64)");
+    switch (k) {
+#define CF_CASE(K)                                                        \
+    case K:                                                               \
+        axpy_dot_k<K>(n, V, c, t, partial, nblocks, row_t, stream);       \
+        break;
+        CF_DAV_SIZES(CF_CASE)
+#undef CF_CASE
+    }
+    CF_HIP(hipGetLastError());
+}
+#undef CF_DAV_SIZES
 
 // Fused forms for n <= kFusedMaxN (one workgroup per fused_axpy_blocks slot):
 // x, hx, the correction t and this block's part of |r|^2 in one pass ...

@@ -238,7 +238,9 @@ typedef struct {
      * preconditioned Davidson from the Lanczos start vector (8 plain Krylov
      * steps first), basis of min(ncv, 24) vectors restarted to 4 Ritz
      * vectors; same precision phases and stopping tests.  Needs a space with
-     * a diagonal (FCI, DOCI). */
+     * a diagonal (FCI, DOCI).
+     * CF_SOLVER_AUTO: CF_DAVIDSON for a space with a diagonal and at least
+     * 2,000 determinants, else CF_LANCZOS. */
     int method;
 } cf_solve_opts;
 enum { CF_LANCZOS = 0, CF_DAVIDSON = 1, CF_SOLVER_AUTO = 2 };

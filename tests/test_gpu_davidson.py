@@ -42,7 +42,7 @@ def residual(cim, vec):
 
 
 @pytest.mark.parametrize("case", ["fci_L8_4a4b", "fci_L8_4a3b", "fci_L10_seed8", "doci_L12", "doci_L20_blk",
-                                  "direct_L9"])
+                                  "direct_L9", "fci_L11_large_n"])
 def test_davidson_fp64_equals_lanczos(case):
     say("davidson", case)
     if case == "fci_L8_4a4b":
@@ -53,6 +53,8 @@ def test_davidson_fp64_equals_lanczos(case):
         ham, cls = CA.Hamiltonian(*CA.gen_integrals(10, 8), 5, 5), CA.FCI
     elif case == "doci_L12":
         ham, cls = CA.Hamiltonian(*CA.gen_integrals(12, 5), 6, 6), CA.DOCI
+    elif case == "fci_L11_large_n":  # 213,444 determinants: the two-pass large-n step
+        ham, cls = CA.Hamiltonian(*CA.gen_integrals(11, 2), 5, 5), CA.FCI
     elif case == "doci_L20_blk":
         ham, cls = CA.Hamiltonian(*CA.gen_integrals(20, 6), 10, 10), CA.DOCI
     else:
@@ -79,10 +81,13 @@ def test_davidson_reference_kat():
     assert abs(cim.get_ci_energy() - (-15.225362047972723)) < 1e-10
 
 
-@pytest.mark.parametrize("kind", ["FCI", "DOCI", "direct"])
+@pytest.mark.parametrize("kind", ["FCI", "DOCI", "direct", "FCI_large_n"])
 def test_davidson_fp32_polish_within_1e8(kind):
     say("davidson f32", kind)
-    if kind == "DOCI":
+    if kind == "FCI_large_n":
+        ham = CA.Hamiltonian(*CA.gen_integrals(11, 5), 5, 5)
+        cim = CA.FCI(ham, value_prec=CA.CF_F32)
+    elif kind == "DOCI":
         ham = CA.Hamiltonian(*CA.gen_integrals(16, 8), 8, 8)
         cim = CA.DOCI(ham, value_prec=CA.CF_F32)
     elif kind == "FCI":
@@ -148,3 +153,17 @@ def test_davidson_fp32_phase_stops_at_its_noise_floor():
     assert info.status == 0
     assert info.n_matvec_f32 < 2000
     assert abs(cim.get_ci_energy() - e64) < 1e-10
+
+
+def test_solve_auto_picks_by_size():
+    """method=CF_SOLVER_AUTO on a single solve: Davidson for a space of at
+    least 2,000 determinants (FCI L=8 4a4b, 4,900), Lanczos below (DOCI L=10,
+    252) -- the same products and energy as asking for that solver."""
+    for cls, (L, n, seed), want in ((CA.FCI, (8, 4, 0), CA.CF_DAVIDSON), (CA.DOCI, (10, 5, 2), CA.CF_LANCZOS)):
+        ham = CA.Hamiltonian(*CA.gen_integrals(L, seed), n, n)
+        cim = cls(ham)
+        auto = cim.solve(method=CA.CF_SOLVER_AUTO)
+        e_auto = cim.get_ci_energy()
+        ref = cim.solve(method=want)
+        say("auto", cls.__name__, auto.n_matvec, ref.n_matvec)
+        assert auto.n_matvec == ref.n_matvec and e_auto == cim.get_ci_energy()
