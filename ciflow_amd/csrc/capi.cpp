@@ -1811,7 +1811,8 @@ struct Davidson {
     // Iterates to convergence; x holds the Ritz vector (|x| = 1) on return.
     // Returns 0 converged, 1 iteration limit, 2 breakdown, 3 stalled (with
     // stall > 0: no residual 10% below the best one for `stall` iterations,
-    // the noise floor of a reduced-precision operator).  One host wait per
+    // the noise floor of a reduced-precision operator; the best one is reset
+    // when the Ritz value falls by more than it).  One host wait per
     // iteration: the next basis vector and its product are formed before the
     // residual of the current Ritz vector is read back, so a converged solve
     // ends with one product that is not used.  The first `krylov` expansions
@@ -1820,7 +1821,7 @@ struct Davidson {
     int run(int maxit, int krylov, double tol, double tol_energy, int stall,
             const std::function<void(int, double, double)>& progress, double* theta_out, double* res_out) {
         std::vector<double> A, th, Y;
-        double best = std::numeric_limits<double>::infinity();
+        double best = std::numeric_limits<double>::infinity(), theta_best = 0.0;
         int since_best = 0;
         double* ys = hbuf + 256;  // Ritz coefficients (<= 24), passed to the kernels by value
         double* Ys = hbuf + 320;  // staged restart rotation (<= 24 x 24)
@@ -1886,8 +1887,12 @@ struct Davidson {
             if (tol_energy > 0.0 && th.size() >= 2 && res * res <= tol_energy * (th[1] - th[0])) return 0;
             if (!(hbuf[63] > 1e-28)) return 2;
             if (it >= maxit) return 1;
-            if (res < 0.9 * best) {
+            // a Ritz value that fell by more than the best residual belongs
+            // to another eigenvalue (|theta - lambda| <= |r|): measure the
+            // stall from there
+            if (res < 0.9 * best || th[0] < theta_best - best) {
                 best = res;
+                theta_best = th[0];
                 since_best = 0;
             } else if (stall > 0 && ++since_best >= stall) {
                 return 3;
