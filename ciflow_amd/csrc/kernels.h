@@ -135,12 +135,8 @@ constexpr int kDavMaxBasis = 24;
 struct DavCoeffs {
     double c[kDavMaxBasis];
 };
-// Davidson: x = V y and hx = S y (k columns, one pass); then r = hx - theta x,
-// t = r / (theta - diag) (|theta - diag| clamped to >= shift_min; t = r for
-// shift_min 0) and
-// partial[b] = block b's part of |r|^2 (nblocks = multi_dot_blocks(n)).
-void launch_combine2(uint64_t n, int k, const double* V, const double* S, const double* y, double* x, double* hx,
-                     hipStream_t stream);
+// dst = src / sqrt(*norm2), the squared norm in device memory (zeros when it
+// is <= 1e-28)
 void launch_scale_rnorm(uint64_t n, const double* src, const double* norm2, double* dst, hipStream_t stream);
 // Fused Davidson step for n <= kFusedMaxN: x, hx, t and per-block |r|^2 parts
 // (fused_axpy_blocks(n) of them) in one pass; then, after two multi_axpy_r
@@ -153,11 +149,10 @@ void launch_combine_residual(uint64_t n, int k, const double* V, const double* S
 void launch_scale_rnorm_parts(uint64_t n, const double* src, const double* pnorm, const double* rparts, double* dst,
                               double* out, hipStream_t stream, const double* tparts = nullptr,
                               const double* pnorm2 = nullptr, double eta = 0.0);
-void launch_dav_residual(uint64_t n, double theta, double shift_min, const double* x, const double* hx,
-                         const double* diag, double* t, double* partial, int nblocks, hipStream_t stream);
 // Large-n Davidson step in two basis passes (nblocks = multi_dot_blocks(n),
-// partial of 64 rows x nblocks): x = V y and t from r = S y - theta x as
-// launch_dav_residual, with parts of V^T t in rows 0..k-1, |t|^2 in row_t and
+// partial of 64 rows x nblocks): x = V y, r = S y - theta x and the correction
+// t = r / (theta - diag) (|theta - diag| clamped to >= shift_min; t = r for
+// shift_min 0), with parts of V^T t in rows 0..k-1, |t|^2 in row_t and
 // |r|^2 in row_r; then t -= V c with parts of V^T t (rows 0..k-1) and |t|^2
 // (row_t) of the updated t.
 void launch_combine_res_dot(uint64_t n, int k, const double* V, const double* S, const double* y, double theta,

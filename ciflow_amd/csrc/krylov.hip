@@ -434,76 +434,11 @@ void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream)
 }
 
 // ---------------------------------------------------------------------------
-// Davidson (served solves): Ritz vector and its image in one pass over both
-// bases, then residual, diagonal-preconditioned correction and |r|^2 parts.
+// Davidson: Ritz vector x = V y and its image S y in one pass over both
+// bases, then the residual r = S y - theta x, the diagonal-preconditioned
+// correction t = r / (theta - diag) (|theta - diag| clamped to >= shift_min;
+// t = r when shift_min == 0: a plain Krylov expansion) and norm parts.
 // ---------------------------------------------------------------------------
-// x = V[:, 0:k] y, hx = S[:, 0:k] y (y by value)
-__global__ void __launch_bounds__(256) k_combine2(uint64_t n, int k, const double* __restrict__ V,
-                                                  const double* __restrict__ S, DavCoeffs y,
-                                                  double* __restrict__ x, double* __restrict__ hx) {
-    __shared__ double ys[kDavMaxBasis];
-    if (threadIdx.x < kDavMaxBasis) ys[threadIdx.x] = (int)threadIdx.x < k ? y.c[threadIdx.x] : 0.0;
-    __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
-        double a = 0.0, b = 0.0;
-        for (int i = 0; i < k; ++i) {
-            a += ys[i] * V[(uint64_t)i * n + q];
-            b += ys[i] * S[(uint64_t)i * n + q];
-        }
-        x[q] = a;
-        hx[q] = b;
-    }
-}
-
-void launch_combine2(uint64_t n, int k, const double* V, const double* S, const double* y, double* x, double* hx,
-                     hipStream_t stream) {
-    if (k < 1 || k > kDavMaxBasis) throw Error(ERR_ARG, "combine2: 1 <= k <= 24");
-    DavCoeffs yc{};
-    for (int i = 0; i < k; ++i) yc.c[i] = y[i];
-    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 2048);
-    k_combine2<<<grid, 256, 0, stream>>>(n, k, V, S, yc, x, hx);
-    CF_HIP(hipGetLastError());
-}
-
-// r = hx - theta x; t = r / (theta - diag) with |theta - diag| >= shift_min
-// (t = r when shift_min == 0: a plain Krylov expansion); partial[b] = this
-// block's part of |r|^2
-__global__ void __launch_bounds__(kDotThreads) k_dav_residual(uint64_t n, double theta, double shift_min,
-                                                              const double* __restrict__ x,
-                                                              const double* __restrict__ hx,
-                                                              const double* __restrict__ diag,
-                                                              double* __restrict__ t, double* __restrict__ partial) {
-    __shared__ double red[kDotThreads / 64];
-    double acc = 0.0;
-    const uint64_t stride = (uint64_t)gridDim.x * kDotThreads;
-    for (uint64_t q = (uint64_t)blockIdx.x * kDotThreads + threadIdx.x; q < n; q += stride) {
-        const double r = hx[q] - theta * x[q];
-        acc += r * r;
-        if (shift_min > 0.0) {
-            double den = theta - diag[q];
-            if (fabs(den) < shift_min) den = den < 0.0 ? -shift_min : shift_min;
-            t[q] = r / den;
-        } else {
-            t[q] = r;
-        }
-    }
-    const double s = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0;
-        for (int q = 0; q < kDotThreads / 64; ++q) a += red[q];
-        partial[blockIdx.x] = a;
-    }
-}
-
-void launch_dav_residual(uint64_t n, double theta, double shift_min, const double* x, const double* hx,
-                         const double* diag, double* t, double* partial, int nblocks, hipStream_t stream) {
-    k_dav_residual<<<(unsigned)nblocks, kDotThreads, 0, stream>>>(n, theta, shift_min, x, hx, diag, t, partial);
-    CF_HIP(hipGetLastError());
-}
-
 // Large-n Davidson step in two passes over the basis instead of four: V
 // values are kept in registers so that each pass also forms the next dot
 // products.  The basis size K is a template parameter so that every load of
@@ -531,7 +466,7 @@ __device__ __forceinline__ void block_parts(const double (&acc)[K + 2], int row_
     }
 }
 
-// x = V y, r = S y - theta x, t as k_dav_residual; parts of V^T t (rows
+// x = V y, r = S y - theta x, the correction t; parts of V^T t (rows
 // 0..K-1), |t|^2 (row row_t) and |r|^2 (row row_r)
 template <int K>
 __global__ void __launch_bounds__(kDotThreads) k_combine_res_dot(uint64_t n, const double* __restrict__ V,
