@@ -209,13 +209,13 @@ __global__ void __launch_bounds__(T) k_spmv_doci_blk(DociBlockArgs a, const VT* 
     __shared__ uint32_t sg_base[kBlkStages], sg_orb[kBlkStages];
     const int L = a.L, w = a.w, nh = L - w;
     const uint32_t W = 1u << w, wmask = W - 1;
-    // two staged-block buffers; the own block lives in the second one, which
-    // the first staged block does not touch and the second is written only
-    // after every thread has passed the first stage's barrier (and so has
-    // finished the own-block hops)
+    // one x buffer: the own block, then each staged block in turn; a barrier
+    // before every stage's write lets all threads finish reading the previous
+    // contents (one buffer instead of three leaves room for more resident
+    // workgroups per CU, the kernel's limit)
+    double* xo = smem_blk;
     double* sbuf = smem_blk;
-    double* xo = smem_blk + kBlkRows;
-    VT* tab = reinterpret_cast<VT*>(smem_blk + 2 * kBlkRows);
+    VT* tab = reinterpret_cast<VT*>(smem_blk + kBlkRows);
     uint16_t* rlo = reinterpret_cast<uint16_t*>(tab + L * L);
     const int tid = threadIdx.x;
     const DociBlock blk = a.blocks[blockIdx.x];
@@ -316,10 +316,11 @@ __global__ void __launch_bounds__(T) k_spmv_doci_blk(DociBlockArgs a, const VT* 
             }
         }
     }
-    // staged hops across the split, double-buffered: stage s+1 loads into
-    // registers while stage s is read from LDS; one barrier per stage
+    // staged hops across the split: stage s+1 loads into registers while
+    // stage s is read from LDS; two barriers per stage
     for (int sidx = 0; sidx < nst; ++sidx) {
-        double* buf = sbuf + (sidx & 1) * kBlkRows;
+        double* buf = sbuf;
+        __syncthreads();
 #pragma unroll
         for (int i = 0; i < ROWS; ++i) buf[tid + i * T] = pre[i];
         __syncthreads();
@@ -370,7 +371,7 @@ static void launch_doci_blk(const DociBlockArgs& a, const VT* vk, const double* 
     if (!a.nblocks) return;
     CF_REQUIRE(a.w >= 1 && a.w <= kDociMaxW && a.L - a.w >= 1 && a.L - a.w <= 20,
                "block-staged DOCI SpMV: 1 <= w <= 12 low orbitals and 1..20 high orbitals");
-    const size_t shm = 2 * kBlkRows * sizeof(double) + (size_t)a.L * a.L * sizeof(VT) +
+    const size_t shm = kBlkRows * sizeof(double) + (size_t)a.L * a.L * sizeof(VT) +
                        ((size_t)1 << a.w) * sizeof(uint16_t);
     // rows of the largest block over the workgroup (T * ROWS >= C(w, w/2)); the
     // neighbour-block bases are computed one per thread (T >= ch (nh - ch))
