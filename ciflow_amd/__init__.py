@@ -174,6 +174,7 @@ def lib():
         "cf_crs_set_spmv_shape": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "cf_crs_autotune": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(TuneEntry), C.c_int, C.POINTER(C.c_int)]),
         "cf_rotate_integrals": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp, dp]),
+        "cf_rotate_integrals_noalloc": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]),
         "cf_ham_rotate": (C.c_int, [vp, dp, C.POINTER(vp)]),
         "cf_ham_get": (C.c_int, [vp, dp, dp]),
         "cf_expm_skew": (C.c_int, [C.c_int, dp, dp]),

@@ -28,6 +28,7 @@
 
 namespace cf {
 void gen_integrals(int L, uint64_t seed, double* h, double* V);
+void noalloc_setup(int device);  // noalloc.hip
 }
 
 using namespace cf;
@@ -149,6 +150,11 @@ struct cf_ctx {
         if (tls_alloc_stream == stream) tls_alloc_stream = nullptr;
     }
 };
+
+namespace cf {
+hipStream_t ctx_stream(const cf_ctx* c) { return c->stream; }
+int ctx_device(const cf_ctx* c) { return c->device; }
+}  // namespace cf
 
 namespace {
 // Records HIP events around one kernel launch on the context stream.
@@ -978,6 +984,7 @@ int cf_ctx_create(int device, cf_ctx** out) {
         for (int m = 0; m < kBinomCols; ++m) hb[k * kBinomCols + m] = binom_host(k, m);
     c->binom.alloc(hb.size());
     copy_sync(c->binom.p, hb.data(), hb.size() * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream);
+    noalloc_setup(device);
     *out = c.release();
     API_END
 }

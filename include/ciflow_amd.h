@@ -86,6 +86,16 @@ int cf_gen_integrals(int L, uint64_t seed, double* h, double* V);
 /* Batched, host arrays: h[batch][L*L], V[batch][L^4], U[batch][L*L]. */
 int cf_rotate_integrals(cf_ctx* ctx, int L, int batch, const double* h, const double* V, const double* U,
                         double* h_out, double* V_out);
+/* No reference counterpart: cf_rotate_integrals for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Device arrays in and out (same layouts),
+ * asynchronous on the context stream; the quarter transforms' work array is a
+ * 16 MiB per-GPU device array built into the library, so batch*L^4 must not
+ * exceed 2,097,152 (CF_ERR_ARG otherwise) and nothing is allocated.  Calls
+ * that overlap on one GPU (other contexts or threads) share that array and
+ * race: serialise them. */
+int cf_rotate_integrals_noalloc(cf_ctx* ctx, int L, int batch, const double* dh, const double* dV,
+                                const double* dU, double* dh_out, double* dV_out);
 /* Rotated copy of a Hamiltonian kept on the GPU (Econst unchanged). */
 int cf_ham_rotate(const cf_ham* src, const double* U, cf_ham** out);
 /* Read a Hamiltonian's integrals back (h: L*L, V: L^4; either may be NULL). */
