@@ -5,13 +5,18 @@ ground-state solves -- 70% DOCI with L uniform in [12, 20] and 30% FCI with L
 uniform in [8, 11], half filling, integrals from seed 1000+k rotated by
 U = exp(K), K antisymmetric with N(0, 0.05^2) entries.  Each request is
 served with FP32 matrix values (FP32 sigma for FCI) and an FP64 polish, and
-stops at a Kato-Temple energy bound of 1e-9 Eh.  One step = one request.
-  * value: solves/s of K requests submitted at once (closed batch) to the
-    continuously batched server (--slots worker streams),
+stops at a Kato-Temple energy bound of 1e-9 Eh.  The timed stream holds
+--requests (4,096) requests; one step = one slice of requests/steps
+consecutive requests, and the K slices are served back to back.
+  * value: solves/s of the whole stream submitted at once (closed batch) to
+    the continuously batched server (--slots worker streams), first
+    submission to last completion,
   * p50/p99: latency (submit -> result) of Poisson arrivals at 70% of that
-    rate, over the first min(K, 1024) requests,
+    rate, over the first 1,024 requests,
   * one_at_a_time_fp64: the plain path (one request at a time, FP64, ARPACK
-    tolerance 1e-12) on every 4th request, and max |dE| against it,
+    tolerance 1e-12) on every 16th request of each (kind, L) class (256+
+    requests), its class-weighted rate over the whole stream, and max |dE|
+    against the served energies,
   * roofline: the stored-CRS SpMV of FCI L=12 6a6b (1,553,018,544 nonzeros),
     timed with HIP events on the library stream during one FP64 solve.
 --workload fci12: FCI L=12 6a6b seed 1 build + solve per step.
@@ -41,12 +46,16 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "spmv_doci", "rotate"],
                    default="stream")
-    p.add_argument("--steps", type=int, default=None, help="stream: requests (4096); fci12: solves (4)")
-    p.add_argument("--warmup", type=int, default=None, help="stream: warm-up requests (64); fci12: solves (1)")
+    p.add_argument("--steps", type=int, default=None,
+                   help="stream: slices of the --requests stream (16); fci12: solves (4)")
+    p.add_argument("--warmup", type=int, default=None, help="stream: warm-up slices (2); fci12: solves (1)")
+    p.add_argument("--requests", type=int, default=4096, help="stream: requests in the timed stream")
+    p.add_argument("--latency-requests", type=int, default=1024, help="stream: requests in the Poisson run")
     p.add_argument("--slots", type=int, default=8)
     p.add_argument("--load", type=float, default=0.7, help="Poisson rate as a fraction of the measured rate")
     p.add_argument("--rate", type=float, default=0.0, help="Poisson rate (solves/s); overrides --load")
-    p.add_argument("--baseline-every", type=int, default=4, help="0: skip the one-at-a-time FP64 baseline")
+    p.add_argument("--baseline-every", type=int, default=16,
+                   help="one-at-a-time FP64 baseline on every n-th request of each class (0: skip)")
     p.add_argument("--no-latency", action="store_true", help="skip the Poisson latency run")
     p.add_argument("--L", type=int, default=12)
     p.add_argument("--nel", type=int, default=6)
@@ -122,47 +131,106 @@ def cpu_baseline_fci(h, V, L, nel, nnz_upper, n, n_matvec, lines):
     }
 
 
-def cpu_baseline_stream(reqs, nmv_by_class, lines=96):
-    """Per (kind, L) class of the stream: the oracle's rotation of the
-    integrals (OrbitalTransform::fillHamCI restated), the CRS build of
-    `lines` evenly spaced alpha lines (DOCI: rows) with PushToRow on all
-    host threads, and the serial upper-triangle mvprod on them; extrapolated
-    by rows to the class's full matrix and to its mean FP64 matvec count on
-    the GPU (the reference's one-at-a-time FP64 path).  Stream CPU time =
-    sum over the requests of their class time."""
+def _median3(f):
+    """Median wall time of three calls of f (the timing rule of the CPU baseline)."""
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        f()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline_stream(reqs, nmv_by_class, lines=48):
+    """Per (kind, L) class of the stream, the reference's one-at-a-time CPU path
+    on a bounded sample: the oracle's rotation of the integrals
+    (OrbitalTransform::fillHamCI restated), the CRS build of `lines` evenly
+    spaced alpha lines (DOCI: 16 blocks of <= 4096 rows) with PushToRow and
+    build_parallel on all host threads, and two products over those rows:
+    the serial upper-triangle SparseMatrix_CRS_Sym::mvprod (the reference's
+    g++ path, SparseMatrix_CRS.cpp:712-720) and an OpenMP full-row gather (the
+    open stand-in for the Intel build's threaded mkl_cspblas_dcsrsymv, :705-707;
+    timed on the sampled upper rows and scaled per nonzero to the full matrix).
+    Every stage is the median of three runs.  Extrapolated by rows (gather: by
+    nonzeros) to the class's full matrix and its mean FP64 matvec count on the
+    GPU (the one-at-a-time FP64 path).  Stream CPU time = sum over the requests
+    of their class time."""
     from math import comb
 
     from ciflow_amd import stream as S
     from oracle import oracle as O
     threads = _oracle_threads()
-    per = {}
-    t_sample = 0.0
+    lib = O.lib()
+    per, per_omp = {}, {}
+    t0_all = time.perf_counter()
     for key in sorted({(r.kind, r.L) for r in reqs}):
         r = next(q for q in reqs if (q.kind, q.L) == key)
         kind = "DOCI" if r.kind == 1 else "FCI"
         h, V, U = S.request_arrays(r)
-        t0 = time.perf_counter()
-        hr, Vr = O.rotate_integrals(U, h, V)
-        t_rot = time.perf_counter() - t0
+        box = {}
+        t_rot = _median3(lambda: box.__setitem__("hv", O.rotate_integrals(U, h, V)))
+        hr, Vr = box["hv"]
         d1 = comb(r.L, r.n)
         n = d1 if kind == "DOCI" else d1 * d1
-        # FCI: `lines` alpha strings (d_beta rows each); DOCI: 32 blocks of up to 4096 rows
-        ranges = _line_ranges(d1, lines) if kind == "FCI" else _line_ranges(d1, 32, 4096)
-        tb, tm, rows, _ = _sample_build_mv(kind, hr, Vr, r.n, ranges, n, threads)
+        ranges = _line_ranges(d1, lines) if kind == "FCI" else _line_ranges(d1, 16, 4096)
+        mats = []
+
+        def build():
+            mats.clear()
+            for s0, s1 in ranges:
+                mats.append(O.build_crs(kind, hr, Vr, r.n, r.n, start=int(s0), end=int(s1), threads=threads))
+        tb = _median3(build)
+        rows = sum(m.n for m in mats)
+        nnz_s = sum(m.nnz for m in mats)
+        x = np.random.default_rng(0).standard_normal(n)
+        y = np.zeros(n)
+
+        def mv_sym():
+            for m in mats:
+                lib.or_mvprod_sym(m.n, m.row.ctypes.data_as(O.u64p), m.col.ctypes.data_as(O.u32p),
+                                  m.data.ctypes.data_as(O.dp), x.ctypes.data_as(O.dp), y.ctypes.data_as(O.dp))
+
+        def mv_omp():
+            for m in mats:
+                lib.or_mvprod_full(m.n, m.row.ctypes.data_as(O.u64p), m.col.ctypes.data_as(O.u32p),
+                                   m.data.ctypes.data_as(O.dp), x.ctypes.data_as(O.dp), y.ctypes.data_as(O.dp),
+                                   threads)
+        tm = _median3(mv_sym)
+        to = _median3(mv_omp)
+        # full nonzeros of the class: 2 * upper - n (diagonal stored once)
+        if kind == "FCI":
+            da = d1
+            nnz_full = n * (1 + 2 * r.n * (r.L - r.n) + 2 * comb(r.n, 2) * comb(r.L - r.n, 2)
+                            + (r.n * (r.L - r.n)) ** 2)
+        else:
+            nnz_full = n * (1 + r.n * (r.L - r.n))
         scale = n / rows
         nmv = nmv_by_class.get(key, 0.0)
         per[key] = t_rot + tb * scale + nmv * tm * scale
-        t_sample += t_rot + tb + tm
+        per_omp[key] = t_rot + tb * scale + nmv * to * nnz_full / nnz_s
     total = sum(per[(r.kind, r.L)] for r in reqs)
+    total_omp = sum(per_omp[(r.kind, r.L)] for r in reqs)
     return {
         "value": len(reqs) / total, "unit": "solves/s", "cores": threads, "kind": "port",
-        "sample": (f"per (kind, L) class: oracle integral rotation + CRS build (PushToRow, build_parallel on "
-                   f"{threads} threads) of {lines} alpha lines (DOCI: 32 blocks of <= 4096 rows) + serial "
-                   f"upper-triangle mvprod on "
-                   f"them, extrapolated by rows to the full matrix and to the class's mean FP64 matvec count "
-                   f"({len(per)} classes, {len(reqs)} requests)"),
-        "sample_s": t_sample,
+        "sample": (f"per (kind, L) class ({len(per)} classes, {len(reqs)} requests): oracle integral rotation + "
+                   f"CRS build (PushToRow, build_parallel on {threads} threads) of {lines} alpha lines (DOCI: 16 "
+                   f"blocks of <= 4096 rows) + the serial upper-triangle mvprod (reference g++ path) on them, "
+                   f"median of 3 runs each, extrapolated by rows to the full matrix and to the class's mean FP64 "
+                   f"matvec count"),
+        "variants": {"omp_full_row_gather_solves_per_s": len(reqs) / total_omp,
+                     "note": ("same build; products as an OpenMP full-row gather on "
+                              f"{threads} threads (stand-in for threaded MKL dcsrsymv), scaled per nonzero")},
+        "sample_s": time.perf_counter() - t0_all,
     }
+
+
+def stratified_sample(reqs, every):
+    """Every `every`-th request of each (kind, L) class in stream order (each
+    class contributes ceil(count / every) requests)."""
+    by = {}
+    for i, r in enumerate(reqs):
+        by.setdefault((r.kind, r.L), []).append(i)
+    return sorted(i for idx in by.values() for i in idx[::every])
 
 
 # ---------------------------------------------------------------------------
@@ -228,51 +296,83 @@ def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
 # ---------------------------------------------------------------------------
 # workloads
 # ---------------------------------------------------------------------------
+def _slices(total, k):
+    """Split range(total) into k contiguous slices whose sizes differ by at most one."""
+    edges = np.linspace(0, total, k + 1).round().astype(int)
+    return [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:])]
+
+
 def bench_stream(a, CA):
+    """The served stream at its stated size: --requests (4,096) requests, one
+    step = one slice of requests/steps of them.  The K slices are submitted
+    back to back to one continuously batched server (a closed batch of the whole
+    stream) and the timed region runs from the first submission to the last
+    completion; the W warm-up steps are slices of the same size from a
+    warm-up stream of the same mix (other seeds), served before the timed run."""
     from ciflow_amd import stream as S
-    K = a.steps if a.steps is not None else 4096
-    W = a.warmup if a.warmup is not None else 64
+    K = a.steps if a.steps is not None else 16
+    W = a.warmup if a.warmup is not None else 2
+    NREQ = a.requests
+    if K < 1 or K > NREQ:
+        raise SystemExit(f"--steps must be in [1, {NREQ}] (one step = a slice of the {NREQ}-request stream)")
+    slices = _slices(NREQ, K)
+    slice_sz = NREQ / K
+    nwarm = int(round(W * slice_sz))
     method = {"auto": CA.CF_SOLVER_AUTO, "lanczos": CA.CF_LANCZOS, "davidson": CA.CF_DAVIDSON}[a.solver]
     served = dict(precision=CA.CF_F32, check_every=2, tol_energy=1e-9, direct_fci=True, method=method)
-    reqs = S.make_stream(K, rate=0.0)
-    warm = S.make_stream(W, rate=0.0, first_seed=500000, mix_seed=99) if W > 0 else []
+    reqs = S.make_stream(NREQ, rate=0.0)
+    warm = S.make_stream(nwarm, rate=0.0, first_seed=500000, mix_seed=99) if nwarm > 0 else []
     t0 = time.perf_counter()
     arrs = S.prepare(reqs)
     warr = S.prepare(warm) if warm else []
-    log(f"prepared {K}+{W} requests in {time.perf_counter() - t0:.1f} s")
+    log(f"prepared {NREQ}+{nwarm} requests in {time.perf_counter() - t0:.1f} s")
     srv = CA.SolveServer(slots=a.slots)
     try:
         if warm:
-            S.run_stream(warm, warr, server=srv, timeout=600, **served)
-        log("timed closed batch ...")
-        out = S.run_stream(reqs, arrs, server=srv, timeout=1200, **served)
+            S.run_stream(warm, warr, server=srv, timeout=900, **served)
+            log(f"warm-up: {W} steps ({nwarm} requests)")
+        log(f"timed: {K} steps of {slices[0][1] - slices[0][0]}-{max(b - a_ for a_, b in slices)} requests "
+            f"({NREQ} requests) ...")
+        out = S.run_stream(reqs, arrs, server=srv, timeout=1500, **served)
         rate = out["solves_per_s"]
         log(f"closed batch: {rate:.2f} solves/s ({out['wall_s']:.2f} s)")
-        nlat = min(K, 1024)
+        res = out["results"]
+        # the slice boundaries in completion time: when the last request of slice j finished
+        t_first = min(d["t_submit"] for d in res)
+        step_end = [max(res[i]["t_end"] for i in range(s0, s1)) - t_first for s0, s1 in slices]
+        nlat = min(NREQ, a.latency_requests)
         lam = a.rate if a.rate > 0 else a.load * rate
         lat = {"p50_ms": None, "p99_ms": None, "solves_per_s": None, "mean_ms": None}
         if not a.no_latency:
             lreqs = S.with_arrivals(reqs[:nlat], lam)
-            lat = S.run_stream(lreqs, arrs[:nlat], server=srv, timeout=1200, **served)
+            log(f"Poisson arrivals at {lam:.1f}/s over {nlat} requests ...")
+            lat = S.run_stream(lreqs, arrs[:nlat], server=srv, timeout=1500, **served)
             log(f"Poisson {lam:.1f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms")
     finally:
         srv.close()
-    # the plain path: one request at a time, FP64 values, tolerance 1e-12 (ARPACK semantics)
+    # the plain path: one request at a time, FP64 values, tolerance 1e-12 (ARPACK semantics),
+    # on a class-stratified sample of the same requests
     base = None
     dE = np.zeros(1)
     nmv_by_class = {}
-    sub = list(range(0, K, a.baseline_every)) if a.baseline_every > 0 else []
+    sub = stratified_sample(reqs, a.baseline_every) if a.baseline_every > 0 else []
     if sub:
         breqs = [reqs[i] for i in sub]
-        base = S.run_stream(breqs, [arrs[i] for i in sub], slots=1, precision=CA.CF_F64, timeout=1200)
+        log(f"one-at-a-time FP64 over {len(sub)} requests ...")
+        base = S.run_stream(breqs, [arrs[i] for i in sub], slots=1, precision=CA.CF_F64, timeout=1500)
         log(f"one-at-a-time FP64: {base['solves_per_s']:.2f} solves/s over {len(sub)} requests")
-        Eserved = np.array([out["results"][i]["energy"] for i in sub])
+        Eserved = np.array([res[i]["energy"] for i in sub])
         Ebase = np.array([d["energy"] for d in base["results"]])
         dE = np.abs(Eserved - Ebase)
         for r, d in zip(breqs, base["results"]):
             nmv_by_class.setdefault((r.kind, r.L), []).append(d["n_matvec"])
         nmv_by_class = {k: float(np.mean(v)) for k, v in nmv_by_class.items()}
-    res = out["results"]
+        # the one-at-a-time rate of the whole stream: class mix weighted by each class's mean time
+        t_cls = {}
+        for r, d in zip(breqs, base["results"]):
+            t_cls.setdefault((r.kind, r.L), []).append(d["t_end"] - d["t_start"])
+        t_stream = sum(float(np.mean(t_cls[(r.kind, r.L)])) for r in reqs)
+        base_stream_rate = NREQ / t_stream
     roof = None
     if not a.no_roofline:
         log("SpMV roofline (FCI L=12) ...")
@@ -302,17 +402,22 @@ def bench_stream(a, CA):
                                 "K~N(0,0.05^2), FP32 values + FP64 polish, Kato-Temple energy stop 1e-9 Eh"),
                    "solver": {"auto": "Davidson for FCI and DOCI >= 2,000 determinants, Lanczos for smaller DOCI",
                               "lanczos": "thick-restart Lanczos", "davidson": "Davidson"}[a.solver],
-                   "requests": K, "doci": int((kinds == 1).sum()), "fci": int((kinds == 0).sum()),
+                   "requests": NREQ, "step": f"one slice of {NREQ}/{K} = {slice_sz:.1f} consecutive requests",
+                   "warmup_requests": nwarm,
+                   "doci": int((kinds == 1).sum()), "fci": int((kinds == 0).sum()),
                    "slots": a.slots, "closed_batch": True,
                    "inputs": "h, V, U copied from host memory at submission (1.3 MB per L=20 request)"},
+        "step_completion_s": step_end,
         "p50_ms": lat["p50_ms"],
         "p99_ms": lat["p99_ms"],
         "latency_run": {"arrivals": "Poisson", "rate_per_s": lam, "requests": nlat,
                         "achieved_solves_per_s": lat["solves_per_s"], "mean_ms": lat["mean_ms"]},
         "closed_batch_latency_ms": {"p50": out["p50_ms"], "p99": out["p99_ms"]},
         "one_at_a_time_fp64": ({"solves_per_s": base["solves_per_s"], "requests": len(sub),
-                                "sample": f"every {a.baseline_every}th request", "tol": 1e-12} if base else None),
-        "speedup_vs_one_at_a_time_fp64": rate / base["solves_per_s"] if base else None,
+                                "stream_weighted_solves_per_s": base_stream_rate,
+                                "sample": f"every {a.baseline_every}th request of each (kind, L) class",
+                                "tol": 1e-12} if base else None),
+        "speedup_vs_one_at_a_time_fp64": rate / base_stream_rate if base else None,
         "max_abs_dE_vs_fp64": float(dE.max()) if base else None,
         "mean_matvec": {"f32": float(np.mean([d["n_matvec_f32"] for d in res])),
                         "f64": float(np.mean([d["n_matvec"] for d in res]))},
