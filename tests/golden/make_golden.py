@@ -12,6 +12,11 @@ Contents:
   config2     (--slow, ~10 min on 8 cores) FCI L=12 6a6b seed 1, 853,776
               determinants: the oracle's full build_parallel + Lanczos energy.
   config3     (--slow) DOCI L=24 12 pairs seed 2, 2,704,156 determinants.
+  stream      (--stream, ~5 min on 8 cores) the first two requests of every
+              (kind, L) class of the served 4,096-request stream (DOCI L =
+              12..20, FCI L = 8..11): the oracle's FP64 rotation of the
+              request's integrals, full build_parallel build and Lanczos
+              (tol 1e-12) energy.
 
 Usage: python tests/golden/make_golden.py [--slow]
 (without --slow the slow entries already in golden.json are kept)
@@ -53,8 +58,8 @@ def crs_digest(m):
     }
 
 
-def ground_state(kind: int, L: int, nel: int, seed: int):
-    h, V = O.gen_integrals(L, seed)
+def ground_state(kind: int, L: int, nel: int, seed: int, hv=None):
+    h, V = O.gen_integrals(L, seed) if hv is None else hv
     lib = O.lib()
     lib.or_ground_state.argtypes = [C.c_int] * 4 + [O.dp, O.dp, C.c_double, C.c_int, C.c_int, C.c_double, O.dp,
                                                     C.POINTER(C.c_int), O.u64p, O.dp]
@@ -63,6 +68,28 @@ def ground_state(kind: int, L: int, nel: int, seed: int):
                              O.num_threads(), 42, 1e-12, C.byref(E), C.byref(nmv), C.byref(nnz), None)
     assert st == 0
     return {"energy": E.value, "n_matvec": nmv.value, "nnz_upper": int(nnz.value)}
+
+
+def stream_golden(per_class: int = 2):
+    """Oracle FP64 energies of the first `per_class` requests of each class of
+    the served stream (the requests' integrals come from the library's host
+    generators, which are bit-identical to the oracle's)."""
+    from ciflow_amd import stream as S
+    reqs = S.make_stream(4096)
+    picked = {}
+    for r in reqs:
+        lst = picked.setdefault((r.kind, r.L), [])
+        if len(lst) < per_class:
+            lst.append(r)
+    out = []
+    for key in sorted(picked):
+        for r in picked[key]:
+            h, V, U = S.request_arrays(r)
+            hr, Vr = O.rotate_integrals(U, h, V)
+            g = ground_state(r.kind, r.L, r.n, r.seed, hv=(hr, Vr))
+            print(f"stream request {r.k}: kind {r.kind} L {r.L} E {g['energy']:.12f}", flush=True)
+            out.append({"k": r.k, "kind": r.kind, "L": r.L, "n": r.n, "seed": r.seed, **g})
+    return out
 
 
 def main():
@@ -96,6 +123,10 @@ def main():
         for k in ("config2", "config3"):
             if k in old:
                 out[k] = old[k]
+    if "--stream" in sys.argv:
+        out["stream"] = stream_golden()
+    elif "stream" in old:
+        out["stream"] = old["stream"]
     with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(json.dumps(out, indent=1))

@@ -300,3 +300,39 @@ def test_det_file_full_beh2_list_is_the_permuted_fci_matrix():
     assert np.abs(Hd - Hf).max() <= 1e-12 * np.abs(Hf).max()
     E, _, _ = O.lanczos(O.full_from_upper(m))
     assert abs((E + ec) - golden()["beh2"]["fci_energy"]) < 1e-10
+
+
+@pytest.mark.parametrize("L,npair,g", [(4, 2, -0.5), (6, 3, -0.5), (8, 4, -0.5), (10, 5, -0.5), (12, 6, -0.5),
+                                       (12, 6, 0.3), (10, 3, -1.0), (12, 4, -0.2), (9, 4, -0.5), (11, 2, 0.7)])
+def test_richardson_gaudin_matches_doci_eigh(L, npair, g):
+    """The eigenvalue-based Richardson-Gaudin solver against a dense eigh of the
+    oracle's DOCI matrix of the RedBCS model (eps_i = i): the check that pins it
+    before it serves as the exact energy of the L = 24 instance."""
+    from oracle.richardson import redbcs_integrals, richardson_gaudin_energy
+    eps = np.arange(L, dtype=float)
+    E, _ = richardson_gaudin_energy(eps, npair, g)
+    h, V = redbcs_integrals(eps, g)
+    assert np.array_equal((h, V)[1], redbcs_ham(L, eps, g)[1])
+    w = np.linalg.eigvalsh(O.dense_from_upper(O.build_crs("DOCI", h, V, npair, npair)))
+    assert abs(E - w[0]) <= 1e-12 * max(1.0, abs(w[0]))
+
+
+def test_richardson_gaudin_closed_forms():
+    """One pair: E is the lowest root of 1 + g sum_i 1/(2 eps_i - E) = 0
+    (Richardson's equation for N = 1); g -> 0: E -> sum of the N lowest 2 eps_i
+    plus g N to first order; every level filled: E = 2 sum eps + g N (only the
+    diagonal b_i^+ b_i of the pair hopping acts)."""
+    from scipy.optimize import brentq
+
+    from oracle.richardson import richardson_gaudin_energy
+    eps = np.array([0.0, 0.7, 1.9, 2.4, 4.0])
+    g = -0.35
+    E, _ = richardson_gaudin_energy(eps, 1, g)
+    f = lambda x: 1.0 + g * np.sum(1.0 / (2 * eps - x))
+    E1 = brentq(f, -50.0, 2 * eps[0] - 1e-9)
+    assert abs(E - E1) <= 1e-12
+    gsmall = 1e-7
+    E, _ = richardson_gaudin_energy(eps, 2, gsmall, steps=10)
+    assert abs(E - (2 * (eps[0] + eps[1]) + gsmall * 2)) <= 1e-12
+    E, _ = richardson_gaudin_energy(eps, 5, g, steps=50)
+    assert abs(E - (2 * eps.sum() + g * 5)) <= 1e-11
