@@ -44,7 +44,8 @@ METRIC = "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of pe
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "spmv_doci", "rotate"],
+    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "spmv_doci", "rotate",
+                                          "sigma"],
                    default="stream")
     p.add_argument("--steps", type=int, default=None,
                    help="stream: slices of the --requests stream (16); fci12: solves (4)")
@@ -58,6 +59,7 @@ def parse():
                    help="one-at-a-time FP64 baseline on every n-th request of each class (0: skip)")
     p.add_argument("--no-latency", action="store_true", help="skip the Poisson latency run")
     p.add_argument("--L", type=int, default=12)
+    p.add_argument("--sigma-L", default="8,9,10,11,12,16", help="sigma: orbital counts (FCI, half filling)")
     p.add_argument("--nel", type=int, default=6)
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--precision", choices=["f64", "f32"], default="f64")
@@ -66,7 +68,8 @@ def parse():
     p.add_argument("--no-fp64", action="store_true", help="direct16: skip the FP64 solve")
     p.add_argument("--cpu-lines", type=int, default=6, help="alpha lines in the FCI CPU-baseline sample")
     p.add_argument("--solver", choices=["auto", "lanczos", "davidson"], default="auto",
-                   help="stream: served solver (auto: Davidson for FCI and DOCI >= 2,000 determinants)")
+                   help="stream: served solver (auto: DOCI up to 13,312 determinants in batched one-launch "
+                        "solves, Davidson for FCI and larger DOCI)")
     return p.parse_args()
 
 
@@ -296,6 +299,15 @@ def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
 # ---------------------------------------------------------------------------
 # workloads
 # ---------------------------------------------------------------------------
+def class_service_ms(reqs, res):
+    """Mean service time (start -> end, ms) and count per (kind, L) class; a
+    batched request's service time is its batch's."""
+    by = {}
+    for r, d in zip(reqs, res):
+        by.setdefault(f"{'DOCI' if r.kind == 1 else 'FCI'}{r.L}", []).append((d["t_end"] - d["t_start"]) * 1e3)
+    return {k: {"n": len(v), "mean_ms": float(np.mean(v))} for k, v in sorted(by.items())}
+
+
 def _slices(total, k):
     """Split range(total) into k contiguous slices whose sizes differ by at most one."""
     edges = np.linspace(0, total, k + 1).round().astype(int)
@@ -400,7 +412,8 @@ def bench_stream(a, CA):
         "config": {"workload": ("served stream of rotated-integral ground-state solves: 70% DOCI L~U[12,20], "
                                 "30% FCI L~U[8,11] (on-the-fly sigma), half filling, seeds 1000+k, U=exp(K) "
                                 "K~N(0,0.05^2), FP32 values + FP64 polish, Kato-Temple energy stop 1e-9 Eh"),
-                   "solver": {"auto": "Davidson for FCI and DOCI >= 2,000 determinants, Lanczos for smaller DOCI",
+                   "solver": {"auto": ("DOCI <= 13,312 determinants: batched FP64 Lanczos, one workgroup per "
+                                       "request (cf_solve_doci_batch); FCI and larger DOCI: Davidson"),
                               "lanczos": "thick-restart Lanczos", "davidson": "Davidson"}[a.solver],
                    "requests": NREQ, "step": f"one slice of {NREQ}/{K} = {slice_sz:.1f} consecutive requests",
                    "warmup_requests": nwarm,
@@ -421,6 +434,7 @@ def bench_stream(a, CA):
         "max_abs_dE_vs_fp64": float(dE.max()) if base else None,
         "mean_matvec": {"f32": float(np.mean([d["n_matvec_f32"] for d in res])),
                         "f64": float(np.mean([d["n_matvec"] for d in res]))},
+        "class_service_ms": class_service_ms(reqs, res),
         "roofline": roof,
         "cpu_baseline": cpu,
     }
@@ -635,6 +649,43 @@ def bench_spmv(a, CA, doci=False):
     return {"metric": METRIC, "workload": wl, "n": n, "nnz": nnz, **out}
 
 
+def bench_sigma(a, CA):
+    """On-the-fly FCI sigma alone (CI_Big::mvprod's role): for each L in
+    --sigma-L, FCI floor(L/2) a + b electrons with seeded integrals (seed 3),
+    --steps applications (after one warm-up) per value precision and alpha-beta
+    kernel (2: kernel 2; 3: packed-offset kernel 3); sigma time from HIP events
+    on the library stream (the whole product: same-spin parts and alpha-beta
+    kernel)."""
+    from math import comb
+    ctx = CA.Context(0)
+    reps = a.steps if a.steps is not None else 3
+    out = []
+    for L in [int(v) for v in a.sigma_L.split(",")]:
+        nel = L // 2
+        h, V = CA.gen_integrals(L, 3)
+        d = CA.FCI_Direct(CA.Hamiltonian(h, V, nel, nel, 0.0, ctx), value_prec=CA.CF_F32)
+        x = np.random.default_rng(0).standard_normal(d.get_dim())
+        sa = nel * (L - nel) + 1
+        terms = d.get_dim() * sa * nel * (L - nel + 1)
+        for kern in (2, 3):
+            d.set_kernel(kern)
+            for prec, fam in ((CA.CF_F32, "sigma_f32"), (CA.CF_F64, "sigma_f64")):
+                d.mvprod(x, precision=prec)
+                ctx.enable_timing(True)
+                ctx.reset_timing()
+                for _ in range(reps):
+                    d.mvprod(x, precision=prec)
+                nl, ms = ctx.kernel_time(fam)
+                ctx.enable_timing(False)
+                row = {"L": L, "n": d.get_dim(), "kernel": kern, "prec": fam[-3:], "ms": ms / nl,
+                       "alpha_beta_terms": terms, "terms_per_s": terms / (ms / nl * 1e-3)}
+                log(f"sigma L={L} kernel {kern} {fam[-3:]}: {ms / nl:.3f} ms")
+                out.append(row)
+        del d
+    return {"metric": METRIC, "workload": "on-the-fly FCI sigma, seeded integrals (seed 3), half filling",
+            "sigma": out}
+
+
 def bench_rotate(a, CA):
     """Batched FP64-MFMA integral rotation (OrbitalTransform::fillHamCI): 64
     Hamiltonians at L=20 per call, 8 L^5 + 4 L^3 flop each; kernel time from HIP
@@ -670,7 +721,7 @@ def main():
 
     run = {"stream": bench_stream, "fci12": bench_fci12, "doci24": bench_fci12, "direct16": bench_direct16,
            "spmv": bench_spmv, "spmv_doci": lambda a, CA: bench_spmv(a, CA, doci=True),
-           "rotate": bench_rotate}[a.workload]
+           "rotate": bench_rotate, "sigma": bench_sigma}[a.workload]
     out = run(a, CA)
     print(json.dumps(out))
 

@@ -157,6 +157,7 @@ def lib():
         "cf_crs_info": (C.c_int, [vp, u64p, u64p, u64p]),
         "cf_crs_count_upper": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p]),
         "cf_crs_export_upper": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p, u32p, dp]),
+        "cf_crs_export_upper_u32": (C.c_int, [vp, C.c_uint64, C.c_uint64, u32p, u32p, dp]),
         "cf_crs_count_full": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p]),
         "cf_crs_export_full": (C.c_int, [vp, C.c_uint64, C.c_uint64, u64p, u32p, dp]),
         "cf_mvprod": (C.c_int, [vp, dp, dp]),
@@ -196,6 +197,7 @@ def lib():
         "cf_server_stats": (C.c_int, [vp, u64p, u64p, u64p, u64p]),
         "cf_server_destroy": (C.c_int, [vp]),
         "cf_request_footprint": (C.c_int, [C.POINTER(Request), u64p]),
+        "cf_solve_doci_batch": (C.c_int, [vp, C.c_int, C.POINTER(Request), C.POINTER(Result)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -416,6 +418,18 @@ class SparseMatrix_CRS_Sym:
 
     def export_upper(self, r0: int = 0, r1: int | None = None):
         return self._export(True, r0, self.getn() if r1 is None else r1)
+
+    def export_upper_u32(self, r0: int = 0, r1: int | None = None):
+        """export_upper with the reference's 32-bit row pointer (SparseMatrix_CRS.h:111-112)."""
+        r1 = self.getn() if r1 is None else r1
+        cnt = C.c_uint64()
+        _check(lib().cf_crs_count_upper(self.ptr, r0, r1, C.byref(cnt)))
+        row = np.zeros(r1 - r0 + 1, dtype=np.uint32)
+        col = np.zeros(max(cnt.value, 1), dtype=np.uint32)
+        val = np.zeros(max(cnt.value, 1))
+        _check(lib().cf_crs_export_upper_u32(self.ptr, r0, r1, row.ctypes.data_as(u32p), col.ctypes.data_as(u32p),
+                                             _d(val)))
+        return row, col[:cnt.value], val[:cnt.value]
 
     def export_full(self, r0: int = 0, r1: int | None = None):
         return self._export(False, r0, self.getn() if r1 is None else r1)
@@ -825,6 +839,25 @@ class SolveServer(_Handle):
         v = [C.c_uint64() for _ in range(4)]
         _check(lib().cf_server_stats(self.ptr, *[C.byref(x) for x in v]))
         return dict(zip(["submitted", "finished", "hbm_in_use", "hbm_budget"], [x.value for x in v]))
+
+
+def solve_doci_batch(requests, ctx: "Context | None" = None) -> list[dict]:
+    """cf_solve_doci_batch: many small DOCI solves in one launch.  requests:
+    dicts with h, V, npair and optionally U, econst, tol, tol_energy,
+    check_every, tag; returns one result dict per request (SolveServer.poll's
+    fields)."""
+    ctx = ctx or default_context()
+    reqs = (Request * len(requests))()
+    keep = []
+    for i, q in enumerate(requests):
+        r, arrs = SolveServer._request(q.get("tag", i), CF_DOCI, q["h"], q["V"], q["npair"], q["npair"],
+                                       q.get("U"), q.get("econst", 0.0), CF_F64, 0, q.get("tol", 0.0),
+                                       q.get("check_every", 0), q.get("tol_energy", 0.0), False, CF_SOLVER_AUTO)
+        reqs[i] = r
+        keep.append(arrs)
+    res = (Result * len(requests))()
+    _check(lib().cf_solve_doci_batch(ctx.ptr, len(requests), reqs, res))
+    return [res[i].as_dict() for i in range(len(requests))]
 
 
 class RCILanczos:

@@ -16,9 +16,11 @@
 #include <vector>
 
 #include "kernels.h"
+#include "host.h"
 #include "lanczos.h"
 #include "rdm.h"
 #include "sigma.h"
+#include "batch.h"
 #include "spmv_doci.h"
 #include "spmv_ell.h"
 #include "spmv_fci.h"
@@ -34,17 +36,8 @@ void noalloc_setup(int device);  // noalloc.hip
 using namespace cf;
 
 namespace {
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
+int fail(int code, const std::string& msg) { return set_last_error(code, msg); }
 }  // namespace
-
-namespace cf {
-int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
-}  // namespace cf
 
 namespace {
 // Copies ordered on the handle's stream (which does not synchronise with the
@@ -85,6 +78,12 @@ struct cf_ctx {
     DBuf<uint64_t> binom;  // C(k, m): (kMaxL+1) x kBinomCols
     bool timing = false;
     std::map<std::string, TimerFamily> timers;
+    // strings and hop columns of DOCI spaces, per (L, pairs), for batched solves
+    struct DociTables {
+        DBuf<str_t> s;
+        DBuf<uint32_t> col;
+    };
+    std::map<std::pair<int, int>, DociTables> doci_tabs;
     std::vector<hipEvent_t> pool;
     DBuf<double> hx, hy;  // staging for host-pointer products
     double* pinned = nullptr;  // kPinned doubles of page-locked memory for small read-backs
@@ -137,12 +136,13 @@ struct cf_ctx {
                 (void)hipEventDestroy(pr.second);
             }
         for (auto e : pool) (void)hipEventDestroy(e);
-        if (blas_) (void)rocblas_destroy_handle(blas_);
         if (pinned) (void)hipHostFree(pinned);
+        if (blas_) (void)rocblas_destroy_handle(blas_);
         // buffers ordered on the stream go back to the pool before it is destroyed
         binom.release();
         hx.release();
         hy.release();
+        doci_tabs.clear();
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
@@ -789,90 +789,6 @@ struct DeviceBackend : KrylovBackend {
     }
 };
 
-struct HostBackend : KrylovBackend {
-    uint64_t n;
-    int M;
-    std::vector<double> Vb, w;
-    HostBackend(uint64_t n_, int ncv) : n(n_) {
-        M = (uint64_t)ncv > n ? (int)n : ncv;
-        Vb.assign((size_t)(M + 1) * n, 0.0);
-        w.assign(n, 0.0);
-    }
-    double* v(int i) { return Vb.data() + (size_t)i * n; }
-    double dot(const double* a, const double* b) {
-        double s = 0.0;
-        for (uint64_t q = 0; q < n; ++q) s += a[q] * b[q];
-        return s;
-    }
-    void start_random(uint64_t seed) override {
-        double* v0 = v(0);
-        for (uint64_t q = 0; q < n; ++q) {
-            uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (q + 1);
-            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-            z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-            z ^= z >> 31;
-            v0[q] = 1.0 + 0.5 * ((double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5);
-        }
-        normalize0();
-    }
-    void start_keep() override { normalize0(); }
-    void apply(int) override { throw Error(ERR_STATE, "host backend is driven by reverse communication"); }
-    void orth(int j, double* alpha, double* beta) override {
-        double a = 0.0;
-        std::vector<double> c(j + 1);
-        for (int pass = 0; pass < 2; ++pass) {
-            for (int i = 0; i <= j; ++i) c[i] = dot(v(i), w.data());
-            a += c[j];
-            for (int i = 0; i <= j; ++i) {
-                const double* vi = v(i);
-                for (uint64_t q = 0; q < n; ++q) w[q] -= c[i] * vi[q];
-            }
-        }
-        *alpha = a;
-        *beta = std::sqrt(dot(w.data(), w.data()));
-    }
-    void set_next(int j1, double beta) override {
-        double* d = v(j1);
-        for (uint64_t q = 0; q < n; ++q) d[q] = w[q] / beta;
-    }
-    void rotate(int m, int k, const double* Y) override {
-        std::vector<double> row(m);
-        for (uint64_t q = 0; q < n; ++q) {
-            for (int i = 0; i < m; ++i) row[i] = Vb[(size_t)i * n + q];
-            for (int c = 0; c < k; ++c) {
-                double s = 0.0;
-                for (int i = 0; i < m; ++i) s += row[i] * Y[(size_t)i * m + c];
-                Vb[(size_t)c * n + q] = s;
-            }
-        }
-    }
-    void move(int from, int to) override { std::memcpy(v(to), v(from), n * sizeof(double)); }
-    void normalize0() override {
-        double s = 1.0 / std::sqrt(dot(v(0), v(0)));
-        double* v0 = v(0);
-        for (uint64_t q = 0; q < n; ++q) v0[q] *= s;
-    }
-    void rayleigh(double* theta, double* resid) override {
-        double th = dot(v(0), w.data());
-        double r2 = 0.0;
-        const double* v0 = v(0);
-        for (uint64_t q = 0; q < n; ++q) {
-            double r = w[q] - th * v0[q];
-            r2 += r * r;
-        }
-        *theta = th;
-        *resid = std::sqrt(r2);
-    }
-};
-
-void sign_fix(std::vector<double>& x) {
-    size_t im = 0;
-    for (size_t q = 1; q < x.size(); ++q)
-        if (std::fabs(x[q]) > std::fabs(x[im])) im = q;
-    if (!x.empty() && x[im] < 0)
-        for (auto& v : x) v = -v;
-}
-
 // The solve's vector becomes the space's resident eigenvector.  With a host
 // buffer it is read back, given the sign convention (largest-magnitude
 // component positive) and returned; without one it stays on the device as it
@@ -898,13 +814,6 @@ void keep_eigvec(cf_space* sp, uint64_t n, const double* dvec, double* eigvec) {
     std::memcpy(eigvec, hv.data(), n * sizeof(double));
 }
 }  // namespace
-
-struct cf_rci {
-    std::unique_ptr<HostBackend> b;
-    std::unique_ptr<ThickRestartLanczos> sm;
-    uint64_t n = 0;
-    bool started = false, done = false;
-};
 
 // ---------------------------------------------------------------------------
 // API
@@ -957,10 +866,6 @@ SpmvChoice make_choice(int kernel, int p1, int p2, uint64_t db, int prec) {
 }  // namespace
 
 extern "C" {
-
-int cf_version(void) { return 100; }
-
-const char* cf_last_error(void) { return g_err.c_str(); }
 
 int cf_ctx_create(int device, cf_ctx** out) {
     API_BEGIN
@@ -1054,13 +959,6 @@ int cf_ctx_reset_timing(cf_ctx* ctx) {
     API_END
 }
 
-int cf_gen_integrals(int L, uint64_t seed, double* h, double* V) {
-    API_BEGIN
-    CF_REQUIRE(L >= 1 && L <= kMaxL && h && V, "bad arguments");
-    gen_integrals(L, seed, h, V);
-    API_END
-}
-
 int cf_ham_create(cf_ctx* ctx, int L, int nup, int ndown, const double* h, const double* V, double econst,
                   cf_ham** out) {
     API_BEGIN
@@ -1139,110 +1037,6 @@ int cf_space_create(cf_ham* ham, int kind, cf_space** out) {
     }
     CF_HIP(hipStreamSynchronize(c->stream));
     *out = sp.release();
-    API_END
-}
-
-int cf_read_integrals(const char* path, int* L, int* nup, int* ndown, double* econst, double* h, double* V,
-                      int* irreps) {
-    API_BEGIN
-    CF_REQUIRE(path && L && nup && ndown && econst, "NULL argument");
-    std::ifstream f(path);
-    if (!f.is_open()) throw Error(ERR_ARG, std::string("cannot open ") + path);
-    std::string line;
-    auto next = [&](const char* what) {
-        if (!std::getline(f, line)) throw Error(ERR_ARG, std::string("integral file ends before ") + what);
-        while (!line.empty() && line.back() == '\r') line.pop_back();
-    };
-    auto value_of = [&](const char* what) -> std::string {
-        next(what);
-        const size_t eq = line.find('=');
-        if (eq == std::string::npos) throw Error(ERR_ARG, std::string("integral file: no '=' on the ") + what + " line");
-        return line.substr(eq + 1);
-    };
-    do next("the \"****  Molecular Integrals\" header");
-    while (line.find("****  Molecular Integrals") == std::string::npos);
-    const int na = std::atoi(value_of("Nalpha").c_str());
-    const int nb = std::atoi(value_of("Nbeta").c_str());
-    value_of("Symmetry Label");
-    value_of("Nirreps");
-    const double ec = std::strtod(value_of("Nuclear Repulsion Energy").c_str(), nullptr);
-    const int nL = std::atoi(value_of("Number Of Molecular Orbitals").c_str());
-    CF_REQUIRE(nL >= 1 && nL <= kMaxL, "integral file: number of orbitals out of range");
-    next("the irrep header");
-    next("the orbital irreps");
-    {
-        std::istringstream is(line);
-        for (int i = 0; i < nL; ++i) {
-            int ir = 0;
-            if (!(is >> ir)) throw Error(ERR_ARG, "integral file: fewer orbital irreps than orbitals");
-            if (irreps) irreps[i] = ir;
-        }
-    }
-    *L = nL;
-    *nup = na;
-    *ndown = nb;
-    *econst = ec;
-    if (!h && !V) return CF_OK;
-    CF_REQUIRE(h && V, "h and V must both be given");
-    const size_t L2 = (size_t)nL * nL, L4 = L2 * L2;
-    std::fill(h, h + L2, 0.0);
-    std::fill(V, V + L4, 0.0);
-    do next("the one-electron integrals");
-    while (line.rfind("**", 0) != 0);  // lines up to the first starting with "**"
-    auto bad = [&](int x) { return x < 0 || x >= nL; };
-    for (;;) {  // one-electron lines up to the next line starting with '*'
-        next("the two-electron integrals");
-        if (line.empty()) continue;
-        if (line[0] == '*') break;
-        std::istringstream is(line);
-        int i, j;
-        double v;
-        if (!(is >> i >> j >> v) || bad(i) || bad(j)) throw Error(ERR_ARG, "integral file: bad one-electron line: " + line);
-        h[(size_t)i * nL + j] = v;
-        h[(size_t)j * nL + i] = v;
-    }
-    auto at = [&](int p, int q, int r, int t) -> double& { return V[(((size_t)p * nL + q) * nL + r) * nL + t]; };
-    for (;;) {  // two-electron lines up to the next line starting with '*'
-        next("the end of the two-electron integrals");
-        if (line.empty()) continue;
-        if (line[0] == '*') break;
-        std::istringstream is(line);
-        int a, b, c, d;
-        double v;
-        if (!(is >> a >> b >> c >> d >> v) || bad(a) || bad(b) || bad(c) || bad(d))
-            throw Error(ERR_ARG, "integral file: bad two-electron line: " + line);
-        // chemist (ab|cd) and its 8-fold symmetry, stored as physics <ac|bd>
-        const int P[8][4] = {{a, b, c, d}, {b, a, c, d}, {a, b, d, c}, {b, a, d, c},
-                             {c, d, a, b}, {d, c, a, b}, {c, d, b, a}, {d, c, b, a}};
-        for (const auto& x : P) at(x[0], x[2], x[1], x[3]) = v;
-    }
-    API_END
-}
-
-int cf_read_determinants(const char* path, uint64_t cap, uint64_t* up, uint64_t* down, uint64_t* count) {
-    API_BEGIN
-    CF_REQUIRE(path && count, "NULL argument");
-    CF_REQUIRE(cap == 0 || (up && down), "NULL output arrays");
-    std::ifstream f(path);
-    if (!f.is_open()) throw Error(ERR_ARG, std::string("cannot open ") + path);
-    std::string line;
-    uint64_t n = 0;
-    while (std::getline(f, line)) {
-        while (!line.empty() && line.back() == '\r') line.pop_back();
-        if (line.find('#') != std::string::npos) continue;
-        const size_t cut = line.find('|');
-        if (cut == std::string::npos) continue;
-        if (cut > 64)
-            throw Error(ERR_UNSUPPORTED, "determinant strings longer than 64 orbitals");
-        const uint64_t u = std::strtoull(line.substr(0, cut).c_str(), nullptr, 2);
-        const uint64_t d = std::strtoull(line.substr(cut + 1).c_str(), nullptr, 2);
-        if (n < cap) {
-            up[n] = u;
-            down[n] = d;
-        }
-        ++n;
-    }
-    *count = n;
     API_END
 }
 
@@ -1514,6 +1308,26 @@ int cf_crs_export_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t
     return export_rows(m, row_begin, row_end, false, row, col, val);
 }
 
+int cf_crs_export_upper_u32(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint32_t* row, uint32_t* col,
+                            double* val) {
+    uint64_t count = 0;
+    int st = cf_crs_count_upper(m, row_begin, row_end, &count);
+    if (st != CF_OK) return st;
+    if (!row) return fail(CF_ERR_ARG, "NULL argument");
+    if (count > UINT32_MAX)
+        return fail(CF_ERR_UNSUPPORTED, "upper-triangle nonzeros " + std::to_string(count) +
+                                            " exceed the 32-bit row pointer of SparseMatrix_CRS");
+    try {
+        std::vector<uint64_t> r64(row_end - row_begin + 1);
+        st = export_rows(m, row_begin, row_end, true, r64.data(), col, val);
+        if (st != CF_OK) return st;
+        for (size_t i = 0; i < r64.size(); ++i) row[i] = (uint32_t)r64[i];
+    } catch (const std::bad_alloc&) {
+        return fail(CF_ERR_NOMEM, "host allocation failed");
+    }
+    return CF_OK;
+}
+
 int cf_mvprod(cf_crs* m, const double* x, double* y) {
     API_BEGIN
     CF_REQUIRE(m && x && y, "NULL argument");
@@ -1756,6 +1570,7 @@ struct Davidson {
     int M, keep;
     const double* diag;
     DBuf<double> V, S, x, hx, t, partial, dg, Yd, dc, pnorm, pnorm2, rparts, tparts;
+    DBuf<double> x1, r1parts, r1t, r1;  // second Ritz pair's residual (energy stop only)
     int nb;
     bool fused;  // small n: step kernels reduce their partials themselves
     int Gd;      // workgroups of the wide dot products
@@ -1815,6 +1630,24 @@ struct Davidson {
         launch_scale_copy(n, t.p, 1.0 / std::sqrt(hbuf[0]), v(0), c->stream);
         add_column();
     }
+    // ||S y1 - theta1 V y1|| for the Ritz vector y1 of the first k basis
+    // vectors: the residual of the second Ritz pair, which places a valid
+    // lower end theta1 - ||r1|| of the Kato-Temple gap.  Uses hx and t (free
+    // once the next basis vector is formed) and one more vector.
+    double ritz_residual(int k, const double* y1, double theta1) {
+        if (!x1.p) {
+            x1.alloc(n);
+            r1parts.alloc(256);
+            r1t.alloc(256);
+            r1.alloc(1);
+        }
+        launch_combine_residual(n, k, V.p, S.p, y1, theta1, 0.0, diag, x1.p, hx.p, t.p, r1parts.p, r1t.p, c->stream);
+        launch_reduce_partials(1, fused_axpy_blocks(n), r1parts.p, r1.p, c->stream);
+        double* hr = hbuf + 200;
+        CF_HIP(hipMemcpyAsync(hr, r1.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CF_HIP(hipStreamSynchronize(c->stream));
+        return std::sqrt(std::max(0.0, hr[0]));
+    }
     // Iterates to convergence; x holds the Ritz vector (|x| = 1) on return.
     // Returns 0 converged, 1 iteration limit, 2 breakdown, 3 stalled (with
     // stall > 0: no residual 10% below the best one for `stall` iterations,
@@ -1832,6 +1665,7 @@ struct Davidson {
         int since_best = 0;
         double* ys = hbuf + 256;  // Ritz coefficients (<= 24), passed to the kernels by value
         double* Ys = hbuf + 320;  // staged restart rotation (<= 24 x 24)
+        std::vector<double> y1;   // second Ritz vector's coefficients
         for (int it = 0;; ++it) {
             A.assign((size_t)m * m, 0.0);
             for (int i = 0; i < m; ++i)
@@ -1847,9 +1681,15 @@ struct Davidson {
                 m = keep;
                 ++nrestart;
                 for (int i = 0; i < m; ++i) ys[i] = i == 0 ? 1.0 : 0.0;
+                y1.assign(m, 0.0);
+                if (m >= 2) y1[1] = 1.0;
             } else {
                 for (int i = 0; i < m; ++i) ys[i] = Y[(size_t)i * m + 0];
+                y1.assign(m, 0.0);
+                if (m >= 2)
+                    for (int i = 0; i < m; ++i) y1[i] = Y[(size_t)i * m + 1];
             }
+            const int mk = m;  // basis size of this Ritz step
             const double shift = it < krylov ? 0.0 : 1e-4;
             if (fused) {  // 8 launches besides the product
                 launch_combine_residual(n, m, V.p, S.p, ys, th[0], shift, diag, x.p, hx.p, t.p, rparts.p, tparts.p,
@@ -1891,7 +1731,11 @@ struct Davidson {
             *res_out = res;
             if (progress) progress(napply, th[0], res);
             if (res <= tol * std::fabs(th[0])) return 0;
-            if (tol_energy > 0.0 && th.size() >= 2 && res * res <= tol_energy * (th[1] - th[0])) return 0;
+            // Kato-Temple: the optimistic gap theta1 - theta0 first, then the
+            // safeguarded one with the second pair's residual (one more pass)
+            if (tol_energy > 0.0 && mk >= 2 && th.size() >= 2 && res * res <= tol_energy * (th[1] - th[0]) &&
+                temple_accept(res, th[0], th[1], ritz_residual(mk, y1.data(), th[1]), tol_energy))
+                return 0;
             if (!(hbuf[63] > 1e-28)) return 2;
             if (it >= maxit) return 1;
             // a Ritz value that fell by more than the best residual belongs
@@ -2020,11 +1864,9 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
 struct cf_sigma {
     cf_space* sp = nullptr;
     bool f32 = false;
-    DBuf<double> Ha, Hb_own;
-    DBuf<float> Ha32, Hb32_own;
+    DBuf<double> Ha, Hb_own;  // dense same-spin string Hamiltonians
     const double* Hb = nullptr;
-    const float* Hb32 = nullptr;
-    DBuf<float> x32, y32;
+    DBuf<float> x32;          // FP32 operand of the alpha-beta kernel
     rocblas_handle blas = nullptr;  // the context's handle
     SigmaArgs args;
     DBuf<double> vpair64, vself64;  // slice tables of the prefetching alpha-beta kernel
@@ -2042,8 +1884,12 @@ namespace {
                                                  std::to_string((int)st_));                 \
     } while (0)
 
-// y = Ha C + C Hb^T + sigma_ab.  Row-major C (da x db) is the column-major
-// db x da matrix Cc, so y_c = Cc Ha + Hb Cc (Ha, Hb symmetric).
+// y = Ha C + C Hb^T + sigma_ab.  The same-spin parts are two FP64 GEMMs with
+// the dense string Hamiltonians for either operand precision (measured: a
+// kernel applying the excitation lists directly, with FP64 accumulation, was
+// 4-6x slower than the library GEMM at L = 11 and 16); the alpha-beta kernel
+// takes FP32 or FP64 operands and sums in FP64.  Row-major C (da x db) is the
+// column-major db x da matrix Cc, so y_c = Cc Ha + Hb Cc (Ha, Hb symmetric).
 void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
     cf_space* sp = S->sp;
     CF_REQUIRE(S->version == sp->version,
@@ -2051,23 +1897,19 @@ void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
     cf_ctx* c = sp->ham->ctx;
     const int m = (int)sp->db, nn = (int)sp->da;
     KTimer t(c, prec == CF_F32 ? "sigma_f32" : "sigma_f64");
+    const double one = 1.0, zero = 0.0;
+    CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, x, m, S->Ha.p,
+                          nn, &zero, y, m));
+    CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb, m, x, m,
+                          &one, y, m));
     if (prec == CF_F32) {
         CF_REQUIRE(S->f32, "sigma plan was created without FP32 operands");
-        const float one = 1.0f, zero = 0.0f;
         launch_to_f32(sp->n, x, S->x32.p, c->stream);
-        CF_BLAS(rocblas_sgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, S->x32.p, m,
-                              S->Ha32.p, nn, &zero, S->y32.p, m));
-        CF_BLAS(rocblas_sgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb32, m,
-                              S->x32.p, m, &one, S->y32.p, m));
-        launch_to_f64(sp->n, S->y32.p, y, c->stream);
-        launch_sigma_ab2<float>(S->args, S->shape, S->x32.p, y, c->stream);
+        if (S->shape.kernel != 3 || !launch_sigma_ab3<float>(S->args, sp->ham->ndown, S->x32.p, y, c->stream))
+            launch_sigma_ab2<float>(S->args, S->shape, S->x32.p, y, c->stream);
     } else {
-        const double one = 1.0, zero = 0.0;
-        CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, nn, &one, x, m,
-                              S->Ha.p, nn, &zero, y, m));
-        CF_BLAS(rocblas_dgemm(S->blas, rocblas_operation_none, rocblas_operation_none, m, nn, m, &one, S->Hb, m, x,
-                              m, &one, y, m));
-        launch_sigma_ab2<double>(S->args, S->shape, x, y, c->stream);
+        if (S->shape.kernel != 3 || !launch_sigma_ab3<double>(S->args, sp->ham->ndown, x, y, c->stream))
+            launch_sigma_ab2<double>(S->args, S->shape, x, y, c->stream);
     }
 }
 }  // namespace
@@ -2121,19 +1963,7 @@ int cf_sigma_create(cf_space* sp, int prec, cf_sigma** out) {
     } else {
         S->Hb = S->Ha.p;
     }
-    if (S->f32) {
-        S->Ha32.alloc(sp->da * sp->da);
-        launch_dense_string_ham<float>(sp->da, sp->ta->E, sp->ta->full.p, S->Ha32.p, c->stream);
-        if (sp->tb != sp->ta.get()) {
-            S->Hb32_own.alloc(sp->db * sp->db);
-            launch_dense_string_ham<float>(sp->db, sp->tb->E, sp->tb->full.p, S->Hb32_own.p, c->stream);
-            S->Hb32 = S->Hb32_own.p;
-        } else {
-            S->Hb32 = S->Ha32.p;
-        }
-        S->x32.alloc(sp->n);
-        S->y32.alloc(sp->n);
-    }
+    if (S->f32) S->x32.alloc(sp->n);
     S->blas = c->blas();
     // measured on MI355X: 256 beta strings per workgroup up to d_beta = 512 (L = 11:
     // 176 us against 187 us at 512), 512 up to L = 14 (d_beta <= 3432), 768 above
@@ -2182,7 +2012,7 @@ int cf_sigma_set_threads(cf_sigma* s, int threads) {
 int cf_sigma_set_kernel(cf_sigma* s, int kernel) {
     API_BEGIN
     CF_REQUIRE(s, "NULL argument");
-    CF_REQUIRE(kernel == 1 || kernel == 2, "kernel must be 1 or 2");
+    CF_REQUIRE(kernel == 1 || kernel == 2 || kernel == 3, "kernel must be 1, 2 or 3");
     s->shape.kernel = kernel;
     API_END
 }
@@ -2284,78 +2114,6 @@ int cf_rdm1_direct(cf_space* sp, const double* cvec, double* one) {
     API_END
 }
 
-int cf_rci_create(uint64_t n, int ncv, double tol, int maxrestart, uint64_t seed, cf_rci** out) {
-    API_BEGIN
-    CF_REQUIRE(out && n > 0, "bad arguments");
-    CF_REQUIRE(ncv >= 2 && ncv <= 63 && tol > 0.0, "bad ncv / tol");
-    std::unique_ptr<cf_rci> r(new cf_rci());
-    r->n = n;
-    r->b.reset(new HostBackend(n, ncv));
-    int maxr = maxrestart > 0 ? maxrestart : (int)std::min<uint64_t>(3 * n, 1u << 30);
-    r->sm.reset(new ThickRestartLanczos(r->b.get(), n, ncv, tol, maxr, seed, false));
-    *out = r.release();
-    API_END
-}
-
-int cf_rci_step(cf_rci* r, int* ido, double* workd, int* ipntr) {
-    API_BEGIN
-    CF_REQUIRE(r && ido && workd && ipntr, "NULL argument");
-    CF_REQUIRE(!r->done, "reverse communication already finished");
-    const uint64_t n = r->n;
-    if (*ido == 0) {
-        CF_REQUIRE(!r->started, "ido == 0 after the iteration started");
-        r->started = true;
-    } else if (*ido == 1) {
-        CF_REQUIRE(r->started, "ido == 1 before the first call");
-        std::memcpy(r->b->w.data(), workd + n, n * sizeof(double));
-    } else {
-        throw Error(ERR_ARG, "ido must be 0 or 1");
-    }
-    if (r->sm->step() == ThickRestartLanczos::NEED_APPLY) {
-        std::memcpy(workd, r->b->v(r->sm->pending()), n * sizeof(double));
-        ipntr[0] = 1;
-        ipntr[1] = (int)n + 1;
-        *ido = 1;
-    } else {
-        r->done = true;
-        *ido = 99;
-    }
-    API_END
-}
-
-int cf_rci_result(cf_rci* r, double* eigval, double* eigvec, int* info) {
-    API_BEGIN
-    CF_REQUIRE(r && r->done, "reverse communication not finished");
-    if (eigval) *eigval = r->sm->eigval();
-    if (info) *info = r->sm->status();
-    if (eigvec) {
-        std::vector<double> x(r->b->v(0), r->b->v(0) + r->n);
-        sign_fix(x);
-        std::memcpy(eigvec, x.data(), r->n * sizeof(double));
-    }
-    API_END
-}
-
-int cf_rci_destroy(cf_rci* r) {
-    API_BEGIN
-    delete r;
-    API_END
-}
-
-int cf_expm_skew(int L, const double* K, double* U) {
-    API_BEGIN
-    CF_REQUIRE(L >= 1 && L <= kMaxL && K && U, "bad arguments");
-    expm_skew(L, K, U);
-    API_END
-}
-
-int cf_gen_rotation(int L, uint64_t seed, double sigma, double* K) {
-    API_BEGIN
-    CF_REQUIRE(L >= 1 && L <= kMaxL && K, "bad arguments");
-    gen_rotation(L, seed, sigma, K);
-    API_END
-}
-
 int cf_rotate_integrals(cf_ctx* ctx, int L, int batch, const double* h, const double* V, const double* U,
                         double* h_out, double* V_out) {
     API_BEGIN
@@ -2434,6 +2192,152 @@ int cf_rdm(cf_space* sp, const double* cvec, double* one, double* two) {
                   two ? G.data() : nullptr);
     }
     pack_rdm(L, g.data(), two ? G.data() : nullptr, one, two);
+    API_END
+}
+
+// Batched small DOCI solves (one workgroup per request, see batch.hip).
+int cf_solve_doci_batch(cf_ctx* ctx, int count, const cf_request* reqs, cf_result* results) {
+    API_BEGIN
+    CF_REQUIRE(ctx && (count == 0 || (reqs && results)), "NULL argument");
+    CF_REQUIRE(count >= 0, "negative count");
+    if (!count) return CF_OK;
+    ctx->use();
+    hipStream_t st = ctx->stream;
+    const int cap = doci_batch_capacity();
+    cf_solve_opts dflt;
+    cf_solve_opts_default(&dflt);
+    std::vector<int> ok(count, 0);
+    for (int i = 0; i < count; ++i) {
+        const cf_request& r = reqs[i];
+        cf_result& o = results[i];
+        std::memset(&o, 0, sizeof(o));
+        o.tag = r.tag;
+        o.status = CF_ERR_ARG;
+        const char* why = nullptr;
+        if (r.kind != CF_DOCI) why = "batched solves are DOCI only";
+        else if (!r.h || !r.V) why = "NULL integrals";
+        else if (r.L < 1 || r.L > kBatchMaxL || r.nup != r.ndown || r.nup < 1 || r.nup >= r.L)
+            why = "batched DOCI needs 1 <= nalpha == nbeta < L <= 32";
+        else if (binom_host(r.L, r.nup) > (uint64_t)cap) why = "DOCI space too large for a batched solve";
+        if (why) {
+            std::snprintf(o.error, sizeof(o.error), "%s", why);
+            continue;
+        }
+        ok[i] = 1;
+        o.n = binom_host(r.L, r.nup);
+    }
+    // integrals: per L, one batched rotation (requests without U take the identity)
+    std::map<int, std::vector<int>> byL;
+    for (int i = 0; i < count; ++i)
+        if (ok[i]) byL[reqs[i].L].push_back(i);
+    std::vector<DBuf<double>> hbufs, vbufs;
+    std::vector<const double*> hptr(count, nullptr), vptr(count, nullptr);
+    for (auto& g : byL) {
+        const int L = g.first, B = (int)g.second.size();
+        const size_t L2 = (size_t)L * L, L4 = L2 * L2;
+        std::vector<double> hh(B * L2), vv(B * L4), uu(B * L2, 0.0);
+        for (int b = 0; b < B; ++b) {
+            const cf_request& r = reqs[g.second[b]];
+            std::memcpy(&hh[b * L2], r.h, L2 * sizeof(double));
+            std::memcpy(&vv[b * L4], r.V, L4 * sizeof(double));
+            if (r.U) std::memcpy(&uu[b * L2], r.U, L2 * sizeof(double));
+            else
+                for (int a = 0; a < L; ++a) uu[b * L2 + a * L + a] = 1.0;
+        }
+        DBuf<double> dh(B * L2), dV(B * L4), dU(B * L2), tmp(B * L4);
+        hbufs.emplace_back(B * L2);
+        vbufs.emplace_back(B * L4);
+        CF_HIP(hipMemcpyAsync(dh.p, hh.data(), dh.bytes(), hipMemcpyHostToDevice, st));
+        CF_HIP(hipMemcpyAsync(dV.p, vv.data(), dV.bytes(), hipMemcpyHostToDevice, st));
+        CF_HIP(hipMemcpyAsync(dU.p, uu.data(), dU.bytes(), hipMemcpyHostToDevice, st));
+        rotate_integrals_dev(L, B, dh.p, dV.p, dU.p, hbufs.back().p, vbufs.back().p, tmp.p, st);
+        CF_HIP(hipStreamSynchronize(st));  // the host staging arrays go out of scope
+        for (int b = 0; b < B; ++b) {
+            hptr[g.second[b]] = hbufs.back().p + b * L2;
+            vptr[g.second[b]] = vbufs.back().p + b * L4;
+        }
+    }
+    // shared tables of each (L, pairs) shape, built once per context
+    for (int i = 0; i < count; ++i) {
+        if (!ok[i]) continue;
+        const int L = reqs[i].L, np = reqs[i].nup;
+        auto key = std::make_pair(L, np);
+        if (ctx->doci_tabs.count(key)) continue;
+        cf_ctx::DociTables& T = ctx->doci_tabs[key];
+        const uint64_t n = binom_host(L, np);
+        T.s.alloc(n);
+        T.col.alloc(n * (uint64_t)np * (L - np));
+        launch_unrank(L, np, 0, n, ctx->binom.p, T.s.p, st);
+        DociEllArgs a;
+        a.n = n;
+        a.L = L;
+        a.np = np;
+        a.nv = L - np;
+        a.s = T.s.p;
+        launch_doci_ell_fill(a, ctx->binom.p, T.col.p, st);
+    }
+    // jobs by size class: one launch each
+    uint64_t scratch_n = 0;
+    for (int i = 0; i < count; ++i)
+        if (ok[i]) scratch_n += 3 * results[i].n;
+    DBuf<double> scratch(std::max<uint64_t>(scratch_n, 1)), dout(4 * (size_t)count);
+    std::vector<DociBatchJob> jobs;
+    std::vector<int> order;
+    uint64_t off = 0;
+    const int bounds[3] = {1024, 4096, cap};
+    std::vector<std::pair<size_t, size_t>> ranges;  // [first, last) of each class in jobs
+    int lo_n = 0;
+    for (int c = 0; c < 3; ++c) {
+        const size_t first = jobs.size();
+        for (int i = 0; i < count; ++i) {
+            if (!ok[i]) continue;
+            const int n = (int)results[i].n;
+            if (n <= lo_n || n > bounds[c]) continue;
+            const cf_request& r = reqs[i];
+            DociBatchJob J;
+            J.h = hptr[i];
+            J.V = vptr[i];
+            const cf_ctx::DociTables& T = ctx->doci_tabs[std::make_pair(r.L, r.nup)];
+            J.s = T.s.p;
+            J.col = T.col.p;
+            J.tol = r.tol > 0 ? r.tol : dflt.tol;
+            J.tol_energy = r.tol_energy > 0 ? r.tol_energy : 0.0;
+            J.seed = dflt.seed;
+            J.out = dout.p + 4 * (size_t)i;
+            J.scratch = scratch.p + off;
+            off += 3 * (uint64_t)n;
+            J.L = r.L;
+            J.np = r.nup;
+            J.nv = r.L - r.nup;
+            J.n = n;
+            J.check = r.check_every > 0 ? r.check_every : 2;
+            J.max_steps = kBatchMaxSteps;
+            jobs.push_back(J);
+            order.push_back(i);
+        }
+        ranges.emplace_back(first, jobs.size());
+        lo_n = bounds[c];
+    }
+    DBuf<DociBatchJob> djobs(std::max<size_t>(jobs.size(), 1));
+    if (!jobs.empty())
+        CF_HIP(hipMemcpyAsync(djobs.p, jobs.data(), jobs.size() * sizeof(DociBatchJob), hipMemcpyHostToDevice, st));
+    for (int c = 0; c < 3; ++c) {
+        const size_t a = ranges[c].first, b = ranges[c].second;
+        if (b > a) launch_doci_batch_lanczos(djobs.p + a, (int)(b - a), bounds[c], st);
+    }
+    std::vector<double> hout(4 * (size_t)count, 0.0);
+    CF_HIP(hipMemcpyAsync(hout.data(), dout.p, hout.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    CF_HIP(hipStreamSynchronize(st));
+    for (int i : order) {
+        cf_result& o = results[i];
+        o.status = CF_OK;
+        o.eigval = hout[4 * i + 0];
+        o.energy = o.eigval + reqs[i].econst;
+        o.resid = hout[4 * i + 1];
+        o.n_matvec = (int)hout[4 * i + 2];
+        o.n_matvec_f32 = 0;
+        o.solve_status = (int)hout[4 * i + 3];
+    }
     API_END
 }
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "errors.h"
+
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -13,36 +15,12 @@ namespace cf {
 
 typedef uint64_t str_t;  // one spin string: bit k = spatial orbital k occupied
 
-// Error carrying a status code across the C ABI.
-struct Error : std::runtime_error {
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-// Records msg as the calling thread's cf_last_error() text; returns code.
-int set_last_error(int code, const std::string& msg);
-
-enum Status {
-    OK = 0,
-    ERR_ARG = 1,
-    ERR_HIP = 2,
-    ERR_NOMEM = 3,
-    ERR_NOCONV = 4,
-    ERR_UNSUPPORTED = 5,
-    ERR_STATE = 6,
-};
-
 #define CF_HIP(call)                                                                          \
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
         if (e_ != hipSuccess)                                                                 \
             throw ::cf::Error(e_ == hipErrorOutOfMemory ? ::cf::ERR_NOMEM : ::cf::ERR_HIP,    \
                               std::string(#call) + ": " + hipGetErrorString(e_));             \
-    } while (0)
-
-#define CF_REQUIRE(cond, msg)                                  \
-    do {                                                       \
-        if (!(cond)) throw ::cf::Error(::cf::ERR_ARG, (msg));  \
     } while (0)
 
 // Stream that device buffers allocated on this host thread are ordered on:
@@ -112,7 +90,5 @@ inline uint64_t binom_host(int n, int k) {
     return (uint64_t)r;
 }
 
-constexpr int kMaxL = 64;
-constexpr int kBinomCols = 33;  // m <= 32 electrons per spin string
 
 }  // namespace cf

@@ -297,10 +297,10 @@ bool ThickRestartLanczos::absorb(int j1) {
     return false;
 }
 
-bool ThickRestartLanczos::accept(double res, const std::vector<double>& th) const {
+bool ThickRestartLanczos::accept(double res, const std::vector<double>& th, double res1) const {
     if (progress_) progress_(napply_, th[0], res);
     if (res <= tol_ * std::fabs(th[0])) return true;
-    return tol_energy_ > 0.0 && th.size() >= 2 && res * res <= tol_energy_ * (th[1] - th[0]);
+    return tol_energy_ > 0.0 && th.size() >= 2 && temple_accept(res, th[0], th[1], res1, tol_energy_);
 }
 
 // Convergence test on the leading m x m block in the middle of a cycle.
@@ -313,7 +313,7 @@ bool ThickRestartLanczos::converged_early(int m) {
     std::vector<double> th, zlast;
     sym_eig_last_row(m, A, th, zlast);
     const double res = std::fabs(beta_last_ * zlast[0]);
-    if (!accept(res, th)) return false;
+    if (!accept(res, th, m >= 2 ? std::fabs(beta_last_ * zlast[1]) : 0.0)) return false;
     std::vector<double> Y;
     sym_eig(m, std::move(A), th, Y);
     m_ = m;
@@ -332,7 +332,8 @@ void ThickRestartLanczos::eig_and_restart() {
     std::vector<double> th, Y;
     sym_eig(m, A, th, Y);
     const double res = std::fabs(beta_last_ * Y[(size_t)(m - 1) * m + 0]);
-    const bool converged = accept(res, th) || m < M_ || breakdown_;
+    const double res1 = m >= 2 ? std::fabs(beta_last_ * Y[(size_t)(m - 1) * m + 1]) : 0.0;
+    const bool converged = accept(res, th, res1) || m < M_ || breakdown_;
     const bool limit = nrestart_ >= maxrestart_;
     if (converged || limit) {
         status_ = converged ? 0 : 1;

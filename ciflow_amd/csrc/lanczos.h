@@ -52,13 +52,26 @@ void sym_eig(int m, std::vector<double> A, std::vector<double>& w, std::vector<d
 // O(m^2) when A is tridiagonal apart from a leading arrow block.
 void sym_eig_last_row(int m, std::vector<double> A, std::vector<double>& w, std::vector<double>& zlast);
 
+// Kato-Temple stop for the lowest Ritz pair (theta0, residual res) of a
+// Rayleigh-Ritz step: lambda_0 >= theta0 - res^2 / (mu - theta0) for any mu
+// with theta0 < mu <= lambda_1.  The second Ritz value theta1 lies at or above
+// lambda_1 (interlacing), so it is no valid mu by itself; mu = theta1 - res1,
+// with res1 the residual norm of the second Ritz pair, moves it down by the
+// distance within which that pair's eigenvalue is known.  Accepts when
+// mu > theta0 and res^2 <= tol * (mu - theta0).
+inline bool temple_accept(double res, double theta0, double theta1, double res1, double tol) {
+    const double mu = theta1 - res1;
+    return mu > theta0 && res * res <= tol * (mu - theta0);
+}
+
 class ThickRestartLanczos {
 public:
     enum Event { NEED_APPLY = 1, DONE = 99 };
     // check_every > 0 (deferred backends): also test convergence every
     // check_every steps inside a cycle, not only when the basis is full.
-    // tol_energy > 0: also accept the Ritz pair once ||r||^2 / (theta_1 -
-    // theta_0) <= tol_energy, the Kato-Temple bound on its eigenvalue error.
+    // tol_energy > 0: also accept the Ritz pair once the Kato-Temple bound on
+    // its eigenvalue error, ||r||^2 / (mu - theta_0), is <= tol_energy, with
+    // mu = theta_1 - ||r_1|| (temple_accept).
     ThickRestartLanczos(KrylovBackend* b, uint64_t n, int ncv, double tol, int maxrestart, uint64_t seed,
                         bool start_given, int check_every = 0, double tol_energy = 0.0);
     // Advance until the operator must be applied to v_{pending()} (NEED_APPLY,
@@ -88,7 +101,7 @@ private:
     int check_every_ = 0, filled_ = 0;
     double tol_energy_ = 0.0;
     std::function<void(int, double, double)> progress_;
-    bool accept(double res, const std::vector<double>& th) const;
+    bool accept(double res, const std::vector<double>& th, double res1) const;
     Phase phase_ = INIT;
     int pending_ = 0;
     std::vector<double> T_;

@@ -51,17 +51,29 @@ struct cf_server {
 };
 
 namespace {
+// Requests the batched one-launch DOCI solver takes (cf_solve_doci_batch):
+// the server's own choice (CF_SOLVER_AUTO), DOCI, a space of at most
+// kBatchN determinants.
+constexpr uint64_t kBatchN = 13312;
+constexpr size_t kBatchMax = 256;  // requests per batch (one workgroup each)
+
+bool batchable(const cf_request& r) {
+    return r.kind == CF_DOCI && r.method == CF_SOLVER_AUTO && r.nup == r.ndown && r.nup >= 1 && r.nup < r.L &&
+           r.L <= 32 && cf::binom_host(r.L, r.nup) <= kBatchN;
+}
+
 uint64_t estimate_footprint(const cf_request& r) {
     const int L = r.L;
     const uint64_t da = cf::binom_host(L, r.nup), db = cf::binom_host(L, r.ndown);
     const int ncv = r.ncv > 0 ? r.ncv : 42;
     const uint64_t ints = 3 * (uint64_t)L * L * L * L * 8;
     if (r.kind == CF_FCI && r.direct) {
-        // Krylov basis + dense string Hamiltonians (FP64 and FP32) + FP32 vectors + lists
+        // Krylov basis + dense FP64 string Hamiltonians + FP32 operand + lists
         const uint64_t n = da * db;
-        const uint64_t dense = (da * da + (r.nup == r.ndown ? 0 : db * db)) * 12;
-        return (uint64_t)(ncv + 3) * n * 8 + dense + 2 * n * 4 + (da + db) * 4096 + ints + (64ull << 20);
+        const uint64_t dense = (da * da + (r.nup == r.ndown ? 0 : db * db)) * 8;
+        return (uint64_t)(ncv + 3) * n * 8 + dense + n * 4 + (da + db) * 4096 + ints + (64ull << 20);
     }
+    if (batchable(r)) return 3 * da * 8 + 2 * ints + (1ull << 20);  // scratch + integrals (tables shared)
     uint64_t n, row;
     if (r.kind == CF_DOCI) {
         n = da;
@@ -80,6 +92,45 @@ uint64_t estimate_footprint(const cf_request& r) {
     return crs + krylov + ints + (64ull << 20);
 }
 
+void publish(cf_server* S, const cf_result& res, uint64_t footprint) {
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->in_use -= footprint;
+        S->running--;
+        S->finished++;
+        S->done.push_back(res);
+    }
+    S->cv_work.notify_all();
+    S->cv_done.notify_all();
+}
+
+void run_batch(cf_server* S, cf_ctx* ctx, std::vector<Job>& batch) {
+    std::vector<cf_request> reqs(batch.size());
+    std::vector<cf_result> res(batch.size());
+    for (size_t i = 0; i < batch.size(); ++i) {
+        reqs[i] = batch[i].req;
+        reqs[i].h = batch[i].h.data();
+        reqs[i].V = batch[i].V.data();
+        reqs[i].U = batch[i].U.empty() ? nullptr : batch[i].U.data();
+    }
+    const double t0 = now_s();
+    const int st = cf_solve_doci_batch(ctx, (int)batch.size(), reqs.data(), res.data());
+    const double t1 = now_s();
+    for (size_t i = 0; i < batch.size(); ++i) {
+        cf_result r = res[i];
+        if (st != CF_OK) {
+            std::memset(&r, 0, sizeof(r));
+            r.tag = batch[i].req.tag;
+            r.status = st;
+            std::snprintf(r.error, sizeof(r.error), "%s", cf_last_error());
+        }
+        r.t_submit = batch[i].t_submit;
+        r.t_start = t0;
+        r.t_end = t1;
+        publish(S, r, batch[i].footprint);
+    }
+}
+
 void worker_main(cf_server* S) {
     cf_ctx* ctx = nullptr;
     if (cf_ctx_create(S->device, &ctx) != CF_OK) {
@@ -92,6 +143,7 @@ void worker_main(cf_server* S) {
     }
     for (;;) {
         Job job;
+        std::vector<Job> batch;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             for (;;) {
@@ -110,10 +162,28 @@ void worker_main(cf_server* S) {
                     S->pending.erase(it);
                     S->in_use += job.footprint;
                     S->running++;
+                    if (batchable(job.req)) {
+                        // with it, every other queued small DOCI request that fits
+                        batch.push_back(std::move(job));
+                        for (auto jt = S->pending.begin(); jt != S->pending.end() && batch.size() < kBatchMax;) {
+                            if (batchable(jt->req) && S->in_use + jt->footprint <= S->budget) {
+                                S->in_use += jt->footprint;
+                                S->running++;
+                                batch.push_back(std::move(*jt));
+                                jt = S->pending.erase(jt);
+                            } else {
+                                ++jt;
+                            }
+                        }
+                    }
                     break;
                 }
                 S->cv_work.wait(lk);
             }
+        }
+        if (!batch.empty()) {
+            run_batch(S, ctx, batch);
+            continue;
         }
         cf_result res;
         std::memset(&res, 0, sizeof(res));
@@ -174,15 +244,7 @@ void worker_main(cf_server* S) {
         cf_ham_destroy(ham);
         cf_ctx_synchronize(ctx);
         res.t_end = now_s();
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            S->in_use -= job.footprint;
-            S->running--;
-            S->finished++;
-            S->done.push_back(res);
-        }
-        S->cv_work.notify_all();
-        S->cv_done.notify_all();
+        publish(S, res, job.footprint);
     }
 }
 }  // namespace

@@ -26,14 +26,13 @@ struct SigmaArgs {
 // Alpha-beta kernel choice: 1 = stage the V slice and C row per alpha action
 // (gathers V from L^4), 2 = slice tables with the next action's V slice and C
 // row prefetched into registers (double-buffered LDS up to 8 row elements per
-// thread).
+// thread), 3 = kernel 2's scheme with packed LDS offsets and exact list
+// lengths (its own workgroup sizes; shapes without an instance run kernel 2).
 struct SigmaShape {
     int threads = 512;
-    int kernel = 2;
+    int kernel = 3;
 };
 
-template <typename T>
-void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream_t s);
 size_t sigma_ab_lds_bytes(const SigmaArgs& a, bool f32);
 // threads: workgroup size of the alpha-beta kernel (128, 256, 512, 768 or 1024);
 // a workgroup owns one alpha string and `threads` beta strings.
@@ -46,6 +45,13 @@ void launch_sigma_tables(const SigmaArgs& a, CT* vpair, CT* vself, hipStream_t s
 template <typename CT>
 void launch_sigma_ab2(const SigmaArgs& a, SigmaShape sh, const CT* C, double* sig, hipStream_t s);
 size_t sigma_ab2_lds_bytes(const SigmaArgs& a, bool f32, bool dbuf);
+// Kernel 3 (packed LDS offsets, negated slice copy, exact list lengths); false
+// when the space's shape has no instance (the caller runs kernel 2).
+// nb_electrons: beta electrons (the number operators of every beta string).
+template <typename CT>
+bool launch_sigma_ab3(const SigmaArgs& a, int nb_electrons, const CT* C, double* sig, hipStream_t s);
+template <typename T>
+void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream_t s);
 void launch_to_f32(uint64_t n, const double* x, float* y, hipStream_t s);
 void launch_to_f64(uint64_t n, const float* x, double* y, hipStream_t s);
 void launch_row_dots(uint64_t da, uint64_t db, int S, const Exc* sub, const double* C, double* out, hipStream_t s);

@@ -8,8 +8,8 @@
 // Ha / Hb: the same-spin string Hamiltonians (one-string diagonal, single
 //   excitations h_pq + sum_z (V_pzqz - V_pzzq) with their phase, same-spin
 //   doubles) -- the entries of the sorted excitation lists, scattered into a
-//   dense d x d matrix once per Hamiltonian.  The two products are plain dense
-//   GEMMs (rocBLAS, FP64 or FP32 MFMA): 2 d^3 flop each.
+//   dense d x d matrix once per Hamiltonian.  The two products are FP64 GEMMs
+//   (rocBLAS on MFMA) for either operand precision: 2 d^3 flop each.
 // sigma_ab(Ia,Ib) = sum_{a in A(Ia)} sum_{b in A(Ib)} s_a s_b V[p_a][q_b][r_a][s_b] C(J_a, J_b),
 //   A(I) = the single excitations of I (p in I -> r, J = I - p + r, phase
 //   s) plus the number operators (p = r in I, J = I, s = +1): this is
@@ -47,7 +47,6 @@ void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream
     CF_HIP(hipGetLastError());
 }
 template void launch_dense_string_ham<double>(uint64_t, int, const Exc*, double*, hipStream_t);
-template void launch_dense_string_ham<float>(uint64_t, int, const Exc*, float*, hipStream_t);
 
 constexpr int kMaxSingles = 64;  // n (L - n) <= 64; L <= 31 (q = L marks the zero slot)
 
@@ -465,6 +464,208 @@ void launch_sigma_ab2(const SigmaArgs& a0, SigmaShape sh, const CT* C, double* s
 }
 template void launch_sigma_ab2<double>(const SigmaArgs&, SigmaShape, const double*, double*, hipStream_t);
 template void launch_sigma_ab2<float>(const SigmaArgs&, SigmaShape, const float*, double*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Kernel 3.  The same alpha-action loop as kernel 2 with the per-term work cut
+// to two LDS reads and three VALU operations.  Measured on kernel 2 (FCI
+// L=16, rocprofv3 counters): 12 VALU and 3 SALU wave-instructions per term
+// against 2.4 LDS instructions, with the LDS pipe 16% busy -- the kernel was
+// bound by instruction issue, not by LDS bandwidth or bank conflicts.  Here:
+//  * each beta action of a thread is one register holding two LDS byte
+//    offsets, (offset of C(J_b) in the staged row) << 13 | (offset of the V
+//    slice entry); the beta sign picks a negated copy of the slice staged
+//    next to it, so no sign select is left;
+//  * the beta number operators (q in Ib, J = Ib) are ordinary entries of the
+//    list, whose exact length NS is a template parameter (no list-length
+//    tests, fully unrolled);
+//  * FP32 operands: one FP32 product, converted and added in FP64; FP64
+//    operands: one FP64 FMA.
+// Staged buffer (bytes, element size s): [0, L^2 s) V slice, [L^2 s, 2 L^2 s)
+// its negation, [2 L^2 s, 2 L^2 s + d_b s) the C row.  BUFB > 0: two buffers
+// BUFB bytes apart (BUFB a compile-time constant, so buffer 1's reads carry
+// it as the instruction's immediate offset); BUFB = 0: one buffer and two
+// barriers per action.
+// ---------------------------------------------------------------------------
+template <typename CT, int NT, int PF, int PV, int NS, int BUFB, int LDSB>
+__global__ void __launch_bounds__(NT) k_sigma_ab3(SigmaArgs a, const CT* __restrict__ C, double* __restrict__ sig) {
+    // static LDS (the kernel's only LDS): its base is the constant 0, so the
+    // packed offsets address it directly
+    __shared__ __align__(16) unsigned char smem3[LDSB];
+    constexpr uint32_t S = sizeof(CT);
+    const uint64_t db = a.db;
+    const uint32_t nchunk = a.nchunk;
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t lb = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (lb >= a.nblocks) return;
+    const uint64_t Ia = lb / nchunk;
+    const uint64_t Ib = (uint64_t)(lb - Ia * nchunk) * NT + threadIdx.x;
+    const bool act = Ib < db;
+    const int L = a.L, L2 = L * L;
+    const uint32_t vneg = (uint32_t)L2 * S, crow = 2u * (uint32_t)L2 * S;
+    // this thread's beta actions: the singles of its self + singles list
+    // (sorted by target, the self entry J = Ib at position `self`), then its
+    // number operators; every register index is a compile-time constant
+    uint32_t code[NS];
+    {
+        const int ns1 = a.Sb - 1;
+        const Exc* lb_ = a.subb + (act ? Ib : 0) * (uint64_t)a.Sb;
+        int lo = 0, hi = a.Sb - 1;  // position of the self entry (J == Ib)
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (lb_[mid].J < (uint32_t)Ib) lo = mid + 1;
+            else hi = mid;
+        }
+        const int self = lo;
+        str_t u = act ? a.sb[Ib] : 0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            if (j < ns1) {
+                const Exc x = lb_[j + (j >= self ? 1 : 0)];
+                code[j] = ((crow + x.J * S) << 13) | ((uint32_t)(x.f * L + x.t) * S + (x.sign < 0 ? vneg : 0u));
+            } else {  // number operator q in Ib: J = Ib
+                const int q = u ? __builtin_ctzll(u) : 0;
+                u &= u - 1;
+                code[j] = ((crow + (uint32_t)(act ? Ib : 0) * S) << 13) | ((uint32_t)(q * L + q) * S);
+            }
+        }
+    }
+    const Exc* la = a.suba + Ia * (uint64_t)a.Sa;
+    const CT* vpair = vpair_tab<CT>(a);
+    const CT* vself = vself_tab<CT>(a) + Ia * (uint64_t)L2;
+    CT pv[PV], pf[PF];
+    bool neg_next = false;
+    auto fetch = [&](int e) {
+        const Exc x = la[e];
+        neg_next = x.type == 1 && x.sign < 0;
+        const CT* vsrc = x.type == 0 ? vself : vpair + (uint64_t)(x.f * L + x.t) * L2;
+#pragma unroll
+        for (int k = 0; k < PV; ++k) {
+            const int i = k * NT + threadIdx.x;
+            pv[k] = i < 2 * L2 ? vsrc[i < L2 ? i : i - L2] : CT(0);
+        }
+        const CT* src = C + (uint64_t)x.J * db;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const uint64_t i = (uint64_t)k * NT + threadIdx.x;
+            pf[k] = i < db ? src[i] : CT(0);
+        }
+    };
+    auto store = [&](unsigned char* buf) {
+        CT* V = (CT*)buf;
+        CT* row = (CT*)(buf + crow);
+#pragma unroll
+        for (int k = 0; k < PV; ++k) {
+            const int i = k * NT + threadIdx.x;
+            if (i < 2 * L2) V[i] = i < L2 ? pv[k] : -pv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const uint64_t i = (uint64_t)k * NT + threadIdx.x;
+            if (i < db) row[i] = pf[k];
+        }
+    };
+    auto terms = [&](const unsigned char* buf) -> double {
+        double t = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const uint32_t c = code[j];
+            const CT v = *(const CT*)(buf + (c & 0x1FFFu));
+            const CT x = *(const CT*)(buf + (c >> 13));
+            if (sizeof(CT) == 4) t += (double)(v * x);
+            else t = fma((double)v, (double)x, t);
+        }
+        return t;
+    };
+    fetch(0);
+    store(smem3);
+    bool neg = neg_next;
+    __syncthreads();
+    double acc = 0.0;
+    const int Sa = a.Sa;
+    if (BUFB > 0) {
+        for (int e = 0; e < Sa; e += 2) {
+            // action e from buffer 0, action e + 1 from buffer 1
+            if (e + 1 < Sa) fetch(e + 1);
+            if (act) {
+                const double t = terms(smem3);
+                acc += neg ? -t : t;
+            }
+            if (e + 1 >= Sa) break;
+            store(smem3 + BUFB);
+            neg = neg_next;
+            __syncthreads();
+            if (e + 2 < Sa) fetch(e + 2);
+            if (act) {
+                const double t = terms(smem3 + BUFB);
+                acc += neg ? -t : t;
+            }
+            if (e + 2 >= Sa) break;
+            store(smem3);
+            neg = neg_next;
+            __syncthreads();
+        }
+    } else {
+        for (int e = 0; e < Sa; ++e) {
+            const bool more = e + 1 < Sa;
+            if (more) fetch(e + 1);
+            if (act) {
+                const double t = terms(smem3);
+                acc += neg ? -t : t;
+            }
+            if (!more) break;
+            __syncthreads();  // every thread has read the buffer
+            store(smem3);
+            neg = neg_next;
+            __syncthreads();
+        }
+    }
+    if (act) sig[Ia * db + Ib] += acc;
+}
+
+// Shapes of kernel 3: NS = beta singles + beta electrons of the space;
+// workgroup size, row elements per thread and buffer spacing per NS.
+// LDSB: static LDS bytes (two buffers of BUFB, or one buffer of LDSB bytes).
+template <typename CT, int NT, int PF, int NS, int BUFB, int LDSB>
+static bool launch_ab3_t(const SigmaArgs& a0, const CT* C, double* sig, hipStream_t s) {
+    static_assert(LDSB <= (int)kLdsMax && (BUFB == 0 || LDSB == 2 * BUFB), "kernel 3 LDS layout");
+    constexpr uint32_t S = sizeof(CT);
+    const uint64_t L2 = (uint64_t)a0.L * a0.L;
+    const uint64_t buf = 2 * L2 * S + a0.db * S;
+    if (2 * L2 * S > 0x1FFFu || buf >= (1u << 19)) return false;   // the two packed offsets
+    if ((a0.db + NT - 1) / NT > (uint64_t)PF || 2 * L2 > (uint64_t)NT * 2) return false;
+    if (buf > (uint64_t)(BUFB > 0 ? BUFB : LDSB)) return false;
+    SigmaArgs a = a0;
+    a.nchunk = (uint32_t)((a.db + NT - 1) / NT);
+    const uint64_t nb = a.da * (uint64_t)a.nchunk;
+    if (nb > 0xFFFFFFF0ull) return false;
+    a.nblocks = (uint32_t)nb;
+    const uint64_t grid = (nb + 7) / 8 * 8;
+    if (2 * L2 <= (uint64_t)NT) k_sigma_ab3<CT, NT, PF, 1, NS, BUFB, LDSB><<<(unsigned)grid, NT, 0, s>>>(a, C, sig);
+    else k_sigma_ab3<CT, NT, PF, 2, NS, BUFB, LDSB><<<(unsigned)grid, NT, 0, s>>>(a, C, sig);
+    CF_HIP(hipGetLastError());
+    return true;
+}
+
+template <typename CT>
+bool launch_sigma_ab3(const SigmaArgs& a, int nb_electrons, const CT* C, double* sig, hipStream_t s) {
+    const bool tables = sizeof(CT) == 4 ? a.vpair32 != nullptr : a.vpair64 != nullptr;
+    if (!tables) return false;
+    const int ns = (a.Sb - 1) + nb_electrons;
+    constexpr int S = (int)sizeof(CT);
+    switch (ns) {
+        case 12: return launch_ab3_t<CT, 128, 1, 12, 4096, 8192>(a, C, sig, s);       // L=6 3b, L=7 2b
+        case 20: return launch_ab3_t<CT, 128, 1, 20, 4096, 8192>(a, C, sig, s);       // L=8 4b
+        case 24: return launch_ab3_t<CT, 128, 1, 24, 4096, 8192>(a, C, sig, s);       // L=9 4b
+        case 30: return launch_ab3_t<CT, 256, 1, 30, 4096, 8192>(a, C, sig, s);       // L=10 5b
+        case 35: return launch_ab3_t<CT, 256, 2, 35, 8192, 16384>(a, C, sig, s);      // L=11 5b
+        case 42: return launch_ab3_t<CT, 512, 2, 42, 16384, 32768>(a, C, sig, s);     // L=12 6b
+        // L=16 8b: d_b = 12,870, one buffer
+        case 72: return launch_ab3_t<CT, 512, 26, 72, 0, 2 * 256 * S + 12870 * S>(a, C, sig, s);
+        default: return false;
+    }
+}
+template bool launch_sigma_ab3<double>(const SigmaArgs&, int, const double*, double*, hipStream_t);
+template bool launch_sigma_ab3<float>(const SigmaArgs&, int, const float*, double*, hipStream_t);
 
 __global__ void k_to_f32(uint64_t n, const double* __restrict__ x, float* __restrict__ y) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

@@ -171,6 +171,12 @@ int cf_crs_info(const cf_crs* m, uint64_t* n, uint64_t* nnz_full, uint64_t* nnz_
 int cf_crs_count_upper(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* count);
 int cf_crs_export_upper(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* row, uint32_t* col,
                         double* val);
+/* The same export with the reference's 32-bit row pointer
+ * (std::vector<unsigned int> row, SparseMatrix_CRS.h:111-112): row has
+ * row_end-row_begin+1 uint32_t entries; CF_ERR_UNSUPPORTED (nothing written)
+ * when the exported count does not fit in 32 bits. */
+int cf_crs_export_upper_u32(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint32_t* row, uint32_t* col,
+                            double* val);
 /* Full-row export (both triangles), same conventions. */
 int cf_crs_count_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* count);
 int cf_crs_export_full(cf_crs* m, uint64_t row_begin, uint64_t row_end, uint64_t* row, uint32_t* col,
@@ -237,8 +243,12 @@ typedef struct {
                            inside a restart cycle; 0: only when the basis is full, as
                            ARPACK does.  The FP64 polish of a CF_F32 solve always tests
                            every check_every (4 when 0) steps. */
-    double tol_energy;  /* > 0: also stop once ||r||^2 / (theta_1 - theta_0) <= tol_energy
-                           (Kato-Temple bound on the eigenvalue error, Eh); 0: off */
+    double tol_energy;  /* > 0: also stop once ||r||^2 / (mu - theta_0) <= tol_energy with
+                           mu = theta_1 - ||r_1|| (second Ritz pair): the Kato-Temple
+                           estimate of the eigenvalue error (Eh).  It bounds the error
+                           once the Krylov space has resolved the second eigenvalue; a
+                           partner state closer than that is not seen and the estimate
+                           is then optimistic.  0: off */
     /* Called on the solving thread at every convergence test with the products
      * applied so far, the lowest Ritz value and its residual estimate; NULL: none. */
     void (*progress)(void* user, int n_matvec, double theta, double resid);
@@ -350,9 +360,10 @@ typedef struct {
     double tol_energy;  /* cf_solve_opts.tol_energy */
     int direct;         /* FCI only: 1 = on-the-fly sigma (cf_solve_direct) instead of a stored CRS */
     int method;         /* cf_solve_opts.method: CF_LANCZOS, CF_DAVIDSON, or CF_SOLVER_AUTO:
-                           Davidson for FCI and for DOCI spaces of >= 2,000
-                           determinants, Lanczos for the smallest DOCI ones (whose
-                           steps are launch-bound, where Lanczos's fused step wins) */
+                           the server packs DOCI requests of at most 13,312
+                           determinants into batched one-launch solves
+                           (cf_solve_doci_batch) and takes Davidson for FCI and
+                           the larger DOCI ones */
 } cf_request;
 typedef struct {
     uint64_t tag;
@@ -364,6 +375,20 @@ typedef struct {
     double t_submit, t_start, t_end;  /* steady-clock seconds */
     char error[160];
 } cf_result;
+/* Many small DOCI solves in one launch (the loop above issues them one after
+ * another: OrbitalOptimization::update_cim -> set_ham -> build_ci -> solve,
+ * SimulatedAnnealing.cpp:83-87): every request whose DOCI space has at most
+ * 13,312 determinants (nalpha == nbeta, L <= 32) is solved by one workgroup
+ * running FP64 Lanczos on the matrix of DOCI::construct_CI_matrix
+ * (CIMethod.cpp:611-660) applied without storing it; the integrals are
+ * rotated by U (or taken as they are when U is NULL) in batched FP64-MFMA
+ * transforms.  Stops as cf_solve_opts with tol (<= 0: 1e-12), tol_energy and
+ * check_every (<= 0: 2); at most 400 products (solve_status 1 beyond).  The
+ * precision, ncv, direct and method fields are not used (FP64 throughout).
+ * results[i] belongs to reqs[i]; a request that does not qualify gets an
+ * error status and leaves the others unaffected.  Synchronous on the context
+ * stream; each result is independent of the rest of the batch. */
+int cf_solve_doci_batch(cf_ctx* ctx, int count, const cf_request* reqs, cf_result* results);
 /* hbm_budget_bytes <= 0: 80% of the device memory free at creation. */
 int cf_server_create(int device, int slots, double hbm_budget_bytes, cf_server** out);
 int cf_server_submit(cf_server* s, const cf_request* r);

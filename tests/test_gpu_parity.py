@@ -113,6 +113,19 @@ def test_config1_digest_matches_golden():
     assert (n, nnz_up, nnz_full) == (4900, 886900, 1768900)
 
 
+def test_upper_export_with_32_bit_row_pointer():
+    """cf_crs_export_upper_u32: the reference's unsigned-int row pointer
+    (SparseMatrix_CRS.h:111-112), equal to the 64-bit export, whole and in a
+    row range."""
+    h, V = CA.gen_integrals(8, 0)
+    m = CA.FCI(CA.Hamiltonian(h, V, 4, 4)).get_mat()
+    for r0, r1 in [(0, m.getn()), (1234, 3001)]:
+        row, col, val = m.export_upper(r0, r1)
+        row32, col32, val32 = m.export_upper_u32(r0, r1)
+        assert row32.dtype == np.uint32 and np.array_equal(row32.astype(np.uint64), row)
+        assert np.array_equal(col32, col) and np.array_equal(val32, val)
+
+
 def test_kat_energies_from_the_integral_file():
     """Hamiltonian(filename): the reference's BeH2 dump read by the library,
     DOCI KAT -15.225362047972723 (TestDOCI.cpp:31) and the FCI energy."""
@@ -663,6 +676,27 @@ def test_direct_sigma_matches_crs_spmv(L, na, nb):
     assert np.max(np.abs(y1 - y0)) <= 1e-12 * np.max(np.abs(y0))
     y2 = direct.mvprod(x, precision=CA.CF_F32)
     assert np.max(np.abs(y2 - y0)) <= 3e-6 * np.max(np.abs(y0))
+
+
+@pytest.mark.parametrize("L,na,nb", [(6, 3, 3), (8, 4, 4), (9, 4, 4), (11, 5, 5), (12, 6, 6), (7, 3, 2)])
+def test_sigma_alpha_beta_kernels_agree(L, na, nb):
+    """Kernel 3 (packed offsets, the default) against kernel 2 and the stored
+    CRS SpMV: FP64 to 1e-12 relative, FP32 operands to the FP32 rounding."""
+    say("sigma kernels", L, na, nb)
+    h, V = CA.gen_integrals(L, 4)
+    ham = CA.Hamiltonian(h, V, na, nb)
+    direct = CA.FCI_Direct(ham, value_prec=CA.CF_F32)
+    x = np.random.default_rng(6).standard_normal(direct.get_dim())
+    ys = {}
+    for kern in (2, 3):
+        direct.set_kernel(kern)
+        ys[kern] = (direct.mvprod(x), direct.mvprod(x, precision=CA.CF_F32))
+    scale = np.max(np.abs(ys[2][0]))
+    assert np.max(np.abs(ys[3][0] - ys[2][0])) <= 1e-12 * scale
+    assert np.max(np.abs(ys[3][1] - ys[2][0])) <= 3e-6 * scale
+    if L <= 9:
+        y0 = CA.FCI(ham).get_mat().mvprod(x)
+        assert np.max(np.abs(ys[3][0] - y0)) <= 1e-12 * scale
 
 
 def test_direct_solve_and_rdm1_match_crs_path():
