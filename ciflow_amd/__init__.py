@@ -198,6 +198,7 @@ def lib():
         "cf_server_destroy": (C.c_int, [vp]),
         "cf_request_footprint": (C.c_int, [C.POINTER(Request), u64p]),
         "cf_solve_doci_batch": (C.c_int, [vp, C.c_int, C.POINTER(Request), C.POINTER(Result)]),
+        "cf_solve_batch": (C.c_int, [vp, C.c_int, C.POINTER(Request), C.POINTER(Result)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -857,6 +858,29 @@ def solve_doci_batch(requests, ctx: "Context | None" = None) -> list[dict]:
         keep.append(arrs)
     res = (Result * len(requests))()
     _check(lib().cf_solve_doci_batch(ctx.ptr, len(requests), reqs, res))
+    return [res[i].as_dict() for i in range(len(requests))]
+
+
+def solve_batch(requests, ctx: "Context | None" = None) -> list[dict]:
+    """cf_solve_batch: batched Davidson solves, requests of one shape advanced
+    in lock step.  requests: dicts with kind (CF_FCI / CF_DOCI), h, V, nup,
+    ndown (or npair) and optionally U, econst, precision (CF_F32), ncv, tol,
+    tol_energy, tag; FCI requests run on the on-the-fly sigma.  Returns one
+    result dict per request (SolveServer.poll's fields)."""
+    ctx = ctx or default_context()
+    reqs = (Request * len(requests))()
+    keep = []
+    for i, q in enumerate(requests):
+        nup = q.get("nup", q.get("npair"))
+        ndown = q.get("ndown", q.get("npair"))
+        r, arrs = SolveServer._request(q.get("tag", i), q.get("kind", CF_FCI), q["h"], q["V"], nup, ndown,
+                                       q.get("U"), q.get("econst", 0.0), q.get("precision", CF_F32), q.get("ncv", 0),
+                                       q.get("tol", 0.0), 0, q.get("tol_energy", 0.0), True,
+                                       q.get("method", CF_SOLVER_AUTO))
+        reqs[i] = r
+        keep.append(arrs)
+    res = (Result * len(requests))()
+    _check(lib().cf_solve_batch(ctx.ptr, len(requests), reqs, res))
     return [res[i].as_dict() for i in range(len(requests))]
 
 

@@ -25,8 +25,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-u
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-SOURCES = ["strings.hip", "build.hip", "spmv.hip", "spmv_fci.hip", "spmv_ell.hip", "spmv_doci.hip", "krylov.hip", "rdm.hip", "transform.hip", "sigma.hip", "noalloc.hip", "batch.hip",
-           "lanczos.cpp", "integrals.cpp", "host.cpp", "capi.cpp", "server.cpp"]
+SOURCES = ["strings.hip", "build.hip", "spmv.hip", "spmv_fci.hip", "spmv_ell.hip", "spmv_doci.hip", "krylov.hip", "rdm.hip", "transform.hip", "sigma.hip", "noalloc.hip", "batch.hip", "davbatch.hip",
+           "lanczos.cpp", "integrals.cpp", "host.cpp", "capi.cpp", "solve_batch.cpp", "server.cpp"]
 
 
 def _compile(src: str) -> str:

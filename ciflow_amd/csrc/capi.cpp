@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "capi_types.h"
 #include "kernels.h"
 #include "host.h"
 #include "lanczos.h"
@@ -66,91 +67,6 @@ void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hip
 // ---------------------------------------------------------------------------
 // handles
 // ---------------------------------------------------------------------------
-struct TimerFamily {
-    int64_t launches = 0;
-    double ms = 0.0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
-};
-
-struct cf_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    DBuf<uint64_t> binom;  // C(k, m): (kMaxL+1) x kBinomCols
-    bool timing = false;
-    std::map<std::string, TimerFamily> timers;
-    // strings and hop columns of DOCI spaces, per (L, pairs), for batched solves
-    struct DociTables {
-        DBuf<str_t> s;
-        DBuf<uint32_t> col;
-    };
-    std::map<std::pair<int, int>, DociTables> doci_tabs;
-    std::vector<hipEvent_t> pool;
-    DBuf<double> hx, hy;  // staging for host-pointer products
-    double* pinned = nullptr;  // kPinned doubles of page-locked memory for small read-backs
-    // [0, 256): scalar read-backs; [256, 256 + 64*64): Ritz-vector coefficients
-    static constexpr int kPinned = 256 + 64 * 64;
-    rocblas_handle blas_ = nullptr;  // created on first use, bound to the stream
-
-    rocblas_handle blas() {
-        if (!blas_) {
-            if (rocblas_create_handle(&blas_) != rocblas_status_success) {
-                blas_ = nullptr;
-                throw Error(ERR_HIP, "rocblas_create_handle failed");
-            }
-            if (rocblas_set_stream(blas_, stream) != rocblas_status_success)
-                throw Error(ERR_HIP, "rocblas_set_stream failed");
-        }
-        return blas_;
-    }
-
-    void use() {
-        CF_HIP(hipSetDevice(device));
-        tls_alloc_stream = stream;
-    }
-    hipEvent_t event() {
-        if (!pool.empty()) {
-            hipEvent_t e = pool.back();
-            pool.pop_back();
-            return e;
-        }
-        hipEvent_t e;
-        CF_HIP(hipEventCreate(&e));
-        return e;
-    }
-    void flush(TimerFamily& f) {
-        for (auto& pr : f.pending) {
-            CF_HIP(hipEventSynchronize(pr.second));
-            float ms = 0.f;
-            CF_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
-            f.ms += ms;
-            f.launches++;
-            pool.push_back(pr.first);
-            pool.push_back(pr.second);
-        }
-        f.pending.clear();
-    }
-    ~cf_ctx() {
-        for (auto& kv : timers)
-            for (auto& pr : kv.second.pending) {
-                (void)hipEventDestroy(pr.first);
-                (void)hipEventDestroy(pr.second);
-            }
-        for (auto e : pool) (void)hipEventDestroy(e);
-        if (pinned) (void)hipHostFree(pinned);
-        if (blas_) (void)rocblas_destroy_handle(blas_);
-        // buffers ordered on the stream go back to the pool before it is destroyed
-        binom.release();
-        hx.release();
-        hy.release();
-        doci_tabs.clear();
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-        if (tls_alloc_stream == stream) tls_alloc_stream = nullptr;
-    }
-};
-
 namespace cf {
 hipStream_t ctx_stream(const cf_ctx* c) { return c->stream; }
 int ctx_device(const cf_ctx* c) { return c->device; }
@@ -177,38 +93,6 @@ struct KTimer {
     }
 };
 }  // namespace
-
-struct cf_ham {
-    cf_ctx* ctx = nullptr;
-    int L = 0, nup = 0, ndown = 0;
-    double econst = 0.0;
-    DBuf<double> dh, dV, dW, dJm;
-    HamDev dev() const { return HamDev{L, dh.p, dV.p, dW.p, dJm.p}; }
-};
-
-struct StrTab {
-    int L = 0, nel = 0;
-    uint64_t d = 0;
-    DBuf<str_t> s;
-    DBuf<double> d1;
-    int E = 0, S = 0;
-    DBuf<Exc> full, sub;
-};
-
-struct cf_space {
-    cf_ham* ham = nullptr;
-    int kind = CF_FCI;
-    uint64_t da = 0, db = 0, n = 0;
-    std::unique_ptr<StrTab> ta, tb_own;
-    StrTab* tb = nullptr;
-    DBuf<uint32_t> segoff;
-    DBuf<double> diag;
-    DBuf<double> vec;  // resident eigenvector after cf_solve
-    bool vec_valid = false;
-    bool vec_sign_pending = false;  // sign convention applied when the vector is read
-    uint64_t version = 0;  // incremented by cf_space_set_ham
-    DBuf<str_t> dup, ddown;  // CF_DETS: the determinant list, list order
-};
 
 // One SpMV launch configuration: kernel 1 (row gather, `lanes` per row),
 // 3 (vectorised row gather, `lanes` per row, `unroll`) or 4 (FCI ELL, `ell`).
@@ -263,7 +147,7 @@ struct cf_crs {
     std::unique_ptr<FciSpmvPlan> plan;
 };
 
-namespace {
+namespace cf {
 // One-string diagonal parts and excitation lists (values included) of a
 // string table for the Hamiltonian H; the strings themselves do not depend on H.
 void fill_strtab(cf_ctx* ctx, const cf_ham* H, StrTab* t) {
@@ -307,6 +191,62 @@ StrTab* make_strtab(cf_ctx* ctx, const cf_ham* H, int nel, bool lists) {
     return t.release();
 }
 
+void make_doci_block_tables(cf_ctx* c, int L, int np, int w, cf_ctx::DociBlockTables& T) {
+    const int nh = L - w;
+    CF_REQUIRE(w >= 1 && w <= kDociMaxW && nh >= 1 && nh <= 20,
+               "kernel 6: 1 <= w <= 12 low orbitals and 1..20 high orbitals");
+    std::vector<DociBlock> blocks;
+    for (uint32_t hi = 0; hi < (1u << nh); ++hi) {
+        const int ch = __builtin_popcount(hi), cl = np - ch;
+        if (cl < 0 || cl > w) continue;
+        DociBlock b;
+        b.hi = hi;
+        b.c = (uint32_t)cl;
+        b.R = (uint32_t)binom_host(w, cl);
+        uint64_t base = 0;
+        int mm = cl + 1;
+        for (uint32_t t = hi; t; t &= t - 1) base += binom_host(w + __builtin_ctz(t), mm++);
+        b.base = (uint32_t)base;
+        blocks.push_back(b);
+    }
+    std::sort(blocks.begin(), blocks.end(), [](const DociBlock& x, const DociBlock& y) { return x.base < y.base; });
+    const uint32_t W = 1u << w;
+    std::vector<uint16_t> rank_lo(W), lo_of;
+    std::vector<uint32_t> lo_off(w + 2, 0);
+    for (int cl = 0; cl <= w; ++cl) {
+        lo_off[cl] = (uint32_t)lo_of.size();
+        for (uint32_t p = 0; p < W; ++p)  // increasing integers = rank order within a popcount
+            if (__builtin_popcount(p) == cl) {
+                rank_lo[p] = (uint16_t)(lo_of.size() - lo_off[cl]);
+                lo_of.push_back((uint16_t)p);
+            }
+    }
+    lo_off[w + 1] = (uint32_t)lo_of.size();
+    T.blocks.alloc(blocks.size());
+    T.rank_lo.alloc(W);
+    T.lo_of.alloc(W);
+    T.lo_off.alloc(w + 2);
+    copy_sync(T.blocks.p, blocks.data(), blocks.size() * sizeof(DociBlock), hipMemcpyHostToDevice, c->stream);
+    copy_sync(T.rank_lo.p, rank_lo.data(), W * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream);
+    copy_sync(T.lo_of.p, lo_of.data(), W * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream);
+    copy_sync(T.lo_off.p, lo_off.data(), (w + 2) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+    DociBlockArgs& b = T.args;
+    b.n = binom_host(L, np);
+    b.L = L;
+    b.np = np;
+    b.w = w;
+    b.blocks = T.blocks.p;
+    b.nblocks = (uint32_t)blocks.size();
+    b.rank_lo = T.rank_lo.p;
+    b.lo_of = T.lo_of.p;
+    b.lo_off = T.lo_off.p;
+    b.binom = c->binom.p;
+    b.diag = nullptr;
+    b.zero = 0.0;
+}
+}  // namespace cf
+
+namespace {
 void ensure_staging(cf_ctx* c, uint64_t n) {
     if (c->hx.n < n) c->hx.alloc(n);
     if (c->hy.n < n) c->hy.alloc(n);
@@ -404,58 +344,16 @@ void make_doci_ell(cf_crs* m) {
 void make_doci_blocks(cf_crs* m, int w) {
     make_doci_table(m);
     cf_space* sp = m->sp;
-    cf_ctx* c = sp->ham->ctx;
-    const int L = sp->ham->L, np = sp->ham->nup, nh = L - w;
-    CF_REQUIRE(w >= 1 && w <= kDociMaxW && nh >= 1 && nh <= 20,
-               "kernel 6: 1 <= w <= 12 low orbitals and 1..20 high orbitals");
-    std::vector<DociBlock> blocks;
-    for (uint32_t hi = 0; hi < (1u << nh); ++hi) {
-        const int ch = __builtin_popcount(hi), cl = np - ch;
-        if (cl < 0 || cl > w) continue;
-        DociBlock b;
-        b.hi = hi;
-        b.c = (uint32_t)cl;
-        b.R = (uint32_t)binom_host(w, cl);
-        uint64_t base = 0;
-        int mm = cl + 1;
-        for (uint32_t t = hi; t; t &= t - 1) base += binom_host(w + __builtin_ctz(t), mm++);
-        b.base = (uint32_t)base;
-        blocks.push_back(b);
-    }
-    std::sort(blocks.begin(), blocks.end(), [](const DociBlock& x, const DociBlock& y) { return x.base < y.base; });
-    const uint32_t W = 1u << w;
-    std::vector<uint16_t> rank_lo(W), lo_of;
-    std::vector<uint32_t> lo_off(w + 2, 0);
-    for (int cl = 0; cl <= w; ++cl) {
-        lo_off[cl] = (uint32_t)lo_of.size();
-        for (uint32_t p = 0; p < W; ++p)  // increasing integers = rank order within a popcount
-            if (__builtin_popcount(p) == cl) {
-                rank_lo[p] = (uint16_t)(lo_of.size() - lo_off[cl]);
-                lo_of.push_back((uint16_t)p);
-            }
-    }
-    lo_off[w + 1] = (uint32_t)lo_of.size();
-    m->dblk.alloc(blocks.size());
-    m->drank_lo.alloc(W);
-    m->dlo_of.alloc(W);
-    m->dlo_off.alloc(w + 2);
-    copy_sync(m->dblk.p, blocks.data(), blocks.size() * sizeof(DociBlock), hipMemcpyHostToDevice, c->stream);
-    copy_sync(m->drank_lo.p, rank_lo.data(), W * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream);
-    copy_sync(m->dlo_of.p, lo_of.data(), W * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream);
-    copy_sync(m->dlo_off.p, lo_off.data(), (w + 2) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
-    DociBlockArgs& b = m->bargs;
-    b.n = m->n;
-    b.L = L;
-    b.np = np;
-    b.w = w;
-    b.blocks = m->dblk.p;
-    b.nblocks = (uint32_t)blocks.size();
-    b.rank_lo = m->drank_lo.p;
-    b.lo_of = m->dlo_of.p;
-    b.lo_off = m->dlo_off.p;
-    b.binom = c->binom.p;
-    b.diag = sp->diag.p;
-    b.zero = m->zero;
+    cf_ctx::DociBlockTables T;
+    make_doci_block_tables(sp->ham->ctx, sp->ham->L, sp->ham->nup, w, T);
+    m->dblk = std::move(T.blocks);
+    m->drank_lo = std::move(T.rank_lo);
+    m->dlo_of = std::move(T.lo_of);
+    m->dlo_off = std::move(T.lo_off);
+    m->bargs = T.args;
+    m->bargs.n = m->n;
+    m->bargs.diag = sp->diag.p;
+    m->bargs.zero = m->zero;
     m->blk_ready = true;
 }
 
@@ -1860,20 +1758,6 @@ void run_solve(cf_space* sp, uint64_t n, const cf_solve_opts* opts, std::functio
     info->ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 }  // namespace
-
-struct cf_sigma {
-    cf_space* sp = nullptr;
-    bool f32 = false;
-    DBuf<double> Ha, Hb_own;  // dense same-spin string Hamiltonians
-    const double* Hb = nullptr;
-    DBuf<float> x32;          // FP32 operand of the alpha-beta kernel
-    rocblas_handle blas = nullptr;  // the context's handle
-    SigmaArgs args;
-    DBuf<double> vpair64, vself64;  // slice tables of the prefetching alpha-beta kernel
-    DBuf<float> vpair32, vself32;
-    SigmaShape shape;
-    uint64_t version = 0;  // space Hamiltonian version the tables were made for
-};
 
 namespace {
 #define CF_BLAS(call)                                                                       \

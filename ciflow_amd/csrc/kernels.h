@@ -199,4 +199,64 @@ void launch_rotate_basis(uint64_t n, int m, int k, double* Vb, const double* Y, 
 void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream);
 void launch_copy(uint64_t n, const double* src, double* dst, hipStream_t stream);
 
+// ---------------------------------------------------------------------------
+// Batched Davidson (davbatch.hip): `lanes` requests of one shape (same n)
+// advanced in lock step, one launch per step for all of them.
+// ---------------------------------------------------------------------------
+constexpr int kBdMaxLanes = 32;
+constexpr int kBdPartial = 64 * 64;  // per lane: dot partials, rows x fused_dot_blocks(n)
+constexpr int kBdParts = 5 * 256;    // per lane: pnorm, pnorm2, rparts, tparts, r1parts
+enum BdMode : int { BD_IDLE = 0, BD_RITZ = 1, BD_START_X = 2, BD_START_SEED = 3 };
+// One lane's part of an iteration, written by the host before each one.
+struct BdStep {
+    double ys[kDavMaxBasis];  // BD_RITZ: coefficients of the lowest Ritz vector
+    double y1[kDavMaxBasis];  // second Ritz vector (second-pair residual)
+    double theta, theta1, shift;
+    uint64_t seed;            // BD_START_SEED
+    int mode;
+    int m;        // basis vectors in use; the step forms and multiplies vector m
+    int f32;      // the product takes FP32 operand values
+    int restart;  // first rotate V and S to their `keep` Ritz vectors (Y: mr x mr)
+    int mr, keep;
+    int r2;       // second-pair residual requested
+    int pad;
+};
+// Per-lane operator data, fixed for a batch.
+struct BdOpLane {
+    const double* diag = nullptr;  // preconditioner diagonal
+    const double* Ha = nullptr;    // FCI: dense alpha string Hamiltonian (da x da)
+    const double* Hb = nullptr;    // dense beta string Hamiltonian (db x db)
+    const double* vk64 = nullptr;  // DOCI: pair table
+    const float* vk32 = nullptr;
+};
+// The batch's buffers: lane l's vectors at a fixed stride from the bases.
+struct BdView {
+    uint64_t n = 0;
+    int M = 0;           // basis vectors per lane
+    int Ga = 0, Gd = 0;  // fused_axpy_blocks(n), fused_dot_blocks(n)
+    double* V = nullptr;  // [lanes][M][n]
+    double* S = nullptr;  // [lanes][M][n] images H V
+    double* x = nullptr;  // [lanes][n] Ritz vector
+    double* hx = nullptr;
+    double* t = nullptr;  // correction / new vector before normalisation
+    float* x32 = nullptr;  // [lanes][n] FP32 copy of the new vector (or null)
+    double* partial = nullptr;  // [lanes][kBdPartial]
+    double* parts = nullptr;    // [lanes][kBdParts]
+    double* dg = nullptr;       // [lanes][64]: rows 0..m dots V^T S_m, 62 |r|^2, 63 |t|^2
+    double* Y = nullptr;        // [lanes][kDavMaxBasis^2] restart rotations
+    const BdStep* step = nullptr;  // [lanes]
+    const BdOpLane* op = nullptr;  // [lanes]
+};
+// Restart rotations, then the new vector of every lane: x = V y, r = S y -
+// theta x, t = r / (theta - diag), two selective Gram-Schmidt passes, v_m =
+// t / |t| (and its FP32 copy when x32 is set and the lane's product is FP32).
+void bd_form_vectors(const BdView& b, int lanes, bool any_restart, hipStream_t s);
+// dg rows 0..m: V^T S_m after the product.
+void bd_new_column(const BdView& b, int lanes, hipStream_t s);
+// r1parts of |S y1 - theta1 V y1|^2 for lanes with r2 set.
+void bd_resid2(const BdView& b, int lanes, hipStream_t s);
+// FCI same-spin part S_m = Ha C + C Hb (C = V_m as a row-major da x db matrix),
+// FP64 MFMA; overwrites S_m (the alpha-beta kernel adds to it).
+void bd_same_spin(const BdView& b, int lanes, uint64_t da, uint64_t db, hipStream_t s);
+
 }  // namespace cf

@@ -3,6 +3,7 @@
 // All HBM-bound: a basis of k vectors of length n is stored column-major
 // (vector i at Vb + i*n).
 #include "kernels.h"
+#include "krylov_dev.h"
 
 #include <algorithm>
 
@@ -16,37 +17,6 @@ int multi_dot_blocks(uint64_t n) {
     if (nb < 1) nb = 1;
     if (nb > 1024) nb = 1024;
     return (int)nb;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Selective second Gram-Schmidt pass (Daniel-Gragg-Kaufman-Stewart): the pass
-// is skipped when the first one kept more than eta of the vector,
-// sum(after) > eta^2 sum(before), both given as np <= 256 per-block parts.
-// Every thread of the block gets the same answer (fixed summation order).
-__device__ bool dgks_skip(const double* __restrict__ before, const double* __restrict__ after, int np, double eta2) {
-    __shared__ double red[2][16];
-    const int t = threadIdx.x;
-    double b = t < np ? before[t] : 0.0, a = t < np ? after[t] : 0.0;
-    b = wave_sum(b);
-    a = wave_sum(a);
-    if ((t & 63) == 0) {
-        red[0][t >> 6] = b;
-        red[1][t >> 6] = a;
-    }
-    __syncthreads();
-    double sb = 0.0, sa = 0.0;
-    const int waves = (blockDim.x + 63) >> 6;
-    for (int w = 0; w < waves; ++w) {
-        sb += red[0][w];
-        sa += red[1][w];
-    }
-    __syncthreads();
-    return sa > eta2 * sb;
 }
 
 // partial[i * nblocks + bx] = sum over this block's elements of Vb[i] . w
@@ -249,28 +219,6 @@ void launch_multi_dot_wide(uint64_t n, int k, const double* Vb, const double* w,
     CF_HIP(hipGetLastError());
 }
 
-// cs[i] = sum_b partial[i*G + b] for i < k (G a power of two <= 64, k <= 64),
-// the same arithmetic in every workgroup of 256 threads; all loads are issued
-// before the first shuffle.
-__device__ __forceinline__ void reduce_coeffs(int k, int G, const double* __restrict__ partial, double* cs) {
-    const int per_round = 256 / G;
-    const int gi = threadIdx.x / G, b = threadIdx.x % G;
-    double v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int i = r * per_round + gi;
-        v[r] = (i < k) ? partial[i * G + b] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        if (r * per_round >= k) break;
-        double x = v[r];
-        for (int off = G >> 1; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        const int i = r * per_round + gi;
-        if (b == 0 && i < k) cs[i] = x;
-    }
-}
-
 __global__ void __launch_bounds__(256) k_local3_r(uint64_t n, int G, const double* __restrict__ partial,
                                                   const double* __restrict__ v1, const double* __restrict__ b_in,
                                                   const double* __restrict__ v0, double* __restrict__ w,
@@ -415,14 +363,8 @@ void launch_rotate_basis(uint64_t n, int m, int k, double* Vb, const double* Y, 
 // Deterministic start vector: v[q] = 1 + 0.5*(u(seed, q) - 0.5), u in [0,1).
 __global__ void k_fill_start(uint64_t n, uint64_t seed, double* __restrict__ v) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
-        uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (q + 1);
-        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-        z ^= z >> 31;
-        double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
-        v[q] = 1.0 + 0.5 * (u - 0.5);
-    }
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+        v[q] = start_element(seed, q);
 }
 
 void launch_fill_start(uint64_t n, uint64_t seed, double* v, hipStream_t stream) {

@@ -11,6 +11,7 @@
 // different requests overlap on the device; a request's own kernels are
 // deterministic (fixed-order reductions), so its result does not depend on
 // what else is in flight.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -22,6 +23,7 @@
 
 #include "../../include/ciflow_amd.h"
 #include "common.h"
+#include "davbatch.h"
 
 namespace {
 double now_s() {
@@ -62,11 +64,32 @@ bool batchable(const cf_request& r) {
            r.L <= 32 && cf::binom_host(r.L, r.nup) <= kBatchN;
 }
 
+// Requests the batched Davidson solver takes (cf_solve_batch): the server's
+// own choice (CF_SOLVER_AUTO), FCI on the on-the-fly sigma or DOCI above the
+// one-launch size, of a shape it has kernels for.
+constexpr size_t kDavBatchMax = 32;  // requests per lock-step batch
+
+bool dav_batchable(const cf_request& r) {
+    return r.method == CF_SOLVER_AUTO && !batchable(r) && cf::bd_accepts(r, nullptr);
+}
+
 uint64_t estimate_footprint(const cf_request& r) {
     const int L = r.L;
     const uint64_t da = cf::binom_host(L, r.nup), db = cf::binom_host(L, r.ndown);
     const int ncv = r.ncv > 0 ? r.ncv : 42;
     const uint64_t ints = 3 * (uint64_t)L * L * L * L * 8;
+    if (dav_batchable(r)) {
+        // basis and images (2 M vectors), x, H x, t, the FP32 operand copy,
+        // integrals and the operator tables
+        const uint64_t n = r.kind == CF_DOCI ? da : da * db;
+        const uint64_t M = (uint64_t)std::min(std::min(ncv, 24), (int)std::min<uint64_t>(n, 24));
+        uint64_t ops = n * 8 + (uint64_t)L * L * 12;  // diagonal, pair tables
+        if (r.kind == CF_FCI) {
+            const uint64_t dense = (da * da + (r.nup == r.ndown ? 0 : db * db)) * 8;
+            ops += dense + (uint64_t)L * L * L * L * 12 + (da + db) * (uint64_t)L * L * 12 + (da + db) * 4096;
+        }
+        return (2 * M + 3) * n * 8 + n * 4 + ops + ints + (16ull << 20);
+    }
     if (r.kind == CF_FCI && r.direct) {
         // Krylov basis + dense FP64 string Hamiltonians + FP32 operand + lists
         const uint64_t n = da * db;
@@ -131,6 +154,34 @@ void run_batch(cf_server* S, cf_ctx* ctx, std::vector<Job>& batch) {
     }
 }
 
+// A lock-step batch of same-shape Davidson requests (cf_solve_batch).
+void run_dav_batch(cf_server* S, cf_ctx* ctx, std::vector<Job>& batch) {
+    std::vector<cf_request> reqs(batch.size());
+    std::vector<cf_result> res(batch.size());
+    for (size_t i = 0; i < batch.size(); ++i) {
+        reqs[i] = batch[i].req;
+        reqs[i].h = batch[i].h.data();
+        reqs[i].V = batch[i].V.data();
+        reqs[i].U = batch[i].U.empty() ? nullptr : batch[i].U.data();
+    }
+    const double t0 = now_s();
+    const int st = cf_solve_batch(ctx, (int)batch.size(), reqs.data(), res.data());
+    const double t1 = now_s();
+    for (size_t i = 0; i < batch.size(); ++i) {
+        cf_result r = res[i];
+        if (st != CF_OK) {
+            std::memset(&r, 0, sizeof(r));
+            r.tag = batch[i].req.tag;
+            r.status = st;
+            std::snprintf(r.error, sizeof(r.error), "%s", cf_last_error());
+        }
+        r.t_submit = batch[i].t_submit;
+        r.t_start = t0;
+        r.t_end = t1;
+        publish(S, r, batch[i].footprint);
+    }
+}
+
 void worker_main(cf_server* S) {
     cf_ctx* ctx = nullptr;
     if (cf_ctx_create(S->device, &ctx) != CF_OK) {
@@ -143,7 +194,7 @@ void worker_main(cf_server* S) {
     }
     for (;;) {
         Job job;
-        std::vector<Job> batch;
+        std::vector<Job> batch, dav;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             for (;;) {
@@ -162,7 +213,22 @@ void worker_main(cf_server* S) {
                     S->pending.erase(it);
                     S->in_use += job.footprint;
                     S->running++;
-                    if (batchable(job.req)) {
+                    if (dav_batchable(job.req)) {
+                        // with it, queued requests of the same shape that fit
+                        const uint64_t key = cf::bd_class_key(job.req);
+                        dav.push_back(std::move(job));
+                        for (auto jt = S->pending.begin(); jt != S->pending.end() && dav.size() < kDavBatchMax;) {
+                            if (dav_batchable(jt->req) && cf::bd_class_key(jt->req) == key &&
+                                S->in_use + jt->footprint <= S->budget) {
+                                S->in_use += jt->footprint;
+                                S->running++;
+                                dav.push_back(std::move(*jt));
+                                jt = S->pending.erase(jt);
+                            } else {
+                                ++jt;
+                            }
+                        }
+                    } else if (batchable(job.req)) {
                         // with it, every other queued small DOCI request that fits
                         batch.push_back(std::move(job));
                         for (auto jt = S->pending.begin(); jt != S->pending.end() && batch.size() < kBatchMax;) {
@@ -183,6 +249,10 @@ void worker_main(cf_server* S) {
         }
         if (!batch.empty()) {
             run_batch(S, ctx, batch);
+            continue;
+        }
+        if (!dav.empty()) {
+            run_dav_batch(S, ctx, dav);
             continue;
         }
         cf_result res;

@@ -33,6 +33,20 @@ struct SigmaShape {
     int kernel = 3;
 };
 
+// The lanes of a batched Davidson product (kernel 3): lane l (grid row l)
+// runs on args[l] with the new vector of its step as operand (V_m, or its
+// FP32 copy in x32) and adds into S_m; lanes whose step is idle or of the
+// other operand precision do nothing.
+struct SigmaBatch {
+    const SigmaArgs* args = nullptr;  // [lanes], nchunk / nblocks from sigma_ab3_batch_args
+    const BdStep* step = nullptr;
+    const double* V = nullptr;
+    double* S = nullptr;
+    const float* x32 = nullptr;
+    uint64_t n = 0;
+    int M = 0;
+};
+
 size_t sigma_ab_lds_bytes(const SigmaArgs& a, bool f32);
 // threads: workgroup size of the alpha-beta kernel (128, 256, 512, 768 or 1024);
 // a workgroup owns one alpha string and `threads` beta strings.
@@ -50,6 +64,12 @@ size_t sigma_ab2_lds_bytes(const SigmaArgs& a, bool f32, bool dbuf);
 // nb_electrons: beta electrons (the number operators of every beta string).
 template <typename CT>
 bool launch_sigma_ab3(const SigmaArgs& a, int nb_electrons, const CT* C, double* sig, hipStream_t s);
+// Sets a.nchunk / a.nblocks of kernel 3's instance for this shape and operand
+// precision; false when there is none.
+bool sigma_ab3_batch_args(SigmaArgs& a, int nb_electrons, bool f32);
+// shape: any lane's args (the lanes share da, db, L and the list lengths).
+template <typename CT>
+void launch_sigma_ab3_batch(const SigmaArgs& shape, int nb_electrons, const SigmaBatch& sb, int lanes, hipStream_t s);
 template <typename T>
 void launch_dense_string_ham(uint64_t d, int E, const Exc* full, T* H, hipStream_t s);
 void launch_to_f32(uint64_t n, const double* x, float* y, hipStream_t s);

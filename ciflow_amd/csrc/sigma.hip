@@ -486,12 +486,25 @@ template void launch_sigma_ab2<float>(const SigmaArgs&, SigmaShape, const float*
 // it as the instruction's immediate offset); BUFB = 0: one buffer and two
 // barriers per action.
 // ---------------------------------------------------------------------------
-template <typename CT, int NT, int PF, int PV, int NS, int BUFB, int LDSB>
-__global__ void __launch_bounds__(NT) k_sigma_ab3(SigmaArgs a, const CT* __restrict__ C, double* __restrict__ sig) {
+template <typename CT, int NT, int PF, int PV, int NS, int BUFB, int LDSB, bool BATCH>
+__global__ void __launch_bounds__(NT) k_sigma_ab3(SigmaArgs a0, const CT* __restrict__ C0, double* __restrict__ sig0,
+                                                  SigmaBatch sb) {
     // static LDS (the kernel's only LDS): its base is the constant 0, so the
     // packed offsets address it directly
     __shared__ __align__(16) unsigned char smem3[LDSB];
     constexpr uint32_t S = sizeof(CT);
+    SigmaArgs a = a0;
+    const CT* __restrict__ C = C0;
+    double* __restrict__ sig = sig0;
+    if (BATCH) {  // lane blockIdx.y of a batched Davidson product
+        const int lane = blockIdx.y;
+        const BdStep& st = sb.step[lane];
+        if (st.mode == BD_IDLE || st.f32 != (S == 4 ? 1 : 0)) return;
+        a = sb.args[lane];
+        const uint64_t off = ((uint64_t)lane * sb.M + st.m) * sb.n;
+        C = S == 4 ? reinterpret_cast<const CT*>(sb.x32 + (uint64_t)lane * sb.n) : reinterpret_cast<const CT*>(sb.V + off);
+        sig = sb.S + off;
+    }
     const uint64_t db = a.db;
     const uint32_t nchunk = a.nchunk;
     const uint32_t per = gridDim.x >> 3;
@@ -626,7 +639,7 @@ __global__ void __launch_bounds__(NT) k_sigma_ab3(SigmaArgs a, const CT* __restr
 // workgroup size, row elements per thread and buffer spacing per NS.
 // LDSB: static LDS bytes (two buffers of BUFB, or one buffer of LDSB bytes).
 template <typename CT, int NT, int PF, int NS, int BUFB, int LDSB>
-static bool launch_ab3_t(const SigmaArgs& a0, const CT* C, double* sig, hipStream_t s) {
+static bool ab3_prepare(const SigmaArgs& a0, SigmaArgs& a, uint64_t& grid) {
     static_assert(LDSB <= (int)kLdsMax && (BUFB == 0 || LDSB == 2 * BUFB), "kernel 3 LDS layout");
     constexpr uint32_t S = sizeof(CT);
     const uint64_t L2 = (uint64_t)a0.L * a0.L;
@@ -634,38 +647,87 @@ static bool launch_ab3_t(const SigmaArgs& a0, const CT* C, double* sig, hipStrea
     if (2 * L2 * S > 0x1FFFu || buf >= (1u << 19)) return false;   // the two packed offsets
     if ((a0.db + NT - 1) / NT > (uint64_t)PF || 2 * L2 > (uint64_t)NT * 2) return false;
     if (buf > (uint64_t)(BUFB > 0 ? BUFB : LDSB)) return false;
-    SigmaArgs a = a0;
+    a = a0;
     a.nchunk = (uint32_t)((a.db + NT - 1) / NT);
     const uint64_t nb = a.da * (uint64_t)a.nchunk;
     if (nb > 0xFFFFFFF0ull) return false;
     a.nblocks = (uint32_t)nb;
-    const uint64_t grid = (nb + 7) / 8 * 8;
-    if (2 * L2 <= (uint64_t)NT) k_sigma_ab3<CT, NT, PF, 1, NS, BUFB, LDSB><<<(unsigned)grid, NT, 0, s>>>(a, C, sig);
-    else k_sigma_ab3<CT, NT, PF, 2, NS, BUFB, LDSB><<<(unsigned)grid, NT, 0, s>>>(a, C, sig);
+    grid = (nb + 7) / 8 * 8;
+    return true;
+}
+
+// sb: a batched product over `lanes` lanes (grid rows), each with its own args.
+template <typename CT, int NT, int PF, int NS, int BUFB, int LDSB>
+static bool launch_ab3_t(const SigmaArgs& a0, const CT* C, double* sig, hipStream_t s, SigmaArgs* prepared = nullptr,
+                         const SigmaBatch* sb = nullptr, int lanes = 1) {
+    SigmaArgs a;
+    uint64_t grid = 0;
+    if (!ab3_prepare<CT, NT, PF, NS, BUFB, LDSB>(a0, a, grid)) return false;
+    if (prepared) {  // shape query only
+        *prepared = a;
+        return true;
+    }
+    const uint64_t L2 = (uint64_t)a0.L * a0.L;
+    const bool pv1 = 2 * L2 <= (uint64_t)NT;
+    if (sb) {
+        const dim3 g((unsigned)grid, (unsigned)lanes);
+        if (pv1) k_sigma_ab3<CT, NT, PF, 1, NS, BUFB, LDSB, true><<<g, NT, 0, s>>>(a, nullptr, nullptr, *sb);
+        else k_sigma_ab3<CT, NT, PF, 2, NS, BUFB, LDSB, true><<<g, NT, 0, s>>>(a, nullptr, nullptr, *sb);
+    } else {
+        const SigmaBatch none{};
+        if (pv1) k_sigma_ab3<CT, NT, PF, 1, NS, BUFB, LDSB, false><<<(unsigned)grid, NT, 0, s>>>(a, C, sig, none);
+        else k_sigma_ab3<CT, NT, PF, 2, NS, BUFB, LDSB, false><<<(unsigned)grid, NT, 0, s>>>(a, C, sig, none);
+    }
     CF_HIP(hipGetLastError());
     return true;
+}
+
+template <typename CT>
+static bool ab3_dispatch(const SigmaArgs& a, int nb_electrons, const CT* C, double* sig, hipStream_t s,
+                         SigmaArgs* prepared, const SigmaBatch* sb, int lanes) {
+    const int ns = (a.Sb - 1) + nb_electrons;
+    constexpr int S = (int)sizeof(CT);
+#define CF_AB3(NT, PF, NSV, BUFB, LDSB) \
+    launch_ab3_t<CT, NT, PF, NSV, BUFB, LDSB>(a, C, sig, s, prepared, sb, lanes)
+    switch (ns) {
+        case 12: return CF_AB3(128, 1, 12, 4096, 8192);       // L=6 3b, L=7 2b
+        case 20: return CF_AB3(128, 1, 20, 4096, 8192);       // L=8 4b
+        case 24: return CF_AB3(128, 1, 24, 4096, 8192);       // L=9 4b
+        case 30: return CF_AB3(256, 1, 30, 4096, 8192);       // L=10 5b
+        case 35: return CF_AB3(256, 2, 35, 8192, 16384);      // L=11 5b
+        case 42: return CF_AB3(512, 2, 42, 16384, 32768);     // L=12 6b
+        // L=16 8b: d_b = 12,870, one buffer
+        case 72: return CF_AB3(512, 26, 72, 0, 2 * 256 * S + 12870 * S);
+        default: return false;
+    }
+#undef CF_AB3
 }
 
 template <typename CT>
 bool launch_sigma_ab3(const SigmaArgs& a, int nb_electrons, const CT* C, double* sig, hipStream_t s) {
     const bool tables = sizeof(CT) == 4 ? a.vpair32 != nullptr : a.vpair64 != nullptr;
     if (!tables) return false;
-    const int ns = (a.Sb - 1) + nb_electrons;
-    constexpr int S = (int)sizeof(CT);
-    switch (ns) {
-        case 12: return launch_ab3_t<CT, 128, 1, 12, 4096, 8192>(a, C, sig, s);       // L=6 3b, L=7 2b
-        case 20: return launch_ab3_t<CT, 128, 1, 20, 4096, 8192>(a, C, sig, s);       // L=8 4b
-        case 24: return launch_ab3_t<CT, 128, 1, 24, 4096, 8192>(a, C, sig, s);       // L=9 4b
-        case 30: return launch_ab3_t<CT, 256, 1, 30, 4096, 8192>(a, C, sig, s);       // L=10 5b
-        case 35: return launch_ab3_t<CT, 256, 2, 35, 8192, 16384>(a, C, sig, s);      // L=11 5b
-        case 42: return launch_ab3_t<CT, 512, 2, 42, 16384, 32768>(a, C, sig, s);     // L=12 6b
-        // L=16 8b: d_b = 12,870, one buffer
-        case 72: return launch_ab3_t<CT, 512, 26, 72, 0, 2 * 256 * S + 12870 * S>(a, C, sig, s);
-        default: return false;
-    }
+    return ab3_dispatch<CT>(a, nb_electrons, C, sig, s, nullptr, nullptr, 1);
 }
 template bool launch_sigma_ab3<double>(const SigmaArgs&, int, const double*, double*, hipStream_t);
 template bool launch_sigma_ab3<float>(const SigmaArgs&, int, const float*, double*, hipStream_t);
+
+bool sigma_ab3_batch_args(SigmaArgs& a, int nb_electrons, bool f32) {
+    SigmaArgs out;
+    const bool ok = f32 ? ab3_dispatch<float>(a, nb_electrons, nullptr, nullptr, nullptr, &out, nullptr, 1)
+                        : ab3_dispatch<double>(a, nb_electrons, nullptr, nullptr, nullptr, &out, nullptr, 1);
+    if (ok) a = out;
+    return ok;
+}
+
+template <typename CT>
+void launch_sigma_ab3_batch(const SigmaArgs& shape, int nb_electrons, const SigmaBatch& sb, int lanes, hipStream_t s) {
+    if (lanes < 1 || lanes > kBdMaxLanes) throw Error(ERR_ARG, "batched sigma: 1..32 lanes");
+    if (!ab3_dispatch<CT>(shape, nb_electrons, nullptr, nullptr, s, nullptr, &sb, lanes))
+        throw Error(ERR_UNSUPPORTED, "batched sigma: no kernel 3 instance for this shape");
+}
+template void launch_sigma_ab3_batch<double>(const SigmaArgs&, int, const SigmaBatch&, int, hipStream_t);
+template void launch_sigma_ab3_batch<float>(const SigmaArgs&, int, const SigmaBatch&, int, hipStream_t);
 
 __global__ void k_to_f32(uint64_t n, const double* __restrict__ x, float* __restrict__ y) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;

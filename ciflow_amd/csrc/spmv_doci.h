@@ -52,12 +52,26 @@ struct DociBlockArgs {
     const double* diag = nullptr;
     double zero = 0.0;
 };
+// The lanes of a batched Davidson product (block-staged kernel): lane l
+// (grid row l) takes its pair table and diagonal from op[l] and multiplies
+// the new vector of its step, V_m, into S_m; lanes whose step is idle or of
+// the other value precision do nothing.
+struct DociBatch {
+    const BdStep* step = nullptr;
+    const BdOpLane* op = nullptr;
+    const double* V = nullptr;
+    double* S = nullptr;
+    uint64_t n = 0;
+    int M = 0;
+};
 // Largest low part: C(12, 6) = 924 rows per block, 4 per thread of 256.
 constexpr int kDociMaxW = 12;
 // Low orbitals for L orbitals (the block shape the kernel is run with by default).
 int doci_block_w(int L);
 void launch_spmv_doci_blk_f64(const DociBlockArgs& a, const double* vk, const double* x, double* y, hipStream_t s);
 void launch_spmv_doci_blk_f32(const DociBlockArgs& a, const float* vk, const double* x, double* y, hipStream_t s);
+// a: the shared block tables of the shape (its diag is not used).
+void launch_spmv_doci_blk_batch(const DociBlockArgs& a, bool f32, const DociBatch& bt, int lanes, hipStream_t s);
 
 // col of DociEllArgs from the row strings (binom: the rank table).
 void launch_doci_ell_fill(const DociEllArgs& a, const uint64_t* binom, uint32_t* col, hipStream_t s);

@@ -200,10 +200,26 @@ __device__ __forceinline__ int nth_bit(uint32_t v, int k) {  // position of the 
     return __builtin_ctz(v);
 }
 
-template <typename VT, int ROWS, int T>
-__global__ void __launch_bounds__(T) k_spmv_doci_blk(DociBlockArgs a, const VT* __restrict__ vk,
-                                                      const double* __restrict__ x, double* __restrict__ y) {
+template <typename VT, int ROWS, int T, bool BATCH>
+__global__ void __launch_bounds__(T) k_spmv_doci_blk(DociBlockArgs a, const VT* __restrict__ vk0,
+                                                      const double* __restrict__ x0, double* __restrict__ y0,
+                                                      DociBatch bt) {
     extern __shared__ double smem_blk[];
+    const VT* __restrict__ vk = vk0;
+    const double* __restrict__ x = x0;
+    double* __restrict__ y = y0;
+    const double* __restrict__ diag = a.diag;
+    if (BATCH) {  // lane blockIdx.y of a batched Davidson product
+        const int lane = blockIdx.y;
+        const BdStep& st = bt.step[lane];
+        if (st.mode == BD_IDLE || st.f32 != (sizeof(VT) == 4 ? 1 : 0)) return;
+        const BdOpLane& o = bt.op[lane];
+        vk = sizeof(VT) == 4 ? reinterpret_cast<const VT*>(o.vk32) : reinterpret_cast<const VT*>(o.vk64);
+        diag = o.diag;
+        const uint64_t off = ((uint64_t)lane * bt.M + st.m) * bt.n;
+        x = bt.V + off;
+        y = bt.S + off;
+    }
     __shared__ uint32_t hh_base[kBlkHops];
     __shared__ double hh_val[kBlkHops];
     __shared__ uint32_t sg_base[kBlkStages], sg_orb[kBlkStages];
@@ -274,7 +290,7 @@ __global__ void __launch_bounds__(T) k_spmv_doci_blk(DociBlockArgs a, const VT* 
     for (int q = 0; q < ROWS; ++q) {
         const uint32_t r = tid + q * T;
         if (r < R) {
-            double d = a.diag[base + r];
+            double d = diag[base + r];
             if (!(fabs(d) > a.zero)) d = 0.0;
             double s = (double)(VT)d * xo[r];
             const uint32_t l = lo[q], vl = ~l & wmask;
@@ -366,8 +382,9 @@ __global__ void __launch_bounds__(T) k_spmv_doci_blk(DociBlockArgs a, const VT* 
 
 int doci_block_w(int L) { return L / 2 < kDociMaxW ? L / 2 : kDociMaxW; }
 
-template <typename VT>
-static void launch_doci_blk(const DociBlockArgs& a, const VT* vk, const double* x, double* y, hipStream_t s) {
+template <typename VT, bool BATCH>
+static void launch_doci_blk_t(const DociBlockArgs& a, const VT* vk, const double* x, double* y, hipStream_t s,
+                              const DociBatch& bt, int lanes) {
     if (!a.nblocks) return;
     CF_REQUIRE(a.w >= 1 && a.w <= kDociMaxW && a.L - a.w >= 1 && a.L - a.w <= 20,
                "block-staged DOCI SpMV: 1 <= w <= 12 low orbitals and 1..20 high orbitals");
@@ -377,12 +394,24 @@ static void launch_doci_blk(const DociBlockArgs& a, const VT* vk, const double* 
     // neighbour-block bases are computed one per thread (T >= ch (nh - ch))
     const uint64_t maxR = binom_host(a.w, a.w / 2);
     const int nh = a.L - a.w, maxhh = (nh / 2) * ((nh + 1) / 2);
-    if (maxR <= 64 && maxhh <= 64) k_spmv_doci_blk<VT, 1, 64><<<a.nblocks, 64, shm, s>>>(a, vk, x, y);
-    else if (maxR <= 128 && maxhh <= 128) k_spmv_doci_blk<VT, 1, 128><<<a.nblocks, 128, shm, s>>>(a, vk, x, y);
-    else if (maxR <= 256) k_spmv_doci_blk<VT, 1, 256><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
-    else if (maxR <= 512) k_spmv_doci_blk<VT, 2, 256><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
-    else k_spmv_doci_blk<VT, 4, 256><<<a.nblocks, 256, shm, s>>>(a, vk, x, y);
+    const dim3 g(a.nblocks, (unsigned)lanes);
+    if (maxR <= 64 && maxhh <= 64) k_spmv_doci_blk<VT, 1, 64, BATCH><<<g, 64, shm, s>>>(a, vk, x, y, bt);
+    else if (maxR <= 128 && maxhh <= 128) k_spmv_doci_blk<VT, 1, 128, BATCH><<<g, 128, shm, s>>>(a, vk, x, y, bt);
+    else if (maxR <= 256) k_spmv_doci_blk<VT, 1, 256, BATCH><<<g, 256, shm, s>>>(a, vk, x, y, bt);
+    else if (maxR <= 512) k_spmv_doci_blk<VT, 2, 256, BATCH><<<g, 256, shm, s>>>(a, vk, x, y, bt);
+    else k_spmv_doci_blk<VT, 4, 256, BATCH><<<g, 256, shm, s>>>(a, vk, x, y, bt);
     CF_HIP(hipGetLastError());
+}
+
+template <typename VT>
+static void launch_doci_blk(const DociBlockArgs& a, const VT* vk, const double* x, double* y, hipStream_t s) {
+    launch_doci_blk_t<VT, false>(a, vk, x, y, s, DociBatch{}, 1);
+}
+
+void launch_spmv_doci_blk_batch(const DociBlockArgs& a, bool f32, const DociBatch& bt, int lanes, hipStream_t s) {
+    CF_REQUIRE(lanes >= 1 && lanes <= kBdMaxLanes, "batched DOCI product: 1..32 lanes");
+    if (f32) launch_doci_blk_t<float, true>(a, nullptr, nullptr, nullptr, s, bt, lanes);
+    else launch_doci_blk_t<double, true>(a, nullptr, nullptr, nullptr, s, bt, lanes);
 }
 
 void launch_spmv_doci_blk_f64(const DociBlockArgs& a, const double* vk, const double* x, double* y, hipStream_t s) {

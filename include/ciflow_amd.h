@@ -389,6 +389,23 @@ typedef struct {
  * error status and leaves the others unaffected.  Synchronous on the context
  * stream; each result is independent of the rest of the batch. */
 int cf_solve_doci_batch(cf_ctx* ctx, int count, const cf_request* reqs, cf_result* results);
+/* Batched Davidson solves of the same loop (no counterpart in the reference,
+ * which solves each request alone: SimulatedAnnealing.cpp:83-87,
+ * CIMethod.cpp:241-268): FCI requests on the on-the-fly sigma (direct = 1,
+ * the operator of cf_solve_direct) and DOCI requests (the matrix of
+ * DOCI::construct_CI_matrix, CIMethod.cpp:611-660, applied block-staged),
+ * method CF_SOLVER_AUTO or CF_DAVIDSON, grouped by shape (kind, L, nalpha,
+ * nbeta, basis size min(ncv, 24)) and advanced in lock step, up to 32
+ * requests per launch.  Each request runs the solver of cf_solve with
+ * CF_DAVIDSON: precision CF_F32 an FP32-operand phase (tol_f32 of
+ * cf_solve_opts_default, the tol_energy estimate, or a stall) then an FP64
+ * polish; stops on tol (<= 0: 1e-12) or the tol_energy estimate; integrals
+ * rotated by U (NULL: as given).  n_matvec counts the FP64 products,
+ * n_matvec_f32 the FP32 ones.  results[i] belongs to reqs[i]; a request the
+ * batch does not take gets an error status and leaves the others unaffected.
+ * Synchronous on the context stream; each result is the same in any batch
+ * and alone. */
+int cf_solve_batch(cf_ctx* ctx, int count, const cf_request* reqs, cf_result* results);
 /* hbm_budget_bytes <= 0: 80% of the device memory free at creation. */
 int cf_server_create(int device, int slots, double hbm_budget_bytes, cf_server** out);
 int cf_server_submit(cf_server* s, const cf_request* r);
