@@ -45,7 +45,7 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "spmv_doci", "rotate",
-                                          "sigma"],
+                                          "sigma", "batch"],
                    default="stream")
     p.add_argument("--steps", type=int, default=None,
                    help="stream: slices of the --requests stream (16); fci12: solves (4)")
@@ -60,6 +60,8 @@ def parse():
     p.add_argument("--no-latency", action="store_true", help="skip the Poisson latency run")
     p.add_argument("--L", type=int, default=12)
     p.add_argument("--sigma-L", default="8,9,10,11,12,16", help="sigma: orbital counts (FCI, half filling)")
+    p.add_argument("--batch-class", default="FCI11", help="batch: request class (FCI8..FCI11, DOCI17..DOCI20)")
+    p.add_argument("--lanes", type=int, default=32, help="batch: requests per lock-step batch")
     p.add_argument("--nel", type=int, default=6)
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--precision", choices=["f64", "f32"], default="f64")
@@ -308,6 +310,15 @@ def class_service_ms(reqs, res):
     return {k: {"n": len(v), "mean_ms": float(np.mean(v))} for k, v in sorted(by.items())}
 
 
+def latency_by_class(reqs, res):
+    """p50 / p99 latency (submit -> result, ms) and count per (kind, L) class."""
+    by = {}
+    for r, d in zip(reqs, res):
+        by.setdefault(f"{'DOCI' if r.kind == 1 else 'FCI'}{r.L}", []).append((d["t_end"] - d["t_submit"]) * 1e3)
+    return {k: {"n": len(v), "p50_ms": float(np.percentile(v, 50)), "p99_ms": float(np.percentile(v, 99))}
+            for k, v in sorted(by.items())}
+
+
 def _slices(total, k):
     """Split range(total) into k contiguous slices whose sizes differ by at most one."""
     edges = np.linspace(0, total, k + 1).round().astype(int)
@@ -360,6 +371,7 @@ def bench_stream(a, CA):
             log(f"Poisson arrivals at {lam:.1f}/s over {nlat} requests ...")
             lat = S.run_stream(lreqs, arrs[:nlat], server=srv, timeout=1500, **served)
             log(f"Poisson {lam:.1f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms")
+            lat["by_class"] = latency_by_class(lreqs, lat["results"])
     finally:
         srv.close()
     # the plain path: one request at a time, FP64 values, tolerance 1e-12 (ARPACK semantics),
@@ -424,7 +436,8 @@ def bench_stream(a, CA):
         "p50_ms": lat["p50_ms"],
         "p99_ms": lat["p99_ms"],
         "latency_run": {"arrivals": "Poisson", "rate_per_s": lam, "requests": nlat,
-                        "achieved_solves_per_s": lat["solves_per_s"], "mean_ms": lat["mean_ms"]},
+                        "achieved_solves_per_s": lat["solves_per_s"], "mean_ms": lat["mean_ms"],
+                        "by_class": lat.get("by_class")},
         "closed_batch_latency_ms": {"p50": out["p50_ms"], "p99": out["p99_ms"]},
         "one_at_a_time_fp64": ({"solves_per_s": base["solves_per_s"], "requests": len(sub),
                                 "stream_weighted_solves_per_s": base_stream_rate,
@@ -711,6 +724,46 @@ def bench_rotate(a, CA):
             "roofline": {"bound": "mfma", "achieved": tf, "peak": 78.6, "unit": "TFLOP/s", "frac": tf / 78.6}}
 
 
+def bench_batch(a, CA):
+    """Lock-step batched Davidson solves of one stream class (cf_solve_batch):
+    --lanes requests of class --batch-class (the stream's rotated seeded
+    integrals, FP32 operator + FP64 polish, energy stop 1e-9 Eh) per batch, one
+    warm-up batch, then --steps timed batches of fresh requests."""
+    from ciflow_amd import stream as S
+    cls = a.batch_class.upper()
+    kind = CA.CF_DOCI if cls.startswith("DOCI") else CA.CF_FCI
+    L = int(cls[4:] if kind == CA.CF_DOCI else cls[3:])
+    K = a.steps if a.steps is not None else 3
+    B = a.lanes
+    ctx = CA.Context(0)
+
+    def batch(first):
+        reqs = []
+        for k in range(B):
+            r = S.StreamRequest(k, kind, L, L // 2, first + k, 0.0)
+            h, V, U = S.request_arrays(r)
+            reqs.append(dict(kind=kind, h=h, V=V, nup=L // 2, ndown=L // 2, U=U, precision=CA.CF_F32,
+                             tol_energy=1e-9, tag=first + k))
+        return reqs
+
+    CA.solve_batch(batch(900000), ctx)
+    sets = [batch(910000 + 1000 * i) for i in range(K)]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    res = [CA.solve_batch(q, ctx) for q in sets]
+    dt = time.perf_counter() - t0
+    flat = [r for rs in res for r in rs]
+    assert all(r["status"] == 0 for r in flat)
+    rate = len(flat) / dt
+    n32 = float(np.mean([r["n_matvec_f32"] for r in flat]))
+    n64 = float(np.mean([r["n_matvec"] for r in flat]))
+    log(f"batch {cls} x {B}: {dt / K * 1e3:.1f} ms per batch, {rate:.1f} solves/s, products {n32:.1f} + {n64:.1f}")
+    return {"metric": "batched solves/s", "value": rate, "unit": "solves/s", "higher_is_better": True,
+            "workload": f"{B} {cls} requests per lock-step batch (cf_solve_batch), FP32 + FP64 polish, "
+                        f"energy stop 1e-9 Eh", "steps": K, "ms_per_batch": dt / K * 1e3,
+            "mean_products": {"f32": n32, "f64": n64}, "n": flat[0]["n"]}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -721,7 +774,7 @@ def main():
 
     run = {"stream": bench_stream, "fci12": bench_fci12, "doci24": bench_fci12, "direct16": bench_direct16,
            "spmv": bench_spmv, "spmv_doci": lambda a, CA: bench_spmv(a, CA, doci=True),
-           "rotate": bench_rotate, "sigma": bench_sigma}[a.workload]
+           "rotate": bench_rotate, "sigma": bench_sigma, "batch": bench_batch}[a.workload]
     out = run(a, CA)
     print(json.dumps(out))
 

@@ -16,6 +16,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -127,6 +128,15 @@ void publish(cf_server* S, const cf_result& res, uint64_t footprint) {
     S->cv_done.notify_all();
 }
 
+// Returns HBM held past its requests' results (a batch frees it at its end).
+void release(cf_server* S, uint64_t bytes) {
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->in_use -= bytes;
+    }
+    S->cv_work.notify_all();
+}
+
 void run_batch(cf_server* S, cf_ctx* ctx, std::vector<Job>& batch) {
     std::vector<cf_request> reqs(batch.size());
     std::vector<cf_result> res(batch.size());
@@ -154,32 +164,87 @@ void run_batch(cf_server* S, cf_ctx* ctx, std::vector<Job>& batch) {
     }
 }
 
-// A lock-step batch of same-shape Davidson requests (cf_solve_batch).
-void run_dav_batch(cf_server* S, cf_ctx* ctx, std::vector<Job>& batch) {
-    std::vector<cf_request> reqs(batch.size());
-    std::vector<cf_result> res(batch.size());
-    for (size_t i = 0; i < batch.size(); ++i) {
-        reqs[i] = batch[i].req;
-        reqs[i].h = batch[i].h.data();
-        reqs[i].V = batch[i].V.data();
-        reqs[i].U = batch[i].U.empty() ? nullptr : batch[i].U.data();
+// A lock-step batch of same-shape Davidson requests (cf_solve_batch's
+// solver): the batch reserves HBM for up to kDavBatchMax slots, takes the
+// queued requests of its shape that reach the front of the queue as slots
+// free up (continuous batching: a request that arrives while its shape is
+// being served joins the running batch instead of waiting for a worker; one
+// of another shape at the front stops the refill, so the batch drains and
+// frees its worker), publishes each result as soon as its request finishes,
+// and releases the reservation when it ends.
+void run_dav_batch(cf_server* S, cf_ctx* ctx, std::vector<Job>& first) {
+    const uint64_t key = cf::bd_class_key(first[0].req);
+    uint64_t fp = 0, held = 0;
+    for (const Job& j : first) {
+        fp = std::max(fp, j.footprint);
+        held += j.footprint;
     }
-    const double t0 = now_s();
-    const int st = cf_solve_batch(ctx, (int)batch.size(), reqs.data(), res.data());
-    const double t1 = now_s();
-    for (size_t i = 0; i < batch.size(); ++i) {
-        cf_result r = res[i];
-        if (st != CF_OK) {
-            std::memset(&r, 0, sizeof(r));
-            r.tag = batch[i].req.tag;
-            r.status = st;
-            std::snprintf(r.error, sizeof(r.error), "%s", cf_last_error());
+    int cap = (int)first.size();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        while (cap < (int)kDavBatchMax && S->in_use + fp <= S->budget) {
+            S->in_use += fp;
+            held += fp;
+            ++cap;
         }
-        r.t_submit = batch[i].t_submit;
-        r.t_start = t0;
-        r.t_end = t1;
-        publish(S, r, batch[i].footprint);
     }
+    std::deque<Job> jobs;  // by id; stable addresses for the arrays the batch reads
+    std::vector<double> t_in;
+    std::vector<char> out;
+    auto request_of = [](const Job& j) {
+        cf_request r = j.req;
+        r.h = j.h.data();
+        r.V = j.V.data();
+        r.U = j.U.empty() ? nullptr : j.U.data();
+        return r;
+    };
+    const double t0 = now_s();
+    std::vector<std::pair<int, cf_request>> in;
+    for (Job& j : first) {
+        jobs.push_back(std::move(j));
+        t_in.push_back(t0);
+        out.push_back(0);
+        in.emplace_back((int)jobs.size() - 1, request_of(jobs.back()));
+    }
+    cf::BatchFeed feed;
+    feed.more = [&](int max, std::vector<std::pair<int, cf_request>>& more) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) return;
+        const double t = now_s();
+        while ((int)more.size() < max && !S->pending.empty()) {
+            Job& f = S->pending.front();
+            if (!dav_batchable(f.req) || cf::bd_class_key(f.req) != key || f.footprint > fp) break;
+            jobs.push_back(std::move(f));
+            S->pending.pop_front();
+            S->running++;
+            t_in.push_back(t);
+            out.push_back(0);
+            more.emplace_back((int)jobs.size() - 1, request_of(jobs.back()));
+        }
+    };
+    feed.done = [&](int id, const cf_result& r0) {
+        cf_result r = r0;
+        r.t_submit = jobs[id].t_submit;
+        r.t_start = t_in[id];
+        r.t_end = now_s();
+        out[id] = 1;
+        publish(S, r, 0);
+    };
+    const int st = cf::solve_batch_feed(ctx, cap, in, feed);
+    const double t1 = now_s();
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        if (out[i]) continue;  // published when it finished
+        cf_result r;
+        std::memset(&r, 0, sizeof(r));
+        r.tag = jobs[i].req.tag;
+        r.status = st != CF_OK ? st : CF_ERR_STATE;
+        std::snprintf(r.error, sizeof(r.error), "%s", st != CF_OK ? cf_last_error() : "batch ended without a result");
+        r.t_submit = jobs[i].t_submit;
+        r.t_start = t_in[i];
+        r.t_end = t1;
+        publish(S, r, 0);
+    }
+    release(S, held);
 }
 
 void worker_main(cf_server* S) {
@@ -229,10 +294,14 @@ void worker_main(cf_server* S) {
                             }
                         }
                     } else if (batchable(job.req)) {
-                        // with it, every other queued small DOCI request that fits
+                        // with it, every other queued small DOCI request of the same
+                        // shape that fits (other shapes go to other workers, so a
+                        // batch of small spaces does not wait for larger ones)
+                        const int L0 = job.req.L, np0 = job.req.nup;
                         batch.push_back(std::move(job));
                         for (auto jt = S->pending.begin(); jt != S->pending.end() && batch.size() < kBatchMax;) {
-                            if (batchable(jt->req) && S->in_use + jt->footprint <= S->budget) {
+                            if (batchable(jt->req) && jt->req.L == L0 && jt->req.nup == np0 &&
+                                S->in_use + jt->footprint <= S->budget) {
                                 S->in_use += jt->footprint;
                                 S->running++;
                                 batch.push_back(std::move(*jt));

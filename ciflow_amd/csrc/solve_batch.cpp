@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <memory>
@@ -60,14 +61,15 @@ SigmaArgs fci_shape(const cf_request& r) {
 }
 
 struct Lane {
-    int req = -1;
+    int id = -1;  // the caller's id of the request in this slot; -1: free
+    cf_request req{};
     std::unique_ptr<cf_ham> ham;
     std::unique_ptr<cf_space> sp;
     DBuf<double> Ha, Hb, vpair64, vself64, vk64;
     DBuf<float> vpair32, vself32, vk32;
     SigmaArgs args;
     // Davidson state (the fields of run_davidson / Davidson::run)
-    int phase = 1;  // 0 FP32 operator, 1 FP64 operator, 2 finished
+    int phase = 2;  // 0 FP32 operator, 1 FP64 operator, 2 finished
     bool start = true;
     int start_mode = BD_START_SEED;
     int m = 0, it = 0, krylov = 8, stall = 200;
@@ -87,13 +89,15 @@ struct Lane {
 
 struct Batch {
     cf_ctx* c;
-    const cf_request* reqs;
-    int kind, L, nup, ndown, M, keep, B;
+    const BatchFeed* feed;
+    int kind, L, nup, ndown, M, keep, cap;
     uint64_t n, da, db;
     double tol_f32;
     uint64_t seed;
     int maxit;
-    std::vector<Lane> lanes;
+    std::vector<Lane> lanes;  // cap slots
+    std::vector<BdOpLane> hop;
+    std::vector<SigmaArgs> hargs;
     DBuf<double> V, S, x, hx, t, partial, parts, dg, Y;
     DBuf<float> x32;
     DBuf<BdStep> dstep;
@@ -107,175 +111,41 @@ struct Batch {
     double* hr1 = nullptr;
     double* hY = nullptr;
 
-    void setup();
-    void iterate();
+    void alloc();
+    void add(const std::vector<std::pair<int, cf_request>>& in);
+    void run();
+    void finish();
     void end_phase(Lane& q, int code);
     void column(Lane& q, const double* dg);
     void product(bool any32, bool any64);
 };
 
-void Batch::setup() {
+// The batch's vectors for `cap` slots (a slot's vectors at a fixed stride).
+void Batch::alloc() {
     hipStream_t st = c->stream;
-    const size_t L2 = (size_t)L * L, L3 = L2 * L, L4 = L3 * L;
-    // rotated integrals of every lane in one batched FP64-MFMA transform
-    {
-        std::vector<double> hh(B * L2), vv(B * L4), uu(B * L2, 0.0);
-        for (int b = 0; b < B; ++b) {
-            const cf_request& r = reqs[lanes[b].req];
-            std::memcpy(&hh[b * L2], r.h, L2 * sizeof(double));
-            std::memcpy(&vv[b * L4], r.V, L4 * sizeof(double));
-            if (r.U) std::memcpy(&uu[b * L2], r.U, L2 * sizeof(double));
-            else
-                for (int a = 0; a < L; ++a) uu[b * L2 + a * L + a] = 1.0;
-        }
-        DBuf<double> dh(B * L2), dV(B * L4), dU(B * L2), oh(B * L2), oV(B * L4), tmp(B * L4);
-        CF_HIP(hipMemcpyAsync(dh.p, hh.data(), dh.bytes(), hipMemcpyHostToDevice, st));
-        CF_HIP(hipMemcpyAsync(dV.p, vv.data(), dV.bytes(), hipMemcpyHostToDevice, st));
-        CF_HIP(hipMemcpyAsync(dU.p, uu.data(), dU.bytes(), hipMemcpyHostToDevice, st));
-        rotate_integrals_dev(L, B, dh.p, dV.p, dU.p, oh.p, oV.p, tmp.p, st);
-        for (int b = 0; b < B; ++b) {
-            Lane& q = lanes[b];
-            const cf_request& r = reqs[q.req];
-            q.ham.reset(new cf_ham());
-            cf_ham* H = q.ham.get();
-            H->ctx = c;
-            H->L = L;
-            H->nup = nup;
-            H->ndown = ndown;
-            H->econst = r.econst;
-            H->dh.alloc(L2);
-            H->dV.alloc(L4);
-            H->dW.alloc(L3);
-            H->dJm.alloc(L2);
-            CF_HIP(hipMemcpyAsync(H->dh.p, oh.p + b * L2, L2 * sizeof(double), hipMemcpyDeviceToDevice, st));
-            CF_HIP(hipMemcpyAsync(H->dV.p, oV.p + b * L4, L4 * sizeof(double), hipMemcpyDeviceToDevice, st));
-            launch_derived(L, H->dV.p, H->dW.p, H->dJm.p, st);
-        }
-        CF_HIP(hipStreamSynchronize(st));  // the host staging arrays go out of scope
-    }
-    bool want32 = false;
-    for (const Lane& q : lanes) want32 |= reqs[q.req].precision == CF_F32;
+    lanes.resize(cap);
+    hop.assign(cap, BdOpLane{});
+    hargs.assign(cap, SigmaArgs{});
     if (kind == CF_DOCI) {
         auto key = std::make_pair(L, nup);
         if (!c->doci_blk.count(key)) make_doci_block_tables(c, L, nup, doci_block_w(L), c->doci_blk[key]);
         blk = &c->doci_blk[key];
         blk->args.zero = kDociZero;
     }
-    // per lane: strings with their H-dependent tables, diagonal, operator tables
-    std::vector<BdOpLane> ops(B);
-    std::vector<SigmaArgs> args(B);
-    for (int b = 0; b < B; ++b) {
-        Lane& q = lanes[b];
-        const bool f32 = reqs[q.req].precision == CF_F32;
-        q.sp.reset(new cf_space());
-        cf_space* sp = q.sp.get();
-        sp->ham = q.ham.get();
-        sp->kind = kind;
-        if (kind == CF_DOCI) {
-            sp->ta.reset(make_strtab(c, q.ham.get(), nup, false));
-            sp->tb = sp->ta.get();
-            sp->da = sp->db = sp->ta->d;
-        } else {
-            sp->ta.reset(make_strtab(c, q.ham.get(), nup, true));
-            if (ndown == nup) {
-                sp->tb = sp->ta.get();
-            } else {
-                sp->tb_own.reset(make_strtab(c, q.ham.get(), ndown, true));
-                sp->tb = sp->tb_own.get();
-            }
-            sp->da = sp->ta->d;
-            sp->db = sp->tb->d;
-        }
-        sp->n = n;
-        sp->diag.alloc(n);
-        fill_space_diag(sp);
-        ops[b].diag = sp->diag.p;
-        if (kind == CF_DOCI) {
-            DociEllArgs ea;
-            ea.n = n;
-            ea.L = L;
-            ea.np = nup;
-            ea.nv = L - nup;
-            ea.s = sp->ta->s.p;
-            ea.diag = sp->diag.p;
-            ea.zero = kDociZero;
-            q.vk64.alloc(L2);
-            q.vk32.alloc(L2);
-            launch_doci_pair_table(ea, q.ham->dev(), q.vk64.p, q.vk32.p, st);
-            ops[b].vk64 = q.vk64.p;
-            ops[b].vk32 = q.vk32.p;
-        } else {
-            q.Ha.alloc(da * da);
-            launch_dense_string_ham<double>(da, sp->ta->E, sp->ta->full.p, q.Ha.p, st);
-            ops[b].Ha = ops[b].Hb = q.Ha.p;
-            if (sp->tb != sp->ta.get()) {
-                q.Hb.alloc(db * db);
-                launch_dense_string_ham<double>(db, sp->tb->E, sp->tb->full.p, q.Hb.p, st);
-                ops[b].Hb = q.Hb.p;
-            }
-            SigmaArgs& a = q.args;
-            a.da = da;
-            a.db = db;
-            a.L = L;
-            a.Sa = sp->ta->S;
-            a.Sb = sp->tb->S;
-            a.sa = sp->ta->s.p;
-            a.sb = sp->tb->s.p;
-            a.suba = sp->ta->sub.p;
-            a.subb = sp->tb->sub.p;
-            a.V = q.ham->dV.p;
-            q.vpair64.alloc(L4);
-            q.vself64.alloc(da * L2);
-            launch_sigma_tables<double>(a, q.vpair64.p, q.vself64.p, st);
-            a.vpair64 = q.vpair64.p;
-            a.vself64 = q.vself64.p;
-            if (f32) {
-                q.vpair32.alloc(L4);
-                q.vself32.alloc(da * L2);
-                launch_sigma_tables<float>(a, q.vpair32.p, q.vself32.p, st);
-                a.vpair32 = q.vpair32.p;
-                a.vself32 = q.vself32.p;
-            }
-            CF_REQUIRE(sigma_ab3_batch_args(a, ndown, false), "batched sigma: no kernel for this shape");
-            args[b] = a;
-        }
-        // solver state
-        const cf_request& r = reqs[q.req];
-        q.G.assign((size_t)M * M, 0.0);
-        q.tol_energy = r.tol_energy > 0.0 ? r.tol_energy : 0.0;
-        q.best = std::numeric_limits<double>::infinity();
-        if (f32) {
-            q.phase = 0;
-            q.tol = tol_f32;
-            q.stall = 24;
-        } else {
-            q.phase = 1;
-            q.tol = r.tol > 0 ? r.tol : 1e-12;
-            q.stall = 200;
-        }
-        q.krylov = 8;
-        q.start = true;
-        q.start_mode = BD_START_SEED;
-    }
-    // the batch's vectors
-    const size_t Bn = (size_t)B * n;
+    const size_t Bn = (size_t)cap * n;
     V.alloc(Bn * M);
     S.alloc(Bn * M);
     x.alloc(Bn);
     hx.alloc(Bn);
     t.alloc(Bn);
-    if (kind == CF_FCI && want32) x32.alloc(Bn);
-    partial.alloc((size_t)B * kBdPartial);
-    parts.alloc((size_t)B * kBdParts);
-    dg.alloc((size_t)B * 64);
-    Y.alloc((size_t)B * kDavMaxBasis * kDavMaxBasis);
-    dstep.alloc(B);
-    dop.alloc(B);
-    CF_HIP(hipMemcpyAsync(dop.p, ops.data(), B * sizeof(BdOpLane), hipMemcpyHostToDevice, st));
-    if (kind == CF_FCI) {
-        dargs.alloc(B);
-        CF_HIP(hipMemcpyAsync(dargs.p, args.data(), B * sizeof(SigmaArgs), hipMemcpyHostToDevice, st));
-    }
+    if (kind == CF_FCI) x32.alloc(Bn);
+    partial.alloc((size_t)cap * kBdPartial);
+    parts.alloc((size_t)cap * kBdParts);
+    dg.alloc((size_t)cap * 64);
+    Y.alloc((size_t)cap * kDavMaxBasis * kDavMaxBasis);
+    dstep.alloc(cap);
+    dop.alloc(cap);
+    if (kind == CF_FCI) dargs.alloc(cap);
     view.n = n;
     view.M = M;
     view.Ga = fused_axpy_blocks(n);
@@ -293,17 +163,162 @@ void Batch::setup() {
     view.step = dstep.p;
     view.op = dop.p;
     // staging: steps, dg rows, second-pair residual parts, restart rotations
-    const size_t bytes = B * (sizeof(BdStep) + 64 * sizeof(double) + 256 * sizeof(double) +
-                              kDavMaxBasis * kDavMaxBasis * sizeof(double));
+    const size_t bytes = cap * (sizeof(BdStep) + 64 * sizeof(double) + 256 * sizeof(double) +
+                                kDavMaxBasis * kDavMaxBasis * sizeof(double));
     char* p = (char*)c->batch_pinned(bytes);
     hstep = (BdStep*)p;
-    p += B * sizeof(BdStep);
+    p += cap * sizeof(BdStep);
     hdg = (double*)p;
-    p += B * 64 * sizeof(double);
+    p += cap * 64 * sizeof(double);
     hr1 = (double*)p;
-    p += B * 256 * sizeof(double);
+    p += cap * 256 * sizeof(double);
     hY = (double*)p;
-    CF_HIP(hipStreamSynchronize(st));  // the host operator arrays go out of scope
+    (void)st;
+}
+
+// Requests into free slots: rotated integrals (one batched FP64-MFMA
+// transform), strings with their H-dependent tables, diagonal, operator
+// tables, and a fresh solver state.
+void Batch::add(const std::vector<std::pair<int, cf_request>>& in) {
+    if (in.empty()) return;
+    hipStream_t st = c->stream;
+    const size_t L2 = (size_t)L * L, L3 = L2 * L, L4 = L3 * L;
+    std::vector<int> slot;
+    for (int b = 0; b < cap && slot.size() < in.size(); ++b)
+        if (lanes[b].id < 0) slot.push_back(b);
+    CF_REQUIRE(slot.size() == in.size(), "batched Davidson: more requests than free slots");
+    const int k = (int)in.size();
+    std::vector<double> hh(k * L2), vv(k * L4), uu(k * L2, 0.0);
+    for (int i = 0; i < k; ++i) {
+        const cf_request& r = in[i].second;
+        std::memcpy(&hh[i * L2], r.h, L2 * sizeof(double));
+        std::memcpy(&vv[i * L4], r.V, L4 * sizeof(double));
+        if (r.U) std::memcpy(&uu[i * L2], r.U, L2 * sizeof(double));
+        else
+            for (int a = 0; a < L; ++a) uu[i * L2 + a * L + a] = 1.0;
+    }
+    DBuf<double> dh(k * L2), dV(k * L4), dU(k * L2), oh(k * L2), oV(k * L4), tmp(k * L4);
+    CF_HIP(hipMemcpyAsync(dh.p, hh.data(), dh.bytes(), hipMemcpyHostToDevice, st));
+    CF_HIP(hipMemcpyAsync(dV.p, vv.data(), dV.bytes(), hipMemcpyHostToDevice, st));
+    CF_HIP(hipMemcpyAsync(dU.p, uu.data(), dU.bytes(), hipMemcpyHostToDevice, st));
+    rotate_integrals_dev(L, k, dh.p, dV.p, dU.p, oh.p, oV.p, tmp.p, st);
+    for (int i = 0; i < k; ++i) {
+        const int b = slot[i];
+        Lane& q = lanes[b];
+        q = Lane();
+        q.id = in[i].first;
+        q.req = in[i].second;
+        const cf_request& r = q.req;
+        const bool f32 = r.precision == CF_F32;
+        q.ham.reset(new cf_ham());
+        cf_ham* H = q.ham.get();
+        H->ctx = c;
+        H->L = L;
+        H->nup = nup;
+        H->ndown = ndown;
+        H->econst = r.econst;
+        H->dh.alloc(L2);
+        H->dV.alloc(L4);
+        H->dW.alloc(L3);
+        H->dJm.alloc(L2);
+        CF_HIP(hipMemcpyAsync(H->dh.p, oh.p + i * L2, L2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        CF_HIP(hipMemcpyAsync(H->dV.p, oV.p + i * L4, L4 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        launch_derived(L, H->dV.p, H->dW.p, H->dJm.p, st);
+        q.sp.reset(new cf_space());
+        cf_space* sp = q.sp.get();
+        sp->ham = H;
+        sp->kind = kind;
+        if (kind == CF_DOCI) {
+            sp->ta.reset(make_strtab(c, H, nup, false));
+            sp->tb = sp->ta.get();
+            sp->da = sp->db = sp->ta->d;
+        } else {
+            sp->ta.reset(make_strtab(c, H, nup, true));
+            if (ndown == nup) {
+                sp->tb = sp->ta.get();
+            } else {
+                sp->tb_own.reset(make_strtab(c, H, ndown, true));
+                sp->tb = sp->tb_own.get();
+            }
+            sp->da = sp->ta->d;
+            sp->db = sp->tb->d;
+        }
+        sp->n = n;
+        sp->diag.alloc(n);
+        fill_space_diag(sp);
+        BdOpLane& o = hop[b];
+        o = BdOpLane{};
+        o.diag = sp->diag.p;
+        if (kind == CF_DOCI) {
+            DociEllArgs ea;
+            ea.n = n;
+            ea.L = L;
+            ea.np = nup;
+            ea.nv = L - nup;
+            ea.s = sp->ta->s.p;
+            ea.diag = sp->diag.p;
+            ea.zero = kDociZero;
+            q.vk64.alloc(L2);
+            q.vk32.alloc(L2);
+            launch_doci_pair_table(ea, H->dev(), q.vk64.p, q.vk32.p, st);
+            o.vk64 = q.vk64.p;
+            o.vk32 = q.vk32.p;
+        } else {
+            q.Ha.alloc(da * da);
+            launch_dense_string_ham<double>(da, sp->ta->E, sp->ta->full.p, q.Ha.p, st);
+            o.Ha = o.Hb = q.Ha.p;
+            if (sp->tb != sp->ta.get()) {
+                q.Hb.alloc(db * db);
+                launch_dense_string_ham<double>(db, sp->tb->E, sp->tb->full.p, q.Hb.p, st);
+                o.Hb = q.Hb.p;
+            }
+            SigmaArgs& a = q.args;
+            a.da = da;
+            a.db = db;
+            a.L = L;
+            a.Sa = sp->ta->S;
+            a.Sb = sp->tb->S;
+            a.sa = sp->ta->s.p;
+            a.sb = sp->tb->s.p;
+            a.suba = sp->ta->sub.p;
+            a.subb = sp->tb->sub.p;
+            a.V = H->dV.p;
+            q.vpair64.alloc(L4);
+            q.vself64.alloc(da * L2);
+            launch_sigma_tables<double>(a, q.vpair64.p, q.vself64.p, st);
+            a.vpair64 = q.vpair64.p;
+            a.vself64 = q.vself64.p;
+            if (f32) {
+                q.vpair32.alloc(L4);
+                q.vself32.alloc(da * L2);
+                launch_sigma_tables<float>(a, q.vpair32.p, q.vself32.p, st);
+                a.vpair32 = q.vpair32.p;
+                a.vself32 = q.vself32.p;
+            }
+            CF_REQUIRE(sigma_ab3_batch_args(a, ndown, false), "batched sigma: no kernel for this shape");
+            hargs[b] = a;
+        }
+        // solver state: the FP32 phase first for FP32 requests
+        q.G.assign((size_t)M * M, 0.0);
+        q.tol_energy = r.tol_energy > 0.0 ? r.tol_energy : 0.0;
+        q.best = std::numeric_limits<double>::infinity();
+        if (f32) {
+            q.phase = 0;
+            q.tol = tol_f32;
+            q.stall = 24;
+        } else {
+            q.phase = 1;
+            q.tol = r.tol > 0 ? r.tol : 1e-12;
+            q.stall = 200;
+        }
+        q.krylov = 8;
+        q.start = true;
+        q.start_mode = BD_START_SEED;
+    }
+    CF_HIP(hipMemcpyAsync(dop.p, hop.data(), cap * sizeof(BdOpLane), hipMemcpyHostToDevice, st));
+    if (kind == CF_FCI)
+        CF_HIP(hipMemcpyAsync(dargs.p, hargs.data(), cap * sizeof(SigmaArgs), hipMemcpyHostToDevice, st));
+    CF_HIP(hipStreamSynchronize(st));  // the host staging arrays go out of scope
 }
 
 // S_m = H V_m for every active lane: FCI same-spin GEMM then the alpha-beta
@@ -311,7 +326,7 @@ void Batch::setup() {
 void Batch::product(bool any32, bool any64) {
     hipStream_t st = c->stream;
     if (kind == CF_FCI) {
-        bd_same_spin(view, B, da, db, st);
+        bd_same_spin(view, cap, da, db, st);
         SigmaBatch sb;
         sb.args = dargs.p;
         sb.step = dstep.p;
@@ -320,9 +335,11 @@ void Batch::product(bool any32, bool any64) {
         sb.x32 = x32.p;
         sb.n = n;
         sb.M = M;
-        const SigmaArgs& shape = lanes[0].args;
-        if (any32) launch_sigma_ab3_batch<float>(shape, ndown, sb, B, st);
-        if (any64) launch_sigma_ab3_batch<double>(shape, ndown, sb, B, st);
+        const SigmaArgs* shape = nullptr;
+        for (const Lane& q : lanes)
+            if (q.id >= 0) shape = &q.args;
+        if (any32) launch_sigma_ab3_batch<float>(*shape, ndown, sb, cap, st);
+        if (any64) launch_sigma_ab3_batch<double>(*shape, ndown, sb, cap, st);
     } else {
         DociBatch bt;
         bt.step = dstep.p;
@@ -331,14 +348,14 @@ void Batch::product(bool any32, bool any64) {
         bt.S = S.p;
         bt.n = n;
         bt.M = M;
-        if (any32) launch_spmv_doci_blk_batch(blk->args, true, bt, B, st);
-        if (any64) launch_spmv_doci_blk_batch(blk->args, false, bt, B, st);
+        if (any32) launch_spmv_doci_blk_batch(blk->args, true, bt, cap, st);
+        if (any64) launch_spmv_doci_blk_batch(blk->args, false, bt, cap, st);
     }
 }
 
 void Batch::end_phase(Lane& q, int code) {
     if (q.phase == 0) {  // FP32 phase over: FP64 polish from its Ritz vector
-        const cf_request& r = reqs[q.req];
+        const cf_request& r = q.req;
         q.nmv32 = q.napply;
         q.napply = 0;
         q.phase = 1;
@@ -376,16 +393,52 @@ void Batch::column(Lane& q, const double* d) {
     ++q.it;
 }
 
-void Batch::iterate() {
+// Results of the lanes that finished: handed to the caller, slots freed.
+void Batch::finish() {
+    for (Lane& q : lanes) {
+        if (q.id < 0 || q.phase != 2) continue;
+        cf_result o;
+        std::memset(&o, 0, sizeof(o));
+        o.tag = q.req.tag;
+        o.n = n;
+        if (q.failed) {
+            o.status = CF_ERR_NOCONV;
+            std::snprintf(o.error, sizeof(o.error), "%s", q.error.c_str());
+        } else {
+            o.status = CF_OK;
+            o.eigval = q.theta_out;
+            o.energy = q.theta_out + q.req.econst;
+            o.resid = q.res_out;
+            o.n_matvec = q.napply;
+            o.n_matvec_f32 = q.nmv32;
+            o.solve_status = q.code == 0 ? 0 : 1;
+        }
+        const int id = q.id;
+        q = Lane();  // releases the slot's tables (ordered on the stream)
+        if (feed->done) feed->done(id, o);
+    }
+}
+
+void Batch::run() {
     hipStream_t st = c->stream;
     std::vector<double> A, th, Yv;
+    std::vector<std::pair<int, cf_request>> more;
     for (;;) {
+        if (feed->more) {  // free slots take queued requests of the same shape
+            int nfree = 0;
+            for (const Lane& q : lanes) nfree += q.id < 0;
+            if (nfree) {
+                more.clear();
+                feed->more(nfree, more);
+                add(more);
+            }
+        }
         bool any = false, any_restart = false, any32 = false, any64 = false;
-        for (int b = 0; b < B; ++b) {
+        for (int b = 0; b < cap; ++b) {
             Lane& q = lanes[b];
             BdStep& s = hstep[b];
             std::memset(&s, 0, sizeof(s));
-            if (q.phase == 2) continue;
+            if (q.id < 0 || q.phase == 2) continue;
             any = true;
             s.f32 = q.phase == 0 ? 1 : 0;
             any32 |= s.f32 != 0;
@@ -432,17 +485,17 @@ void Batch::iterate() {
             q.th1 = q.have_th1 ? th[1] : 0.0;
         }
         if (!any) return;
-        CF_HIP(hipMemcpyAsync(dstep.p, hstep, B * sizeof(BdStep), hipMemcpyHostToDevice, st));
+        CF_HIP(hipMemcpyAsync(dstep.p, hstep, cap * sizeof(BdStep), hipMemcpyHostToDevice, st));
         if (any_restart)
-            CF_HIP(hipMemcpyAsync(Y.p, hY, (size_t)B * kDavMaxBasis * kDavMaxBasis * sizeof(double),
+            CF_HIP(hipMemcpyAsync(Y.p, hY, (size_t)cap * kDavMaxBasis * kDavMaxBasis * sizeof(double),
                                   hipMemcpyHostToDevice, st));
-        bd_form_vectors(view, B, any_restart, st);
+        bd_form_vectors(view, cap, any_restart, st);
         product(any32, any64);
-        bd_new_column(view, B, st);
-        CF_HIP(hipMemcpyAsync(hdg, dg.p, (size_t)B * 64 * sizeof(double), hipMemcpyDeviceToHost, st));
+        bd_new_column(view, cap, st);
+        CF_HIP(hipMemcpyAsync(hdg, dg.p, (size_t)cap * 64 * sizeof(double), hipMemcpyDeviceToHost, st));
         CF_HIP(hipStreamSynchronize(st));
         std::vector<int> temple;
-        for (int b = 0; b < B; ++b) {
+        for (int b = 0; b < cap; ++b) {
             Lane& q = lanes[b];
             if (hstep[b].mode == BD_IDLE) continue;
             const double* d = hdg + (size_t)b * 64;
@@ -476,10 +529,10 @@ void Batch::iterate() {
         }
         if (!temple.empty()) {
             for (int b : temple) hstep[b].r2 = 1;
-            CF_HIP(hipMemcpyAsync(dstep.p, hstep, B * sizeof(BdStep), hipMemcpyHostToDevice, st));
-            bd_resid2(view, B, st);
+            CF_HIP(hipMemcpyAsync(dstep.p, hstep, cap * sizeof(BdStep), hipMemcpyHostToDevice, st));
+            bd_resid2(view, cap, st);
             CF_HIP(hipMemcpy2DAsync(hr1, 256 * sizeof(double), parts.p + 1024, kBdParts * sizeof(double),
-                                    view.Ga * sizeof(double), B, hipMemcpyDeviceToHost, st));
+                                    view.Ga * sizeof(double), cap, hipMemcpyDeviceToHost, st));
             CF_HIP(hipStreamSynchronize(st));
             for (int b : temple) {
                 Lane& q = lanes[b];
@@ -490,6 +543,7 @@ void Batch::iterate() {
                 else column(q, hdg + (size_t)b * 64);
             }
         }
+        finish();
     }
 }
 
@@ -534,15 +588,59 @@ uint64_t bd_class_key(const cf_request& r) {
 
 }  // namespace cf
 
+namespace cf {
+
+int solve_batch_feed(cf_ctx* ctx, int capacity, const std::vector<std::pair<int, cf_request>>& first,
+                     const BatchFeed& feed) {
+    try {
+        CF_REQUIRE(ctx && !first.empty(), "batched Davidson: no requests");
+        CF_REQUIRE(capacity >= (int)first.size() && capacity <= kBdMaxLanes, "batched Davidson: 1..32 lanes");
+        ctx->use();
+        cf_solve_opts dflt;
+        cf_solve_opts_default(&dflt);
+        const cf_request& r0 = first[0].second;
+        const uint64_t key = bd_class_key(r0);
+        for (const auto& pr : first) {
+            std::string why;
+            CF_REQUIRE(bd_accepts(pr.second, &why), why);
+            CF_REQUIRE(bd_class_key(pr.second) == key, "batched Davidson: requests of different shapes");
+        }
+        Batch bt;
+        bt.c = ctx;
+        bt.feed = &feed;
+        bt.kind = r0.kind;
+        bt.L = r0.L;
+        bt.nup = r0.nup;
+        bt.ndown = r0.ndown;
+        bt.n = space_dim(r0);
+        bt.da = binom_host(r0.L, r0.nup);
+        bt.db = r0.kind == CF_DOCI ? bt.da : binom_host(r0.L, r0.ndown);
+        bt.M = basis_size(r0, bt.n);
+        bt.keep = std::max(1, std::min(4, bt.M - 1));
+        bt.cap = capacity;
+        bt.tol_f32 = dflt.tol_f32;
+        bt.seed = dflt.seed;
+        bt.maxit = 20000;
+        bt.alloc();
+        bt.add(first);
+        bt.run();
+    } catch (const cf::Error& e) {
+        return cf::set_last_error(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return cf::set_last_error(CF_ERR_NOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return cf::set_last_error(CF_ERR_HIP, e.what());
+    }
+    return CF_OK;
+}
+
+}  // namespace cf
+
 extern "C" int cf_solve_batch(cf_ctx* ctx, int count, const cf_request* reqs, cf_result* results) {
     try {
         CF_REQUIRE(ctx && (count == 0 || (reqs && results)), "NULL argument");
         CF_REQUIRE(count >= 0, "negative count");
-        if (!count) return CF_OK;
-        ctx->use();
-        cf_solve_opts dflt;
-        cf_solve_opts_default(&dflt);
-        // requests by class, in request order
+        // requests by shape, in request order
         std::map<uint64_t, std::vector<int>> groups;
         std::vector<uint64_t> order;
         for (int i = 0; i < count; ++i) {
@@ -561,46 +659,16 @@ extern "C" int cf_solve_batch(cf_ctx* ctx, int count, const cf_request* reqs, cf
             if (!groups.count(key)) order.push_back(key);
             groups[key].push_back(i);
         }
+        BatchFeed feed;
+        feed.done = [results](int id, const cf_result& r) { results[id] = r; };
         for (uint64_t key : order) {
             const std::vector<int>& idx = groups[key];
             for (size_t first = 0; first < idx.size(); first += kBdMaxLanes) {
                 const size_t last = std::min(idx.size(), first + (size_t)kBdMaxLanes);
-                const cf_request& r0 = reqs[idx[first]];
-                Batch bt;
-                bt.c = ctx;
-                bt.reqs = reqs;
-                bt.kind = r0.kind;
-                bt.L = r0.L;
-                bt.nup = r0.nup;
-                bt.ndown = r0.ndown;
-                bt.n = space_dim(r0);
-                bt.da = binom_host(r0.L, r0.nup);
-                bt.db = r0.kind == CF_DOCI ? bt.da : binom_host(r0.L, r0.ndown);
-                bt.M = basis_size(r0, bt.n);
-                bt.keep = std::max(1, std::min(4, bt.M - 1));
-                bt.B = (int)(last - first);
-                bt.tol_f32 = dflt.tol_f32;
-                bt.seed = dflt.seed;
-                bt.maxit = 20000;
-                bt.lanes.resize(bt.B);
-                for (int b = 0; b < bt.B; ++b) bt.lanes[b].req = idx[first + b];
-                bt.setup();
-                bt.iterate();
-                for (const Lane& q : bt.lanes) {
-                    cf_result& o = results[q.req];
-                    if (q.failed) {
-                        o.status = CF_ERR_NOCONV;
-                        std::snprintf(o.error, sizeof(o.error), "%s", q.error.c_str());
-                        continue;
-                    }
-                    o.status = CF_OK;
-                    o.eigval = q.theta_out;
-                    o.energy = q.theta_out + reqs[q.req].econst;
-                    o.resid = q.res_out;
-                    o.n_matvec = q.napply;
-                    o.n_matvec_f32 = q.nmv32;
-                    o.solve_status = q.code == 0 ? 0 : 1;
-                }
+                std::vector<std::pair<int, cf_request>> in;
+                for (size_t j = first; j < last; ++j) in.emplace_back(idx[j], reqs[idx[j]]);
+                const int st = solve_batch_feed(ctx, (int)in.size(), in, feed);
+                if (st != CF_OK) return st;
             }
         }
     } catch (const cf::Error& e) {

@@ -129,3 +129,23 @@ def test_rejected_requests_leave_the_rest():
     assert res[0]["status"] != 0 and "Davidson" in res[0]["error"]
     assert res[1]["status"] != 0 and "nalpha == nbeta" in res[1]["error"]
     assert res[2]["status"] == 0 and abs(res[2]["energy"] - oracle_energy(good)) <= 1e-10
+
+
+def test_server_refills_a_running_batch():
+    """Requests of one shape that arrive while their shape is being served
+    join the running batch (one worker slot, so overlapping service means a
+    shared batch): every result equals the request solved alone, bitwise."""
+    from ciflow_amd import stream as S
+    reqs = [S.StreamRequest(k, CA.CF_FCI, 8, 4, 5000 + k, 0.004 * k) for k in range(10)]
+    arrs = S.prepare(reqs)
+    served = dict(precision=CA.CF_F32, check_every=2, tol_energy=1e-9, direct_fci=True, method=CA.CF_SOLVER_AUTO)
+    with CA.SolveServer(slots=1) as srv:
+        out = S.run_stream(reqs, arrs, server=srv, timeout=120, **served)["results"]
+    for r, (h, V, U), d in zip(reqs, arrs, out):
+        alone = CA.solve_batch([dict(kind=CA.CF_FCI, h=h, V=V, nup=4, ndown=4, U=U, precision=CA.CF_F32,
+                                     tol_energy=1e-9, tag=r.k)])[0]
+        say(r.k, d["t_start"] - out[0]["t_start"], d["t_end"] - out[0]["t_start"], d["energy"], alone["energy"])
+        assert d["status"] == 0 and alone["energy"] == d["energy"], r.k
+        assert (alone["n_matvec"], alone["n_matvec_f32"]) == (d["n_matvec"], d["n_matvec_f32"]), r.k
+    joined = [b for b in out if any(a["t_start"] < b["t_start"] < a["t_end"] for a in out)]
+    assert joined, "no request joined a running batch"
