@@ -10,7 +10,7 @@
 // decomposition, and so every reduction order, depends on n alone, so a
 // lane's results are the same in any batch and alone.  The arithmetic of
 // each step is that of the single-request Davidson step kernels (krylov.hip):
-// x = V y and H x = S y in one pass, r = S y - theta x, the correction
+// x = V y and S y in one pass, r = S y - theta x, the correction
 // t = r / (theta - diag) (|theta - diag| >= shift), two selective Gram-Schmidt
 // passes, v_m = t / |t|, then after the product the m + 1 dots V^T S_m.
 #include "kernels.h"
@@ -26,6 +26,48 @@ constexpr int kChunk = 8;  // basis vectors per dot-product block row
 
 __device__ __forceinline__ double* lane_parts(const BdView& b, int lane) {
     return b.parts + (size_t)lane * kBdParts;
+}
+
+// a = sum_i y[i] V[i n + q], c = sum_i y[i] S[i n + q] over i < k, summed in
+// order i = 0, 1, ...; the basis size k varies per lane at run time, so the
+// loads are issued eight at a time before they are used (one at a time, the
+// pass ran at a third of the HBM rate).
+__device__ __forceinline__ void combine2(const double* __restrict__ V, const double* __restrict__ S,
+                                         const double* y, int k, uint64_t n, uint64_t q, double& a, double& c) {
+    int i = 0;
+    for (; i + 8 <= k; i += 8) {
+        double v[8], w[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            v[j] = V[(uint64_t)(i + j) * n + q];
+            w[j] = S[(uint64_t)(i + j) * n + q];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            a += y[i + j] * v[j];
+            c += y[i + j] * w[j];
+        }
+    }
+    for (; i < k; ++i) {
+        a += y[i] * V[(uint64_t)i * n + q];
+        c += y[i] * S[(uint64_t)i * n + q];
+    }
+}
+
+// s = sum_i c[i] V[i n + q] over i < k, in order, loads eight at a time.
+__device__ __forceinline__ double combine1(const double* __restrict__ V, const double* c, int k, uint64_t n,
+                                           uint64_t q) {
+    double s = 0.0;
+    int i = 0;
+    for (; i + 8 <= k; i += 8) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = V[(uint64_t)(i + j) * n + q];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) s += c[i + j] * v[j];
+    }
+    for (; i < k; ++i) s += c[i] * V[(uint64_t)i * n + q];
+    return s;
 }
 
 // Restart: V[:, 0:keep] = V[:, 0:mr] Y[:, 0:keep] in place on FP64 MFMA
@@ -75,8 +117,8 @@ __global__ void __launch_bounds__(256) k_bd_rotate(BdView b) {
     }
 }
 
-// BD_RITZ: x = V y, hx = S y, r = hx - theta x, t = r / (theta - diag) (t = r
-// when shift is 0), parts of |r|^2 and |t|^2.  BD_START_X / BD_START_SEED: t
+// BD_RITZ: x = V y, r = S y - theta x, t = r / (theta - diag) (t = r when
+// shift is 0), parts of |r|^2 and |t|^2.  BD_START_X / BD_START_SEED: t
 // = x or the seeded start vector, parts of |t|^2 (|r|^2 parts zero).
 __global__ void __launch_bounds__(256) k_bd_combine(BdView b) {
     const int lane = blockIdx.y;
@@ -94,7 +136,6 @@ __global__ void __launch_bounds__(256) k_bd_combine(BdView b) {
     const double* V = b.V + (uint64_t)lane * b.M * n;
     const double* S = b.S + (uint64_t)lane * b.M * n;
     double* x = b.x + (uint64_t)lane * n;
-    double* hx = b.hx + (uint64_t)lane * n;
     double* t = b.t + (uint64_t)lane * n;
     const double* diag = b.op[lane].diag;
     double acc = 0.0, acct = 0.0;
@@ -103,12 +144,8 @@ __global__ void __launch_bounds__(256) k_bd_combine(BdView b) {
         double tq;
         if (mode == BD_RITZ) {
             double a = 0.0, c = 0.0;
-            for (int i = 0; i < k; ++i) {
-                a += ys[i] * V[(uint64_t)i * n + q];
-                c += ys[i] * S[(uint64_t)i * n + q];
-            }
+            combine2(V, S, ys, k, n, q, a, c);
             x[q] = a;
-            hx[q] = c;
             const double r = c - theta * a;
             acc += r * r;
             tq = r;
@@ -198,8 +235,7 @@ __global__ void __launch_bounds__(256) k_bd_axpy(BdView b, int pass, double eta2
     double nrm = 0.0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
-        double s = 0.0;
-        for (int i = 0; i < k; ++i) s += cs[i] * Vb[(uint64_t)i * n + q];
+        const double s = combine1(Vb, cs, k, n, q);
         const double wn = w[q] - s;
         w[q] = wn;
         nrm += wn * wn;
@@ -280,10 +316,7 @@ __global__ void __launch_bounds__(256) k_bd_resid2(BdView b) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) {
         double a = 0.0, c = 0.0;
-        for (int i = 0; i < k; ++i) {
-            a += ys[i] * V[(uint64_t)i * n + q];
-            c += ys[i] * S[(uint64_t)i * n + q];
-        }
+        combine2(V, S, ys, k, n, q, a, c);
         const double r = c - theta * a;
         acc += r * r;
     }
@@ -313,29 +346,42 @@ __global__ void __launch_bounds__(64) k_bd_same_spin(BdView b, uint64_t da, uint
     const uint64_t ia0 = Ia0 + r, ia1 = Ia0 + 16 + r, ib0 = Ib0 + r, ib1 = Ib0 + 16 + r;
     const bool ra0 = ia0 < da, ra1 = ia1 < da, rb0 = ib0 < db, rb1 = ib1 < db;
     v4d acc00 = {0.0, 0.0, 0.0, 0.0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
-    for (uint64_t k0 = 0; k0 < da; k0 += 4) {
+    // the next k-step's four operands are loaded while the current step's
+    // four MFMAs run
+    double a0, a1, b0, b1;
+    auto load1 = [&](uint64_t k0) {
         const uint64_t k = k0 + kq;
         const bool kin = k < da;
-        const double a0 = kin && ra0 ? Ha[k * da + ia0] : 0.0;
-        const double a1 = kin && ra1 ? Ha[k * da + ia1] : 0.0;
-        const double b0 = kin && rb0 ? C[k * db + ib0] : 0.0;
-        const double b1 = kin && rb1 ? C[k * db + ib1] : 0.0;
-        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-        acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
-    }
-    for (uint64_t k0 = 0; k0 < db; k0 += 4) {
+        a0 = kin && ra0 ? Ha[k * da + ia0] : 0.0;
+        a1 = kin && ra1 ? Ha[k * da + ia1] : 0.0;
+        b0 = kin && rb0 ? C[k * db + ib0] : 0.0;
+        b1 = kin && rb1 ? C[k * db + ib1] : 0.0;
+    };
+    auto load2 = [&](uint64_t k0) {
         const uint64_t k = k0 + kq;
         const bool kin = k < db;
-        const double a0 = kin && ra0 ? C[ia0 * db + k] : 0.0;
-        const double a1 = kin && ra1 ? C[ia1 * db + k] : 0.0;
-        const double b0 = kin && rb0 ? Hb[k * db + ib0] : 0.0;
-        const double b1 = kin && rb1 ? Hb[k * db + ib1] : 0.0;
-        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc00, 0, 0, 0);
-        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc01, 0, 0, 0);
-        acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc10, 0, 0, 0);
-        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc11, 0, 0, 0);
+        a0 = kin && ra0 ? C[ia0 * db + k] : 0.0;
+        a1 = kin && ra1 ? C[ia1 * db + k] : 0.0;
+        b0 = kin && rb0 ? Hb[k * db + ib0] : 0.0;
+        b1 = kin && rb1 ? Hb[k * db + ib1] : 0.0;
+    };
+    auto step = [&](double x0, double x1, double y0, double y1) {
+        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y0, acc00, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y1, acc01, 0, 0, 0);
+        acc10 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y0, acc10, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y1, acc11, 0, 0, 0);
+    };
+    load1(0);
+    for (uint64_t k0 = 0; k0 < da; k0 += 4) {
+        const double x0 = a0, x1 = a1, y0 = b0, y1 = b1;
+        if (k0 + 4 < da) load1(k0 + 4);
+        else load2(0);
+        step(x0, x1, y0, y1);
+    }
+    for (uint64_t k0 = 0; k0 < db; k0 += 4) {
+        const double x0 = a0, x1 = a1, y0 = b0, y1 = b1;
+        if (k0 + 4 < db) load2(k0 + 4);
+        step(x0, x1, y0, y1);
     }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {

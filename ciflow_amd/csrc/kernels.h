@@ -237,7 +237,6 @@ struct BdView {
     double* V = nullptr;  // [lanes][M][n]
     double* S = nullptr;  // [lanes][M][n] images H V
     double* x = nullptr;  // [lanes][n] Ritz vector
-    double* hx = nullptr;
     double* t = nullptr;  // correction / new vector before normalisation
     float* x32 = nullptr;  // [lanes][n] FP32 copy of the new vector (or null)
     double* partial = nullptr;  // [lanes][kBdPartial]
@@ -250,6 +249,7 @@ struct BdView {
 // Restart rotations, then the new vector of every lane: x = V y, r = S y -
 // theta x, t = r / (theta - diag), two selective Gram-Schmidt passes, v_m =
 // t / |t| (and its FP32 copy when x32 is set and the lane's product is FP32).
+// Every lane's sums run in a fixed order that depends on n alone.
 void bd_form_vectors(const BdView& b, int lanes, bool any_restart, hipStream_t s);
 // dg rows 0..m: V^T S_m after the product.
 void bd_new_column(const BdView& b, int lanes, hipStream_t s);

@@ -98,7 +98,7 @@ struct Batch {
     std::vector<Lane> lanes;  // cap slots
     std::vector<BdOpLane> hop;
     std::vector<SigmaArgs> hargs;
-    DBuf<double> V, S, x, hx, t, partial, parts, dg, Y;
+    DBuf<double> V, S, x, t, partial, parts, dg, Y;
     DBuf<float> x32;
     DBuf<BdStep> dstep;
     DBuf<BdOpLane> dop;
@@ -136,7 +136,6 @@ void Batch::alloc() {
     V.alloc(Bn * M);
     S.alloc(Bn * M);
     x.alloc(Bn);
-    hx.alloc(Bn);
     t.alloc(Bn);
     if (kind == CF_FCI) x32.alloc(Bn);
     partial.alloc((size_t)cap * kBdPartial);
@@ -153,7 +152,6 @@ void Batch::alloc() {
     view.V = V.p;
     view.S = S.p;
     view.x = x.p;
-    view.hx = hx.p;
     view.t = t.p;
     view.x32 = x32.p;
     view.partial = partial.p;
