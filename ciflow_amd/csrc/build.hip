@@ -26,6 +26,8 @@
 // alpha double to the single column ja*db + ib.  segoff[ia] holds the prefix
 // offsets of these segments; a lane finds its segment by binary search.
 #include "kernels.h"
+
+#include <hipcub/hipcub.hpp>
 #include "string_ops.h"
 
 namespace cf {
@@ -422,6 +424,180 @@ __global__ void __launch_bounds__(256) k_det_rows(DetRowsArgs a) {
         cnt += (uint32_t)__popcll(mask);
     }
     if (!FILL && lane == 0) a.rowcnt[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Determinant-list rows by excitation lookup (L <= 32).  The list's keys
+// (up << 32 | down) are sorted once; the columns of row i are the list
+// determinants that at most two excitations of det_i reach: the entries with
+// up_i (one binary-searched range, filtered by |down_i ^ down_j| <= 4), those
+// whose up string is a single excitation of up_i (their range, filtered by
+// |down_i ^ down_j| <= 2, or for long ranges looked up by down_i and its
+// single excitations), and the keys (double excitation of up_i, down_i)
+// looked up exactly.  That is O(n (E_a log n + group sizes)) work instead of
+// n^2 pair tests.  Each candidate's element is get_ham_element in the
+// orientation k_det_rows uses; survivors take slots from a per-wavefront LDS
+// counter (unordered) and each row is sorted by column afterwards, so the
+// CRS equals k_det_rows' bit for bit.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t lower_bound_key(const uint64_t* __restrict__ a, uint64_t lo, uint64_t hi,
+                                                    uint64_t key) {
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int kth_set_bit(str_t v, int k) {
+    for (int i = 0; i < k; ++i) v &= v - 1;
+    return __builtin_ctzll(v);
+}
+
+// (a, b), a < b, of pair index p among k items in the order (0,1), (0,2), ...
+__device__ __forceinline__ void pair_of(int p, int k, int& a, int& b) {
+    a = 0;
+    while (p >= k - 1 - a) {
+        p -= k - 1 - a;
+        ++a;
+    }
+    b = a + 1 + p;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_det_rows_lookup(DetLookupArgs A) {
+    __shared__ uint32_t wcnt[4];
+    const DetRowsArgs& a = A.r;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t i = (uint64_t)blockIdx.x * 4 + w;
+    const bool act = i < a.n;
+    if (lane == 0) wcnt[w] = 0;
+    __syncthreads();
+    if (act) {
+        const int L = A.L;
+        const uint64_t n = a.n;
+        const str_t ui = a.up[i], di = a.down[i];
+        const str_t full = L >= 64 ? ~(str_t)0 : (((str_t)1 << L) - 1);
+        const int na = a.nup, va = L - a.nup;
+        const int nb = a.ndown, vb = L - a.ndown;
+        int64_t base = 0;
+        if (FILL) base = a.rowptr[i];
+        const uint64_t* __restrict__ key = A.skey;
+        auto emit = [&](uint64_t j, str_t uj, str_t dj) {
+            const double v = j >= i ? det_element(ui, di, uj, dj, a.H) : det_element(uj, dj, ui, di, a.H);
+            if (fabs(v) > a.zero) {
+                const uint32_t pos = atomicAdd(&wcnt[w], 1u);
+                if (FILL) {
+                    a.col[base + pos] = (uint32_t)j;
+                    a.val[base + pos] = v;
+                }
+            }
+        };
+        // the sorted range of the entries with up string u
+        auto group = [&](str_t u, uint64_t& g0, uint64_t& g1) {
+            const uint64_t k0 = (uint64_t)u << 32, k1 = k0 + (1ull << 32);
+            g0 = lower_bound_key(key, 0, n, k0);
+            g1 = k1 == 0 ? n : lower_bound_key(key, g0, n, k1);
+        };
+        // every list entry with key k in [lo, hi)
+        auto take_equal = [&](uint64_t lo, uint64_t hi, uint64_t k, str_t uj, str_t dj) {
+            for (uint64_t t = lower_bound_key(key, lo, hi, k); t < hi && key[t] == k; ++t) emit(A.sidx[t], uj, dj);
+        };
+        // same up string: down within two excitations (diagonal, beta singles and doubles)
+        {
+            uint64_t g0, g1;
+            group(ui, g0, g1);
+            for (uint64_t t = g0 + lane; t < g1; t += 64) {
+                const str_t d = (str_t)(key[t] & 0xffffffffull);
+                if (__popcll(di ^ d) <= 4) emit(A.sidx[t], ui, d);
+            }
+        }
+        // up string one alpha single away: down equal or one beta single away
+        for (int sidx = lane; sidx < na * va; sidx += 64) {
+            const int o = kth_set_bit(ui, sidx / va), e = kth_set_bit(~ui & full, sidx % va);
+            const str_t u2 = ui ^ ((str_t)1 << o) ^ ((str_t)1 << e);
+            const uint64_t k0 = (uint64_t)u2 << 32;
+            uint64_t g0, g1;
+            group(u2, g0, g1);
+            if (g1 - g0 <= (uint64_t)(2 * (1 + nb * vb))) {
+                for (uint64_t t = g0; t < g1; ++t) {
+                    const str_t d = (str_t)(key[t] & 0xffffffffull);
+                    if (__popcll(di ^ d) <= 2) emit(A.sidx[t], u2, d);
+                }
+            } else {
+                take_equal(g0, g1, k0 | di, u2, di);
+                for (int q = 0; q < nb * vb; ++q) {
+                    const int bo = kth_set_bit(di, q / vb), be = kth_set_bit(~di & full, q % vb);
+                    const str_t d2 = di ^ ((str_t)1 << bo) ^ ((str_t)1 << be);
+                    take_equal(g0, g1, k0 | d2, u2, d2);
+                }
+            }
+        }
+        // up string one alpha double away, down equal
+        const int pa = na * (na - 1) / 2, pv = va * (va - 1) / 2;
+        for (int didx = lane; didx < pa * pv; didx += 64) {
+            int o0, o1, e0, e1;
+            pair_of(didx / pv, na, o0, o1);
+            pair_of(didx % pv, va, e0, e1);
+            const str_t empty = ~ui & full;
+            const str_t u2 = ui ^ ((str_t)1 << kth_set_bit(ui, o0)) ^ ((str_t)1 << kth_set_bit(ui, o1)) ^
+                             ((str_t)1 << kth_set_bit(empty, e0)) ^ ((str_t)1 << kth_set_bit(empty, e1));
+            take_equal(0, n, ((uint64_t)u2 << 32) | di, u2, di);
+        }
+    }
+    __syncthreads();
+    if (act && lane == 0 && !FILL) a.rowcnt[i] = wcnt[w];
+}
+
+void launch_det_rows_lookup(const DetLookupArgs& a, bool fill, hipStream_t stream) {
+    if (!a.r.n) return;
+    if (a.L > 32) throw Error(ERR_UNSUPPORTED, "determinant-list lookup build: L <= 32");
+    const uint64_t grid = (a.r.n + 3) / 4;
+    if (grid > 0xffffffffull) throw Error(ERR_UNSUPPORTED, "determinant list too long");
+    if (fill) k_det_rows_lookup<true><<<(unsigned)grid, 256, 0, stream>>>(a);
+    else k_det_rows_lookup<false><<<(unsigned)grid, 256, 0, stream>>>(a);
+    CF_HIP(hipGetLastError());
+}
+
+__global__ void k_det_keys(uint64_t n, const str_t* __restrict__ up, const str_t* __restrict__ down,
+                           uint64_t* __restrict__ key, uint32_t* __restrict__ idx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = ((uint64_t)up[i] << 32) | (uint64_t)down[i];
+    idx[i] = (uint32_t)i;
+}
+
+// Sorted keys (up << 32 | down) of the list with the list index of each.
+void sort_det_keys(uint64_t n, const str_t* up, const str_t* down, uint64_t* skey, uint32_t* sidx, int bits,
+                   hipStream_t stream) {
+    if (!n) return;
+    if (n > 0x7fffffffull) throw Error(ERR_UNSUPPORTED, "determinant list too long for the key sort");
+    DBuf<uint64_t> key(n);
+    DBuf<uint32_t> idx(n);
+    k_det_keys<<<(unsigned)((n + 255) / 256), 256, 0, stream>>>(n, up, down, key.p, idx.p);
+    CF_HIP(hipGetLastError());
+    size_t tb = 0;
+    CF_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.p, skey, idx.p, sidx, (int)n, 0, bits, stream));
+    DBuf<unsigned char> tmp(tb);
+    CF_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.p, skey, idx.p, sidx, (int)n, 0, bits, stream));
+}
+
+// Each row's (col, val) pairs sorted by column (rows given by rowptr).
+void sort_rows_by_column(uint64_t n, uint64_t nnz, const int64_t* rowptr, uint32_t* col, double* val,
+                         hipStream_t stream) {
+    if (!nnz) return;
+    if (nnz > 0x7fffffffull || n > 0x7fffffffull) throw Error(ERR_UNSUPPORTED, "row sort: too many entries");
+    DBuf<uint32_t> col2(nnz);
+    DBuf<double> val2(nnz);
+    size_t tb = 0;
+    CF_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tb, col, col2.p, val, val2.p, (int)nnz, (int)n,
+                                                       rowptr, rowptr + 1, 0, 32, stream));
+    DBuf<unsigned char> tmp(tb);
+    CF_HIP(hipcub::DeviceSegmentedRadixSort::SortPairs(tmp.p, tb, col, col2.p, val, val2.p, (int)nnz, (int)n,
+                                                       rowptr, rowptr + 1, 0, 32, stream));
+    CF_HIP(hipMemcpyAsync(col, col2.p, nnz * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream));
+    CF_HIP(hipMemcpyAsync(val, val2.p, nnz * sizeof(double), hipMemcpyDeviceToDevice, stream));
 }
 
 void launch_det_rows(const DetRowsArgs& a, bool fill, hipStream_t stream) {

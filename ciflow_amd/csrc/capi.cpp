@@ -431,6 +431,12 @@ void build_crs_arrays(cf_crs* m) {
     FciRowsArgs fa{};
     DociRowsArgs dargs{};
     DetRowsArgs targs{};
+    // determinant lists with L <= 32 find each row's columns by excitation
+    // lookup in the sorted (up, down) keys instead of n^2 pair tests
+    const bool lookup = sp->kind == CF_DETS && H->L <= 32;
+    DBuf<uint64_t> skey;
+    DBuf<uint32_t> sidx;
+    DetLookupArgs largs{};
     if (sp->kind == CF_DETS) {
         targs.n = sp->n;
         targs.nup = H->nup;
@@ -440,8 +446,20 @@ void build_crs_arrays(cf_crs* m) {
         targs.H = H->dev();
         targs.zero = zero;
         targs.rowcnt = cnt.p;
-        KTimer t(c, "crs_count");
-        launch_det_rows(targs, false, c->stream);
+        if (lookup) {
+            skey.alloc(sp->n);
+            sidx.alloc(sp->n);
+            sort_det_keys(sp->n, sp->dup.p, sp->ddown.p, skey.p, sidx.p, 32 + H->L, c->stream);
+            largs.r = targs;
+            largs.skey = skey.p;
+            largs.sidx = sidx.p;
+            largs.L = H->L;
+            KTimer t(c, "crs_count");
+            launch_det_rows_lookup(largs, false, c->stream);
+        } else {
+            KTimer t(c, "crs_count");
+            launch_det_rows(targs, false, c->stream);
+        }
     } else if (sp->kind == CF_FCI) {
         fa.da = sp->da;
         fa.db = sp->db;
@@ -494,7 +512,17 @@ void build_crs_arrays(cf_crs* m) {
         targs.val = m->val.p;
         targs.val32 = m->val32.p;
         KTimer t(c, "crs_fill");
-        launch_det_rows(targs, true, c->stream);
+        if (lookup && m->nnz <= 0x7fffffffull) {
+            largs.r.rowcnt = nullptr;
+            largs.r.rowptr = m->rowptr.p;
+            largs.r.col = m->col.p;
+            largs.r.val = m->val.p;
+            launch_det_rows_lookup(largs, true, c->stream);
+            sort_rows_by_column(m->n, m->nnz, m->rowptr.p, m->col.p, m->val.p, c->stream);
+            if (m->want32) launch_to_f32(m->nnz, m->val.p, m->val32.p, c->stream);
+        } else {
+            launch_det_rows(targs, true, c->stream);
+        }
     } else if (sp->kind == CF_FCI) {
         fa.rowptr = m->rowptr.p;
         fa.col = m->col.p;

@@ -103,6 +103,20 @@ struct DetRowsArgs {
     float* val32;
 };
 void launch_det_rows(const DetRowsArgs& a, bool fill, hipStream_t stream);
+// The same rows by excitation lookup in the sorted keys (L <= 32); the fill
+// writes each row's entries unordered (sort_rows_by_column orders them).
+struct DetLookupArgs {
+    DetRowsArgs r;                  // n, nup, ndown, up, down, H, zero, rowcnt / rowptr, col, val (val32 unused)
+    const uint64_t* skey = nullptr;  // sorted keys up << 32 | down
+    const uint32_t* sidx = nullptr;  // list index of each sorted key
+    int L = 0;
+};
+void launch_det_rows_lookup(const DetLookupArgs& a, bool fill, hipStream_t stream);
+// bits: significant key bits (32 + L)
+void sort_det_keys(uint64_t n, const str_t* up, const str_t* down, uint64_t* skey, uint32_t* sidx, int bits,
+                   hipStream_t stream);
+void sort_rows_by_column(uint64_t n, uint64_t nnz, const int64_t* rowptr, uint32_t* col, double* val,
+                         hipStream_t stream);
 
 // Exclusive scan of 32-bit counts into 64-bit offsets (n+1 entries).
 void exclusive_scan_u32_to_i64(const uint32_t* cnt, int64_t* out, uint64_t n, hipStream_t stream);

@@ -195,6 +195,38 @@ def test_det_list_random_subset_bit_exact(L, na, nb, count, seed):
     assert abs(cim.get_ci_energy() - w) < 1e-8
 
 
+def test_det_list_block_with_repeats_bit_exact():
+    """A CAS-like block (the first 40 alpha strings of FCI L=10 5a5b with every
+    beta string, 10,080 determinants, shuffled) with three determinants listed
+    twice: the CRS built by excitation lookup in the sorted keys equals the
+    oracle's all-pairs FCI_File::construct_CI_matrix bit for bit."""
+    L, na, nb = 10, 5, 5
+    h, V = CA.gen_integrals(L, 21)
+    A = O.enumerate_strings(L, na)
+    B = O.enumerate_strings(L, nb)
+    ia = np.repeat(np.arange(40), len(B))
+    ib = np.tile(np.arange(len(B)), 40)
+    perm = np.random.default_rng(21).permutation(len(ia))
+    up, down = A[ia[perm]], B[ib[perm]]
+    up = np.concatenate([up, up[[5, 777, 9000]]])
+    down = np.concatenate([down, down[[5, 777, 9000]]])
+    say("det block", len(up))
+    cim = CA.FCI_File(CA.Hamiltonian(h, V, na, nb), (up, down))
+    check_det_crs(cim, O.build_crs_dets(h, V, na, nb, up, down))
+
+
+def test_det_list_beyond_32_orbitals_bit_exact():
+    """L = 34 (keys no longer fit 32 bits per spin): the all-pairs build, bit-exact."""
+    L, na, nb = 34, 2, 2
+    h, V = CA.gen_integrals(L, 4)
+    rng = np.random.default_rng(4)
+    A = O.enumerate_strings(L, na)
+    pick = rng.choice(len(A) * len(A), 300, replace=False)
+    up, down = A[pick // len(A)], A[pick % len(A)]
+    cim = CA.FCI_File(CA.Hamiltonian(h, V, na, nb), (up, down))
+    check_det_crs(cim, O.build_crs_dets(h, V, na, nb, up, down))
+
+
 def test_det_list_rejects_bad_input():
     h, V = CA.gen_integrals(6, 1)
     ham = CA.Hamiltonian(h, V, 3, 3)
