@@ -350,6 +350,32 @@ class Hamiltonian:
         return ham
 
     @classmethod
+    def read(cls, path: str, ctx: Context | None = None) -> "Hamiltonian":
+        """Hamiltonian::read (Hamiltonian.cpp:402-495): the HDF5 file of
+        Hamiltonian::save, symmetry-packed integrals expanded to dense
+        arrays (host library libciflow_h5.so)."""
+        from . import h5
+
+        d = h5.read_hamiltonian(path)
+        ham = cls(d["h"], d["V"], d["nalpha"], d["nbeta"], d["econst"], ctx)
+        ham.irreps, ham.ngroup, ham.overlap = d["irreps"], d["ngroup"], d["overlap"]
+        return ham
+
+    def save(self, path: str, ngroup: int | None = None, irreps=None, overlap=None) -> None:
+        """Hamiltonian::save (Hamiltonian.cpp:322-400) to exactly `path`;
+        the point group and orbital irreps default to those the Hamiltonian
+        was read with (C1 otherwise)."""
+        from . import h5
+
+        irreps = getattr(self, "irreps", None) if irreps is None else irreps
+        if ngroup is None:
+            ngroup = getattr(self, "ngroup", 0 if irreps is None or not np.any(np.asarray(irreps)) else None)
+            if ngroup is None:
+                raise ValueError("orbital irreps given without their point group (ngroup 0-7)")
+        overlap = getattr(self, "overlap", None) if overlap is None else overlap
+        h5.save_hamiltonian(path, self.h, self.V, self.nup, self.ndown, self.econst, ngroup, irreps, overlap)
+
+    @classmethod
     def synthetic(cls, L: int, nup: int, ndown: int, seed: int, ctx: Context | None = None):
         h, V = gen_integrals(L, seed)
         return cls(h, V, nup, ndown, 0.0, ctx)
@@ -434,6 +460,15 @@ class SparseMatrix_CRS_Sym:
 
     def export_full(self, r0: int = 0, r1: int | None = None):
         return self._export(False, r0, self.getn() if r1 is None else r1)
+
+    def write_to_file(self, filename: str, name: str, append: bool = False) -> None:
+        """SparseMatrix_CRS::write_to_file (SparseMatrix_CRS.cpp:484-571):
+        the upper CRS as group `name` of an HDF5 file; read it back with
+        ciflow_amd.h5.read_crs (SparseMatrix_CRS::read_from_file)."""
+        from . import h5
+
+        row, col, val = self.export_upper_u32()
+        h5.write_crs(filename, name, row, col, val, append)
 
     def spmv_bytes(self, precision: int = CF_F64):
         """(kernel id 1-6, algorithmic HBM bytes per product) of the current choice."""
@@ -918,3 +953,6 @@ def arpack_like(matvec, n: int, ncv: int = 42, tol: float = 1e-12):
         i1 = r.ipntr[1] - 1
         r.workd[i1:i1 + n] = matvec(r.workd[i0:i0 + n])
     return r.result()
+
+
+from . import h5  # noqa: E402  (HDF5 files; its library loads on first use)

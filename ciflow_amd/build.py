@@ -62,5 +62,41 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+H5_LIB = os.path.join(HERE, "libciflow_h5.so")
+H5_PREFIXES = [os.environ.get("CIFLOW_HDF5_PREFIX", ""), "/opt/conda", "/usr"]
+
+
+def hdf5_prefix() -> str | None:
+    """Install prefix of the HDF5 C library (headers and libhdf5.so), or None."""
+    for p in H5_PREFIXES:
+        if p and os.path.exists(os.path.join(p, "include", "hdf5.h")) and \
+                any(os.path.exists(os.path.join(p, d, "libhdf5.so")) for d in ("lib", "lib64")):
+            return p
+    return None
+
+
+def build_h5(verbose: bool = True) -> str | None:
+    """Build the host-only HDF5 file library libciflow_h5.so (no GPU code).
+    Returns None, and builds nothing, where no HDF5 C library is installed."""
+    prefix = hdf5_prefix()
+    if prefix is None:
+        if verbose:
+            print("HDF5 C library not found: libciflow_h5.so not built")
+        return None
+    libdir = os.path.join(prefix, "lib" if os.path.exists(os.path.join(prefix, "lib", "libhdf5.so")) else "lib64")
+    tmp = H5_LIB + ".tmp"
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           "-I", os.path.join(prefix, "include"), os.path.join(CSRC, "h5io.cpp"), "-o", tmp,
+           f"-L{libdir}", "-lhdf5", f"-Wl,-rpath,{libdir}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"HDF5 library build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, H5_LIB)
+    if verbose:
+        print(f"built {H5_LIB}")
+    return H5_LIB
+
+
 if __name__ == "__main__":
     build()
+    build_h5()

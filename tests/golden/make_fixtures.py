@@ -11,8 +11,9 @@ nbeta and the orbital irreps.
 
 Run from the repo root:  python tests/golden/make_fixtures.py
 (needs /root/reference; the generated .npz files are committed).  The two
-determinant lists of the reference's selected-CI tests and the BeH2 dump are
-copied unchanged.
+determinant lists of the reference's selected-CI tests, the BeH2 dump, the two
+HDF5 integral files, the RHF test's overlap and the Hamiltonian test's two
+orbital transformations are copied unchanged.
 """
 from __future__ import annotations
 
@@ -36,7 +37,9 @@ def read_dump(path):
     econst = float(val(lines[i + 5]))
     L = int(val(lines[i + 6]))
     irreps = [int(x) for x in lines[i + 8].split()]
-    k = next(k for k in range(i, len(lines)) if lines[k].startswith("****  MO OEI"))
+    # MO dumps mark the sections "****  MO OEI" / "****  MO TEI", the
+    # atomic-orbital dump "*** OEI SO" / "*** TEI"
+    k = next(k for k in range(i, len(lines)) if lines[k].startswith(("****  MO OEI", "*** OEI")))
     h = np.zeros((L, L))
     k += 1
     while not lines[k].startswith("*"):
@@ -45,7 +48,7 @@ def read_dump(path):
         h[a, b] = v
         h[b, a] = v
         k += 1
-    assert lines[k].startswith("****  MO TEI")
+    assert lines[k].startswith(("****  MO TEI", "*** TEI"))
     k += 1
     V = np.zeros((L, L, L, L))
     while not lines[k].startswith("*"):
@@ -64,13 +67,19 @@ def main():
         print("reference data not found", file=sys.stderr)
         return 1
     for name, src in (("beh2_sto3g", "beh2_sto3g_1_5.dat"), ("psi0_631g", "psi0_6-31g0.60.mout"),
-                      ("be_ccpvdz", "Be_ccpVDZ.dat")):
+                      ("be_ccpvdz", "Be_ccpVDZ.dat"), ("beh2_atomic", "beh2atomic.dat")):
         d = read_dump(os.path.join(REF, src))
         np.savez_compressed(os.path.join(OUT, f"{name}.npz"), **d)
         print(name, d["L"], d["nalpha"], d["nbeta"], d["econst"])
     # determinant lists the reference's selected-CI tests read (Permutator_File)
     # and the BeH2 integral dump (for the library's own file reader), byte for byte
-    for src in ("GKCI_Be_cntSp_cc-pVDZ_0100.txt", "fci_beh2_sto-3gdeterminants.dat", "beh2_sto3g_1_5.dat"):
+    # and the reference's HDF5 integral files (Hamiltonian::read) with the
+    # overlap its RHF test loads next to one of them (TestRHF.cpp:29-30) and
+    # the two orbital transformations its Hamiltonian test applies to them
+    # (TestHam.cpp:84,103)
+    for src in ("GKCI_Be_cntSp_cc-pVDZ_0100.txt", "fci_beh2_sto-3gdeterminants.dat", "beh2_sto3g_1_5.dat",
+                "atomicintegralsbeh2_sto3g.h5", "atomicintegralsorthonbeh2_sto3g.h5", "overlap.txt",
+                "unitary-moorthon.h5", "unitary-mounit_ao_to_mo.txt"):
         shutil.copyfile(os.path.join(REF, src), os.path.join(OUT, src))
         print(src)
     return 0

@@ -138,6 +138,37 @@ def test_kat_energies_from_the_integral_file():
     assert abs(f.get_ci_energy() - golden()["beh2"]["fci_energy"]) < 1e-10
 
 
+@pytest.mark.skipif(not CA.h5.available(), reason="libciflow_h5.so not built (no HDF5 C library)")
+def test_hdf5_integrals_and_crs_file(tmp_path):
+    """Hamiltonian::read of the reference's orthonormal-AO BeH2 file (D2h):
+    its FCI energy is the MO-basis one (orbital-rotation invariance); the
+    GPU CRS written by SparseMatrix_CRS::write_to_file reads back equal to the
+    oracle's upper CRS for the same integrals; Hamiltonian::save round-trips;
+    the file's transformation to MOs gives the DOCI KAT."""
+    path = os.path.join(GOLD, "atomicintegralsorthonbeh2_sto3g.h5")
+    ham = CA.Hamiltonian.read(path)
+    assert list(ham.irreps) == [0, 0, 0, 5, 5, 6, 7] and ham.ngroup == 7
+    f = CA.FCI(ham)
+    f.solve()
+    assert abs(f.get_ci_energy() - golden()["beh2"]["fci_energy"]) < 1e-10
+    out = os.path.join(tmp_path, "crs.h5")
+    f.get_mat().write_to_file(out, "fci")
+    row, col, val = CA.h5.read_crs(out, "fci")
+    ref = O.build_crs("FCI", ham.h, ham.V, 3, 3)
+    assert np.array_equal(row, np.asarray(ref.row, dtype=np.uint32))
+    assert np.array_equal(col, ref.col) and np.array_equal(val, ref.data)
+    saved = os.path.join(tmp_path, "ham.h5")
+    ham.save(saved)
+    d = CA.h5.read_hamiltonian(saved)
+    assert np.array_equal(d["h"], ham.h) and np.array_equal(d["V"], ham.V) and d["econst"] == ham.econst
+    # TestHam.cpp:80-93: the reference's transformation to MOs (UnitaryMatrix::loadU),
+    # applied on the GPU, gives the DOCI KAT
+    U = CA.h5.read_unitary(os.path.join(GOLD, "unitary-moorthon.h5"), ham.irreps, ham.ngroup)
+    doci = CA.DOCI(ham.rotate(U))
+    doci.solve()
+    assert abs(doci.get_ci_energy() - (-15.225362047972723)) < 1e-10
+
+
 # ---- selected CI over determinant lists (FCI_File) ----------------------------
 DET_FILES = {"be_gkci100": ("be_ccpvdz.npz", "GKCI_Be_cntSp_cc-pVDZ_0100.txt"),
              "beh2_full_list": ("beh2_sto3g.npz", "fci_beh2_sto-3gdeterminants.dat")}
