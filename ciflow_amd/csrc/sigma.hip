@@ -484,9 +484,13 @@ template void launch_sigma_ab2<float>(const SigmaArgs&, SigmaShape, const float*
 // its negation, [2 L^2 s, 2 L^2 s + d_b s) the C row.  BUFB > 0: two buffers
 // BUFB bytes apart (BUFB a compile-time constant, so buffer 1's reads carry
 // it as the instruction's immediate offset); BUFB = 0: one buffer and two
-// barriers per action.
+// barriers per action.  PIN: the codes are re-declared inside the term loop
+// (an empty asm that "modifies" each one), so the compiler unpacks them per
+// use (two more VALU per term) instead of hoisting 2 NS unpacked offsets
+// into registers -- at NS = 72 that hoisting took 214 registers and left 2
+// waves per SIMD.
 // ---------------------------------------------------------------------------
-template <typename CT, int NT, int PF, int PV, int NS, int BUFB, int LDSB, bool BATCH>
+template <typename CT, int NT, int PF, int PV, int NS, int BUFB, int LDSB, bool BATCH, bool PIN>
 __global__ void __launch_bounds__(NT) k_sigma_ab3(SigmaArgs a0, const CT* __restrict__ C0, double* __restrict__ sig0,
                                                   SigmaBatch sb) {
     // static LDS (the kernel's only LDS): its base is the constant 0, so the
@@ -581,6 +585,10 @@ __global__ void __launch_bounds__(NT) k_sigma_ab3(SigmaArgs a0, const CT* __rest
         double t = 0.0;
 #pragma unroll
         for (int j = 0; j < NS; ++j) {
+            // PIN: the code is redefined here, so its two unpacked offsets are
+            // formed per use instead of being hoisted out of the action loop
+            // into 2 NS more registers
+            if (PIN) asm volatile("" : "+v"(code[j]));
             const uint32_t c = code[j];
             const CT v = *(const CT*)(buf + (c & 0x1FFFu));
             const CT x = *(const CT*)(buf + (c >> 13));
@@ -657,7 +665,7 @@ static bool ab3_prepare(const SigmaArgs& a0, SigmaArgs& a, uint64_t& grid) {
 }
 
 // sb: a batched product over `lanes` lanes (grid rows), each with its own args.
-template <typename CT, int NT, int PF, int NS, int BUFB, int LDSB>
+template <typename CT, int NT, int PF, int NS, int BUFB, int LDSB, bool PIN = false>
 static bool launch_ab3_t(const SigmaArgs& a0, const CT* C, double* sig, hipStream_t s, SigmaArgs* prepared = nullptr,
                          const SigmaBatch* sb = nullptr, int lanes = 1) {
     SigmaArgs a;
@@ -671,12 +679,12 @@ static bool launch_ab3_t(const SigmaArgs& a0, const CT* C, double* sig, hipStrea
     const bool pv1 = 2 * L2 <= (uint64_t)NT;
     if (sb) {
         const dim3 g((unsigned)grid, (unsigned)lanes);
-        if (pv1) k_sigma_ab3<CT, NT, PF, 1, NS, BUFB, LDSB, true><<<g, NT, 0, s>>>(a, nullptr, nullptr, *sb);
-        else k_sigma_ab3<CT, NT, PF, 2, NS, BUFB, LDSB, true><<<g, NT, 0, s>>>(a, nullptr, nullptr, *sb);
+        if (pv1) k_sigma_ab3<CT, NT, PF, 1, NS, BUFB, LDSB, true, PIN><<<g, NT, 0, s>>>(a, nullptr, nullptr, *sb);
+        else k_sigma_ab3<CT, NT, PF, 2, NS, BUFB, LDSB, true, PIN><<<g, NT, 0, s>>>(a, nullptr, nullptr, *sb);
     } else {
         const SigmaBatch none{};
-        if (pv1) k_sigma_ab3<CT, NT, PF, 1, NS, BUFB, LDSB, false><<<(unsigned)grid, NT, 0, s>>>(a, C, sig, none);
-        else k_sigma_ab3<CT, NT, PF, 2, NS, BUFB, LDSB, false><<<(unsigned)grid, NT, 0, s>>>(a, C, sig, none);
+        if (pv1) k_sigma_ab3<CT, NT, PF, 1, NS, BUFB, LDSB, false, PIN><<<(unsigned)grid, NT, 0, s>>>(a, C, sig, none);
+        else k_sigma_ab3<CT, NT, PF, 2, NS, BUFB, LDSB, false, PIN><<<(unsigned)grid, NT, 0, s>>>(a, C, sig, none);
     }
     CF_HIP(hipGetLastError());
     return true;
@@ -689,18 +697,29 @@ static bool ab3_dispatch(const SigmaArgs& a, int nb_electrons, const CT* C, doub
     constexpr int S = (int)sizeof(CT);
 #define CF_AB3(NT, PF, NSV, BUFB, LDSB) \
     launch_ab3_t<CT, NT, PF, NSV, BUFB, LDSB>(a, C, sig, s, prepared, sb, lanes)
+#define CF_AB3P(NT, PF, NSV, BUFB, LDSB) \
+    launch_ab3_t<CT, NT, PF, NSV, BUFB, LDSB, true>(a, C, sig, s, prepared, sb, lanes)
     switch (ns) {
         case 12: return CF_AB3(128, 1, 12, 4096, 8192);       // L=6 3b, L=7 2b
         case 20: return CF_AB3(128, 1, 20, 4096, 8192);       // L=8 4b
         case 24: return CF_AB3(128, 1, 24, 4096, 8192);       // L=9 4b
         case 30: return CF_AB3(256, 1, 30, 4096, 8192);       // L=10 5b
         case 35: return CF_AB3(256, 2, 35, 8192, 16384);      // L=11 5b
-        case 42: return CF_AB3(512, 2, 42, 16384, 32768);     // L=12 6b
-        // L=16 8b: d_b = 12,870, one buffer
-        case 72: return CF_AB3(512, 26, 72, 0, 2 * 256 * S + 12870 * S);
+        // L=12 6b, codes pinned: 0.417 / 0.453 ms against 0.471 / 0.464 ms
+        // (FP32 / FP64 sigma, MI355X); at L <= 11 pinning was neutral or slower
+        case 42: return CF_AB3P(512, 2, 42, 16384, 32768);
+        // L=16 8b: d_b = 12,870.  1024 threads with the codes pinned (at most
+        // 128 registers, 4 waves per SIMD); FP32: two buffers of 53,528 bytes,
+        // FP64: one buffer of 107,056 bytes.  Measured on MI355X: 251 / 302 ms
+        // against 302 / 435 ms for 512 threads with one buffer and hoisted
+        // offsets (214 / 248 registers, 2 waves per SIMD).
+        case 72:
+            if constexpr (S == 4) return CF_AB3P(1024, 13, 72, 53528, 107056);
+            else return CF_AB3P(1024, 13, 72, 0, 107056);
         default: return false;
     }
 #undef CF_AB3
+#undef CF_AB3P
 }
 
 template <typename CT>
