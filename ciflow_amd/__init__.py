@@ -181,6 +181,7 @@ def lib():
         "cf_expm_skew": (C.c_int, [C.c_int, dp, dp]),
         "cf_gen_rotation": (C.c_int, [C.c_int, C.c_uint64, C.c_double, dp]),
         "cf_sigma_create": (C.c_int, [vp, C.c_int, C.POINTER(vp)]),
+        "cf_dets_sigma_create": (C.c_int, [vp, C.c_uint64, C.POINTER(vp)]),
         "cf_sigma_destroy": (C.c_int, [vp]),
         "cf_sigma_set_threads": (C.c_int, [vp, C.c_int]),
         "cf_sigma_set_kernel": (C.c_int, [vp, C.c_int]),
@@ -809,6 +810,68 @@ class FCI_Direct(CIMethod):
         _check(lib().cf_rdm1_direct(self.ptr, _d(np.ascontiguousarray(c)), _d(one)))
         self._one = one
         return one, None
+
+
+class CI_Big(CIMethod):
+    """Selected CI over a determinant list without storing its matrix
+    (CIMethod.h:176-189, CIMethod.cpp:591-609, 943-1116): CI_Big(ham,
+    "dets.txt") or CI_Big(ham, (up, down)).  ``stored_rows`` leading rows and
+    columns are stored as FCI_File stores its matrix (the reference keeps as
+    many as its memory budget allows); every other element is evaluated on
+    the GPU in each product from the sorted determinant keys.  The solve is
+    Lanczos (ARPACK semantics) on that product; construct_density gives
+    DensFILE's RDMs, as for FCI_File."""
+
+    kind = CF_DETS
+
+    def __init__(self, ham: Hamiltonian, dets, stored_rows: int = 0):
+        if isinstance(dets, (str, bytes, os.PathLike)):
+            self.filename = os.fspath(dets)
+            up, down = read_determinants(self.filename)
+        else:
+            self.filename = None
+            up, down = dets
+        self.up = np.ascontiguousarray(up, dtype=np.uint64)
+        self.down = np.ascontiguousarray(down, dtype=np.uint64)
+        self.stored_rows = int(stored_rows)
+        super().__init__(ham, build=False)
+        self.sig = _Handle("cf_sigma_destroy", self.space)
+        _check(lib().cf_dets_sigma_create(self.ptr, self.stored_rows, C.byref(self.sig.ptr)))
+
+    def _create_space(self):
+        _check(lib().cf_space_create_dets(self.ham.ptr, len(self.up), self.up.ctypes.data_as(u64p),
+                                          self.down.ctypes.data_as(u64p), C.byref(self.space.ptr)))
+
+    def _new_space(self, ham: Hamiltonian, build: bool):
+        CI_Big.__init__(self, ham, (self.up, self.down), self.stored_rows)
+
+    def set_ham(self, ham: Hamiltonian, build: bool = True):
+        """CIMethod::set_ham: a new list space and product for the new integrals."""
+        self.sig = None
+        self._new_space(ham, build)
+
+    def get_name(self):
+        return "CI_Big"
+
+    def build_ci(self):
+        raise CIFlowError(CF_ERR_STATE, "CI_Big stores at most its leading block, inside its product")
+
+    def mvprod(self, x, precision: int = CF_F64):
+        """CI_Big::mvprod (CIMethod.cpp:943-984)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        _check(lib().cf_sigma_apply(self.sig.ptr, precision, _d(x), _d(y)))
+        return y
+
+    def solve(self, precision: int = CF_F64, want_vec: bool = True, **kw):
+        """CI_Big::solve / arpackDiagonalize (CIMethod.cpp:1000-1116)."""
+        opts = solve_opts(precision=precision, **kw)
+        info = SolveInfo()
+        vec = np.zeros(self.n) if want_vec else None
+        _check(lib().cf_solve_direct(self.sig.ptr, C.byref(opts), _d(vec) if want_vec else None, C.byref(info)))
+        self._info, self._eigvec, self._eigval = info, vec, info.eigval
+        self._one = self._two = None
+        return info
 
 
 class SolveServer(_Handle):

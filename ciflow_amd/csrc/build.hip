@@ -465,6 +465,71 @@ __device__ __forceinline__ void pair_of(int p, int k, int& a, int& b) {
     b = a + 1 + p;
 }
 
+// emit(j, u_j, d_j) for every list entry j that at most two excitations of
+// det_i reach (row i's candidates), each lane in a fixed order.
+template <class Emit>
+__device__ __forceinline__ void det_row_candidates(const DetLookupArgs& A, uint64_t i, int lane, Emit&& emit) {
+    const DetRowsArgs& a = A.r;
+    const int L = A.L;
+    const uint64_t n = a.n;
+    const str_t ui = a.up[i], di = a.down[i];
+    const str_t full = L >= 64 ? ~(str_t)0 : (((str_t)1 << L) - 1);
+    const int na = a.nup, va = L - a.nup;
+    const int nb = a.ndown, vb = L - a.ndown;
+    const uint64_t* __restrict__ key = A.skey;
+    // the sorted range of the entries with up string u
+    auto group = [&](str_t u, uint64_t& g0, uint64_t& g1) {
+        const uint64_t k0 = (uint64_t)u << 32, k1 = k0 + (1ull << 32);
+        g0 = lower_bound_key(key, 0, n, k0);
+        g1 = k1 == 0 ? n : lower_bound_key(key, g0, n, k1);
+    };
+    // every list entry with key k in [lo, hi)
+    auto take_equal = [&](uint64_t lo, uint64_t hi, uint64_t k, str_t uj, str_t dj) {
+        for (uint64_t t = lower_bound_key(key, lo, hi, k); t < hi && key[t] == k; ++t) emit(A.sidx[t], uj, dj);
+    };
+    // same up string: down within two excitations (diagonal, beta singles and doubles)
+    {
+        uint64_t g0, g1;
+        group(ui, g0, g1);
+        for (uint64_t t = g0 + lane; t < g1; t += 64) {
+            const str_t d = (str_t)(key[t] & 0xffffffffull);
+            if (__popcll(di ^ d) <= 4) emit(A.sidx[t], ui, d);
+        }
+    }
+    // up string one alpha single away: down equal or one beta single away
+    for (int sidx = lane; sidx < na * va; sidx += 64) {
+        const int o = kth_set_bit(ui, sidx / va), e = kth_set_bit(~ui & full, sidx % va);
+        const str_t u2 = ui ^ ((str_t)1 << o) ^ ((str_t)1 << e);
+        const uint64_t k0 = (uint64_t)u2 << 32;
+        uint64_t g0, g1;
+        group(u2, g0, g1);
+        if (g1 - g0 <= (uint64_t)(2 * (1 + nb * vb))) {
+            for (uint64_t t = g0; t < g1; ++t) {
+                const str_t d = (str_t)(key[t] & 0xffffffffull);
+                if (__popcll(di ^ d) <= 2) emit(A.sidx[t], u2, d);
+            }
+        } else {
+            take_equal(g0, g1, k0 | di, u2, di);
+            for (int q = 0; q < nb * vb; ++q) {
+                const int bo = kth_set_bit(di, q / vb), be = kth_set_bit(~di & full, q % vb);
+                const str_t d2 = di ^ ((str_t)1 << bo) ^ ((str_t)1 << be);
+                take_equal(g0, g1, k0 | d2, u2, d2);
+            }
+        }
+    }
+    // up string one alpha double away, down equal
+    const int pa = na * (na - 1) / 2, pv = va * (va - 1) / 2;
+    for (int didx = lane; didx < pa * pv; didx += 64) {
+        int o0, o1, e0, e1;
+        pair_of(didx / pv, na, o0, o1);
+        pair_of(didx % pv, va, e0, e1);
+        const str_t empty = ~ui & full;
+        const str_t u2 = ui ^ ((str_t)1 << kth_set_bit(ui, o0)) ^ ((str_t)1 << kth_set_bit(ui, o1)) ^
+                         ((str_t)1 << kth_set_bit(empty, e0)) ^ ((str_t)1 << kth_set_bit(empty, e1));
+        take_equal(0, n, ((uint64_t)u2 << 32) | di, u2, di);
+    }
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_det_rows_lookup(DetLookupArgs A) {
     __shared__ uint32_t wcnt[4];
@@ -475,16 +540,10 @@ __global__ void __launch_bounds__(256) k_det_rows_lookup(DetLookupArgs A) {
     if (lane == 0) wcnt[w] = 0;
     __syncthreads();
     if (act) {
-        const int L = A.L;
-        const uint64_t n = a.n;
         const str_t ui = a.up[i], di = a.down[i];
-        const str_t full = L >= 64 ? ~(str_t)0 : (((str_t)1 << L) - 1);
-        const int na = a.nup, va = L - a.nup;
-        const int nb = a.ndown, vb = L - a.ndown;
         int64_t base = 0;
         if (FILL) base = a.rowptr[i];
-        const uint64_t* __restrict__ key = A.skey;
-        auto emit = [&](uint64_t j, str_t uj, str_t dj) {
+        det_row_candidates(A, i, lane, [&](uint64_t j, str_t uj, str_t dj) {
             const double v = j >= i ? det_element(ui, di, uj, dj, a.H) : det_element(uj, dj, ui, di, a.H);
             if (fabs(v) > a.zero) {
                 const uint32_t pos = atomicAdd(&wcnt[w], 1u);
@@ -493,61 +552,45 @@ __global__ void __launch_bounds__(256) k_det_rows_lookup(DetLookupArgs A) {
                     a.val[base + pos] = v;
                 }
             }
-        };
-        // the sorted range of the entries with up string u
-        auto group = [&](str_t u, uint64_t& g0, uint64_t& g1) {
-            const uint64_t k0 = (uint64_t)u << 32, k1 = k0 + (1ull << 32);
-            g0 = lower_bound_key(key, 0, n, k0);
-            g1 = k1 == 0 ? n : lower_bound_key(key, g0, n, k1);
-        };
-        // every list entry with key k in [lo, hi)
-        auto take_equal = [&](uint64_t lo, uint64_t hi, uint64_t k, str_t uj, str_t dj) {
-            for (uint64_t t = lower_bound_key(key, lo, hi, k); t < hi && key[t] == k; ++t) emit(A.sidx[t], uj, dj);
-        };
-        // same up string: down within two excitations (diagonal, beta singles and doubles)
-        {
-            uint64_t g0, g1;
-            group(ui, g0, g1);
-            for (uint64_t t = g0 + lane; t < g1; t += 64) {
-                const str_t d = (str_t)(key[t] & 0xffffffffull);
-                if (__popcll(di ^ d) <= 4) emit(A.sidx[t], ui, d);
-            }
-        }
-        // up string one alpha single away: down equal or one beta single away
-        for (int sidx = lane; sidx < na * va; sidx += 64) {
-            const int o = kth_set_bit(ui, sidx / va), e = kth_set_bit(~ui & full, sidx % va);
-            const str_t u2 = ui ^ ((str_t)1 << o) ^ ((str_t)1 << e);
-            const uint64_t k0 = (uint64_t)u2 << 32;
-            uint64_t g0, g1;
-            group(u2, g0, g1);
-            if (g1 - g0 <= (uint64_t)(2 * (1 + nb * vb))) {
-                for (uint64_t t = g0; t < g1; ++t) {
-                    const str_t d = (str_t)(key[t] & 0xffffffffull);
-                    if (__popcll(di ^ d) <= 2) emit(A.sidx[t], u2, d);
-                }
-            } else {
-                take_equal(g0, g1, k0 | di, u2, di);
-                for (int q = 0; q < nb * vb; ++q) {
-                    const int bo = kth_set_bit(di, q / vb), be = kth_set_bit(~di & full, q % vb);
-                    const str_t d2 = di ^ ((str_t)1 << bo) ^ ((str_t)1 << be);
-                    take_equal(g0, g1, k0 | d2, u2, d2);
-                }
-            }
-        }
-        // up string one alpha double away, down equal
-        const int pa = na * (na - 1) / 2, pv = va * (va - 1) / 2;
-        for (int didx = lane; didx < pa * pv; didx += 64) {
-            int o0, o1, e0, e1;
-            pair_of(didx / pv, na, o0, o1);
-            pair_of(didx % pv, va, e0, e1);
-            const str_t empty = ~ui & full;
-            const str_t u2 = ui ^ ((str_t)1 << kth_set_bit(ui, o0)) ^ ((str_t)1 << kth_set_bit(ui, o1)) ^
-                             ((str_t)1 << kth_set_bit(empty, e0)) ^ ((str_t)1 << kth_set_bit(empty, e1));
-            take_equal(0, n, ((uint64_t)u2 << 32) | di, u2, di);
-        }
+        });
     }
     __syncthreads();
     if (act && lane == 0 && !FILL) a.rowcnt[i] = wcnt[w];
+}
+
+// The product of a determinant list's Hamiltonian computed on the fly
+// (CI_Big::mvprod, CIMethod.cpp:943-984): row i, one wavefront, sums
+// H_ij x_j over its candidates, every element in the orientation of the
+// stored matrix and without the drop rule (the reference's on-the-fly part
+// uses every element it evaluates).  Elements with i, j < m belong to the
+// stored leading block, whose product the caller has already written to
+// y[0, m): those rows add, the others overwrite.  Each lane sums its
+// candidates in its fixed traversal order and the wavefront reduces in a
+// fixed tree, so repeated products are bitwise equal.
+__global__ void __launch_bounds__(256) k_det_sigma_lookup(DetLookupArgs A, const double* __restrict__ x,
+                                                          double* __restrict__ y, uint64_t m) {
+    const DetRowsArgs& a = A.r;
+    const int lane = threadIdx.x & 63;
+    const uint64_t i = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.n) return;  // a whole wavefront; no barrier follows
+    const str_t ui = a.up[i], di = a.down[i];
+    double acc = 0.0;
+    det_row_candidates(A, i, lane, [&](uint64_t j, str_t uj, str_t dj) {
+        if (i < m && j < m) return;
+        const double v = j >= i ? det_element(ui, di, uj, dj, a.H) : det_element(uj, dj, ui, di, a.H);
+        acc = fma(v, x[j], acc);
+    });
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) y[i] = i < m ? y[i] + acc : acc;
+}
+
+void launch_det_sigma_lookup(const DetLookupArgs& a, const double* x, double* y, uint64_t m, hipStream_t stream) {
+    if (!a.r.n) return;
+    if (a.L > 32) throw Error(ERR_UNSUPPORTED, "determinant-list on-the-fly product: L <= 32");
+    const uint64_t grid = (a.r.n + 3) / 4;
+    if (grid > 0xffffffffull) throw Error(ERR_UNSUPPORTED, "determinant list too long");
+    k_det_sigma_lookup<<<(unsigned)grid, 256, 0, stream>>>(a, x, y, m);
+    CF_HIP(hipGetLastError());
 }
 
 void launch_det_rows_lookup(const DetLookupArgs& a, bool fill, hipStream_t stream) {

@@ -1802,10 +1802,34 @@ namespace {
 // 4-6x slower than the library GEMM at L = 11 and 16); the alpha-beta kernel
 // takes FP32 or FP64 operands and sums in FP64.  Row-major C (da x db) is the
 // column-major db x da matrix Cc, so y_c = Cc Ha + Hb Cc (Ha, Hb symmetric).
+// CI_Big::mvprod (CIMethod.cpp:943-984): the stored leading block's product
+// into y[0, m), then every other element on the fly.
+void dets_sigma_apply(cf_sigma* S, const double* x, double* y) {
+    cf_space* sp = S->sp;
+    cf_ctx* c = sp->ham->ctx;
+    KTimer t(c, "sigma_dets");
+    if (S->block) spmv(S->block, CF_F64, x, y);
+    DetLookupArgs a{};
+    a.r.n = sp->n;
+    a.r.nup = sp->ham->nup;
+    a.r.ndown = sp->ham->ndown;
+    a.r.up = sp->dup.p;
+    a.r.down = sp->ddown.p;
+    a.r.H = sp->ham->dev();
+    a.skey = S->skey.p;
+    a.sidx = S->sidx.p;
+    a.L = sp->ham->L;
+    launch_det_sigma_lookup(a, x, y, S->stored, c->stream);
+}
+
 void sigma_apply(cf_sigma* S, int prec, const double* x, double* y) {
     cf_space* sp = S->sp;
     CF_REQUIRE(S->version == sp->version,
                "the space's Hamiltonian changed after this sigma was created: create the sigma again");
+    if (S->dets) {
+        dets_sigma_apply(S, x, y);  // FP64 elements evaluated on the fly in either precision
+        return;
+    }
     cf_ctx* c = sp->ham->ctx;
     const int m = (int)sp->db, nn = (int)sp->da;
     KTimer t(c, prec == CF_F32 ? "sigma_f32" : "sigma_f64");
@@ -1933,8 +1957,56 @@ int cf_sigma_destroy(cf_sigma* s) {
     API_BEGIN
     if (s) {
         s->sp->ham->ctx->use();
+        delete s->block;
+        delete s->block_sp;
         delete s;
     }
+    API_END
+}
+
+int cf_dets_sigma_create(cf_space* sp, uint64_t stored_rows, cf_sigma** out) {
+    API_BEGIN
+    CF_REQUIRE(sp && out, "NULL argument");
+    CF_REQUIRE(sp->kind == CF_DETS, "cf_dets_sigma_create is for determinant-list spaces");
+    CF_REQUIRE(sp->ham->L <= 32, "the on-the-fly determinant-list product supports L <= 32");
+    CF_REQUIRE(stored_rows <= sp->n, "stored_rows exceeds the number of determinants");
+    cf_ctx* c = sp->ham->ctx;
+    c->use();
+    std::unique_ptr<cf_sigma> S(new cf_sigma());
+    S->sp = sp;
+    S->version = sp->version;
+    S->dets = true;
+    S->stored = stored_rows;
+    S->skey.alloc(sp->n);
+    S->sidx.alloc(sp->n);
+    sort_det_keys(sp->n, sp->dup.p, sp->ddown.p, S->skey.p, S->sidx.p, 32 + sp->ham->L, c->stream);
+    if (stored_rows > 0) {  // the leading block as its own list, built like FCI_File's matrix
+        std::unique_ptr<cf_space> b(new cf_space());
+        b->ham = sp->ham;
+        b->kind = CF_DETS;
+        b->n = b->da = stored_rows;
+        b->db = 1;
+        b->version = sp->version;
+        b->dup.alloc(stored_rows);
+        b->ddown.alloc(stored_rows);
+        CF_HIP(hipMemcpyAsync(b->dup.p, sp->dup.p, stored_rows * sizeof(str_t), hipMemcpyDeviceToDevice, c->stream));
+        CF_HIP(hipMemcpyAsync(b->ddown.p, sp->ddown.p, stored_rows * sizeof(str_t), hipMemcpyDeviceToDevice,
+                              c->stream));
+        std::unique_ptr<cf_crs> m(new cf_crs());
+        m->sp = b.get();
+        m->n = stored_rows;
+        m->zero = 1e-14;  // the reference's _zero (SparseMatrix_CRS.h:116)
+        m->version = b->version;
+        build_crs_arrays(m.get());
+        // the row-gather choice of cf_crs_build for FP64 values
+        m->auto64.kernel = m->auto32.kernel = 1;
+        m->auto64.lanes = m->auto32.lanes = m->lpr;
+        if (m->n && (double)m->nnz / (double)m->n >= 512) m->auto64 = SpmvChoice{3, 16, 2, {}, {}};
+        S->block_sp = b.release();
+        S->block = m.release();
+    }
+    CF_HIP(hipStreamSynchronize(c->stream));
+    *out = S.release();
     API_END
 }
 

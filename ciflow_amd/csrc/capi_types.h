@@ -177,6 +177,15 @@ struct cf_sigma {
     DBuf<float> vpair32, vself32;
     SigmaShape shape;
     uint64_t version = 0;  // space Hamiltonian version the tables were made for
+    // determinant-list spaces (CI_Big): the product on the fly from the
+    // sorted keys, plus the stored CRS of the leading `stored` x `stored`
+    // block (a space of the first `stored` determinants; both owned)
+    bool dets = false;
+    uint64_t stored = 0;
+    DBuf<uint64_t> skey;
+    DBuf<uint32_t> sidx;
+    cf_space* block_sp = nullptr;
+    cf_crs* block = nullptr;
 };
 
 namespace cf {

@@ -112,6 +112,9 @@ struct DetLookupArgs {
     int L = 0;
 };
 void launch_det_rows_lookup(const DetLookupArgs& a, bool fill, hipStream_t stream);
+// y = H x computed on the fly from the sorted keys (all entries but those of
+// the leading m x m block; rows < m add to y, the others overwrite it).
+void launch_det_sigma_lookup(const DetLookupArgs& a, const double* x, double* y, uint64_t m, hipStream_t stream);
 // bits: significant key bits (32 + L)
 void sort_det_keys(uint64_t n, const str_t* up, const str_t* down, uint64_t* skey, uint32_t* sidx, int bits,
                    hipStream_t stream);

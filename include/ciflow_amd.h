@@ -297,10 +297,24 @@ int cf_sigma_destroy(cf_sigma* s);
  * per workgroup (tuning; no reference counterpart). */
 int cf_sigma_set_threads(cf_sigma* s, int threads);
 /* Alpha-beta kernel (tuning; no reference counterpart): 1 = stage each alpha
- * action's V slice and C row, 2 (default) = per-Hamiltonian slice tables with
- * the next action's slice and row prefetched into registers (shapes it does
- * not cover run kernel 1). */
+ * action's V slice and C row, 2 = per-Hamiltonian slice tables with the next
+ * action's slice and row prefetched into registers (shapes it does not cover
+ * run kernel 1), 3 (default) = kernel 2 with packed LDS offsets and exact
+ * beta-list lengths (shapes without an instance run kernel 2). */
 int cf_sigma_set_kernel(cf_sigma* s, int kernel);
+/* ---- CI_Big (CIMethod.h:176-189, CIMethod.cpp:591-609, 943-1116, 1118-1142):
+ *      a determinant-list space (cf_space_create_dets, L <= 32) whose matrix
+ *      is not stored, or only its leading stored_rows x stored_rows block
+ *      (the reference keeps as many leading rows as its memory budget
+ *      allows).  The block is built as FCI_File builds its matrix (drop rule
+ *      |v| <= 1e-14); every other element is evaluated in each product from
+ *      the sorted determinant keys (at most two excitations), without the
+ *      drop rule, as CI_Big::mvprod does.  cf_sigma_apply[_dev] and
+ *      cf_solve_direct take the handle (FP64 elements in either value
+ *      precision; the solver defaults to Lanczos, there is no diagonal);
+ *      cf_rdm on the space gives DensFILE's RDMs.  stored_rows = 0: nothing
+ *      stored; stored_rows = n: the whole matrix stored. */
+int cf_dets_sigma_create(cf_space* sp, uint64_t stored_rows, cf_sigma** out);
 /* y = H x, device pointers on the context stream; value_prec CF_F32 uses
  * FP32 operands with FP64 accumulation of the alpha-beta part. */
 int cf_sigma_apply_dev(cf_sigma* s, int value_prec, const double* dx, double* dy);
