@@ -538,6 +538,72 @@ def test_config3_doci_L24_rows_bit_exact():
         assert np.array_equal(row, ref.row) and np.array_equal(col, ref.col) and np.array_equal(val, ref.data)
 
 
+def _row_reference(h, V, na, nb, u_r, d_r, cand, x_of):
+    """sum_j H_rj x_j over candidate determinants (u_j, d_j) with the oracle's
+    get_ham_element, and sum_j |H_rj x_j| (the rounding scale)."""
+    acc = scale = 0.0
+    for uj, dj in cand:
+        v = O.ham_element(h, V, na, nb, int(u_r), int(d_r), int(uj), int(dj)) * x_of(uj, dj)
+        acc += v
+        scale += abs(v)
+    return acc, scale
+
+
+def test_config2_default_product_sampled_rows_match_oracle_elements():
+    """FCI L=12 6a6b seed 1 at full size: the default FP64 product (the
+    position-major ELL kernel) and the FP32-value one on sampled rows against
+    the oracle's get_ham_element summed over every determinant within a double
+    excitation (found by popcount over the whole space, independent of the
+    GPU's pattern)."""
+    L, na, nb = 12, 6, 6
+    h, V = CA.gen_integrals(L, 1)
+    cim = CA.FCI(CA.Hamiltonian(h, V, na, nb), value_prec=CA.CF_F32)
+    m = cim.get_mat()
+    assert m.spmv_bytes(CA.CF_F64)[0] == 4
+    strs = O.enumerate_strings(L, na)
+    d = len(strs)
+    rank = {int(s): k for k, s in enumerate(strs)}
+    x = np.random.default_rng(12).standard_normal(cim.get_dim())
+    y64 = m.mvprod(x, CA.CF_F64)
+    y32 = m.mvprod(x, CA.CF_F32)
+    say("config2 product rows")
+    for r in (0, 123457, 500000, 853775):
+        ia, ib = divmod(r, d)
+        ca = [strs[j] for j in _connected(strs, strs[ia], 2)]
+        cb = [strs[j] for j in _connected(strs, strs[ib], 2)]
+        cand = [(ua, ub) for ua in ca for ub in cb
+                if bin(int(strs[ia]) ^ int(ua)).count("1") + bin(int(strs[ib]) ^ int(ub)).count("1") <= 4]
+        acc, scale = _row_reference(h, V, na, nb, strs[ia], strs[ib], cand,
+                                    lambda u, v: x[rank[int(u)] * d + rank[int(v)]])
+        assert abs(y64[r] - acc) <= 1e-13 * scale, (r, y64[r], acc)
+        assert abs(y32[r] - acc) <= 1e-6 * scale, (r, y32[r], acc)
+
+
+def test_config3_default_product_sampled_rows_match_oracle_elements():
+    """DOCI L=24 12 pairs seed 2 at full size: the default product (the
+    block-staged kernel, which stores no columns) on sampled rows against the
+    oracle's get_ham_element between the closed-shell determinants (u, u) and
+    (v, v) -- DOCI's diagonal and pair hop V(j,j,k,k) (CIMethod.cpp:611-660) --
+    summed over every pair string one hop away (popcount over all 2,704,156
+    strings)."""
+    L, npair = 24, 12
+    h, V = CA.gen_integrals(L, 2)
+    cim = CA.DOCI(CA.Hamiltonian(h, V, npair, npair))
+    m = cim.get_mat()
+    assert m.spmv_bytes(CA.CF_F64)[0] == 6
+    strs = O.enumerate_strings(L, npair)
+    x = np.random.default_rng(24).standard_normal(cim.get_dim())
+    y = m.mvprod(x, CA.CF_F64)
+    say("config3 product rows")
+    for r in (0, 999999, 1352078, 2704155):
+        u = strs[r]
+        js = np.nonzero(np.bitwise_count(strs ^ u) <= 2)[0]
+        assert len(js) == 1 + npair * (L - npair)
+        acc, scale = _row_reference(h, V, npair, npair, u, u, [(strs[j], strs[j]) for j in js],
+                                    lambda a, b: x[int(np.searchsorted(strs, a))])
+        assert abs(y[r] - acc) <= 1e-13 * scale, (r, y[r], acc)
+
+
 # ---- (e) integral rotation on FP64 MFMA ---------------------------------------
 @pytest.mark.parametrize("L", [4, 7, 12, 20])
 def test_rotate_integrals_match_oracle(L):
