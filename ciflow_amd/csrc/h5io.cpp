@@ -32,6 +32,7 @@
 // (ijkl, jilk, kjil, jkli, ilkj, lijk, klij, lkji -- FourIndex::getPointer).
 #include <hdf5.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -146,6 +147,7 @@ hsize_t dataset_len(hid_t ds) {
     Hid space(H5Dget_space(ds), H5Sclose, "dataset space");
     const int rank = H5Sget_simple_extent_ndims(space);
     if (rank == 0) return 1;
+    if (rank < 0 || rank > 4) throw H5Error("dataset of unexpected rank");
     hsize_t dims[4] = {0, 0, 0, 0};
     chk(H5Sget_simple_extent_dims(space, dims, nullptr), "dataset dims");
     hsize_t n = 1;
@@ -179,6 +181,18 @@ int fail(int code, const std::string& m) {
     return code;
 }
 
+// Silences the HDF5 library's error stack printing for one call and restores
+// the caller's handler afterwards (failures come back as status codes).
+struct QuietHdf5 {
+    H5E_auto2_t func = nullptr;
+    void* data = nullptr;
+    QuietHdf5() {
+        H5Eget_auto2(H5E_DEFAULT, &func, &data);
+        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+    }
+    ~QuietHdf5() { H5Eset_auto2(H5E_DEFAULT, func, data); }
+};
+
 }  // namespace
 
 extern "C" {
@@ -189,7 +203,7 @@ int cf_h5_read_hamiltonian(const char* path, int* L_out, int* nup, int* ndown, d
                            int* orb2irrep, double* h, double* V, double* overlap, int* has_overlap) {
     try {
         if (!path || !L_out) return fail(1, "NULL argument");
-        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        QuietHdf5 quiet;
         Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose, std::string("open ") + path);
         Hid data(H5Gopen2(file, "/Data", H5P_DEFAULT), H5Gclose, "/Data");
         const int L = read_all<int>(data, "L", H5T_NATIVE_INT).at(0);
@@ -265,7 +279,7 @@ int cf_h5_save_hamiltonian(const char* path, int L, int nup, int ndown, double e
                            const int* orb2irrep, const double* h, const double* V, const double* overlap) {
     try {
         if (!path || !h || !V || L < 1) return fail(1, "NULL argument or L < 1");
-        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        QuietHdf5 quiet;
         const int nI = n_irreps(ngroup);
         std::vector<int> irr(L, 0);
         if (orb2irrep) std::memcpy(irr.data(), orb2irrep, L * sizeof(int));
@@ -273,6 +287,31 @@ int cf_h5_save_hamiltonian(const char* path, int L, int nup, int ndown, double e
         const auto g = irrep_orbitals(L, nI, irr.data());
         std::vector<int> sizes(nI);
         for (int I = 0; I < nI; ++I) sizes[I] = (int)g[I].size();
+        // the packed format holds h's upper triangle and one V element per
+        // eight permutations: refuse what it would lose
+        {
+            double hmax = 0.0;
+            for (int q = 0; q < L * L; ++q) hmax = std::max(hmax, std::fabs(h[q]));
+            const double tol = 1e-12 * std::max(1.0, hmax);  // rotated integrals are symmetric to rounding
+            for (int a = 0; a < L; ++a)
+                for (int b = 0; b < a; ++b)
+                    if (std::fabs(h[a * L + b] - h[b * L + a]) > tol) return fail(1, "h is not symmetric");
+        }
+        {
+            double vmax = 0.0;
+            for (size_t q = 0; q < (size_t)L * L * L * L; ++q) vmax = std::max(vmax, std::fabs(V[q]));
+            const double tol = 1e-12 * std::max(1.0, vmax);
+            auto at = [&](int p, int q, int r, int t) { return V[(((size_t)p * L + q) * L + r) * L + t]; };
+            for (int a = 0; a < L; ++a)
+                for (int b = 0; b < L; ++b)
+                    for (int c = 0; c < L; ++c)
+                        for (int d = 0; d < L; ++d) {
+                            const double v = at(a, b, c, d);
+                            if (std::fabs(at(b, a, d, c) - v) > tol || std::fabs(at(c, b, a, d) - v) > tol ||
+                                std::fabs(at(a, d, c, b) - v) > tol)
+                                return fail(1, "V lacks the eight-fold permutation symmetry of real orbitals");
+                        }
+        }
         // the integrals must respect the point group: nothing nonzero outside the stored blocks
         for (int a = 0; a < L; ++a)
             for (int b = 0; b < L; ++b)
@@ -332,7 +371,7 @@ int cf_h5_save_hamiltonian(const char* path, int L, int nup, int ndown, double e
 int cf_h5_read_unitary(const char* path, int L, int ngroup, const int* orb2irrep, double* U) {
     try {
         if (!path || !U || L < 1) return fail(1, "NULL argument or L < 1");
-        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        QuietHdf5 quiet;
         const int nI = n_irreps(ngroup);
         std::vector<int> irr(L, 0);
         if (orb2irrep) std::memcpy(irr.data(), orb2irrep, L * sizeof(int));
@@ -359,7 +398,7 @@ int cf_h5_read_unitary(const char* path, int L, int ngroup, const int* orb2irrep
 int cf_h5_save_unitary(const char* path, int L, int ngroup, const int* orb2irrep, const double* U) {
     try {
         if (!path || !U || L < 1) return fail(1, "NULL argument or L < 1");
-        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        QuietHdf5 quiet;
         const int nI = n_irreps(ngroup);
         std::vector<int> irr(L, 0);
         if (orb2irrep) std::memcpy(irr.data(), orb2irrep, L * sizeof(int));
@@ -392,7 +431,7 @@ int cf_h5_write_crs(const char* path, const char* name, int append, uint64_t n, 
         if (!path || !name || !row || (nnz && (!col || !data))) return fail(1, "NULL argument");
         if (n >= (1ull << 32) || nnz >= (1ull << 32)) return fail(1, "the reference's CRS holds 32-bit sizes");
         if (row[n] != nnz) return fail(1, "row[n] != nnz");
-        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        QuietHdf5 quiet;
         Hid file(append ? H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT)
                         : H5Fcreate(path, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT),
                  H5Fclose, std::string("open ") + path);
@@ -426,7 +465,7 @@ int cf_h5_read_crs(const char* path, const char* name, uint64_t* n, uint64_t* nn
                    double* data) {
     try {
         if (!path || !name || !n || !nnz) return fail(1, "NULL argument");
-        H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+        QuietHdf5 quiet;
         Hid file(H5Fopen(path, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose, std::string("open ") + path);
         Hid grp(H5Gopen2(file, name, H5P_DEFAULT), H5Gclose, std::string("group ") + name);
         unsigned nn = 0, size = 0;
@@ -438,13 +477,15 @@ int cf_h5_read_crs(const char* path, const char* name, uint64_t* n, uint64_t* nn
             Hid ds(H5Dopen2(grp, "data", H5P_DEFAULT), H5Dclose, "data");
             Hid a(H5Aopen(ds, "size", H5P_DEFAULT), H5Aclose, "data size");
             chk(H5Aread(a, H5T_NATIVE_UINT, &size), "read data size");
+            if (dataset_len(ds) != (hsize_t)size) return fail(1, "data does not hold its size attribute's count");
             if (data && size) chk(H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, data), "read data");
         }
         *n = nn;
         *nnz = size;
-        if (col && size) {
+        {
             Hid ds(H5Dopen2(grp, "col", H5P_DEFAULT), H5Dclose, "col");
-            chk(H5Dread(ds, H5T_NATIVE_UINT, H5S_ALL, H5S_ALL, H5P_DEFAULT, col), "read col");
+            if (dataset_len(ds) != (hsize_t)size) return fail(1, "col does not hold data's count");
+            if (col && size) chk(H5Dread(ds, H5T_NATIVE_UINT, H5S_ALL, H5S_ALL, H5P_DEFAULT, col), "read col");
         }
         if (row) {
             Hid ds(H5Dopen2(grp, "row", H5P_DEFAULT), H5Dclose, "row");

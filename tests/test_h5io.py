@@ -236,7 +236,8 @@ def test_save_refuses_what_the_packed_format_cannot_hold(tmp_path):
     h, V = random_integrals(irreps, 3)
     path = os.path.join(tmp_path, "bad.h5")
     bad = V.copy()
-    bad[0, 0, 0, 1] = 0.3  # irr 0^0 != 0^1
+    for perm in PERMS:  # irr 0^0 != 0^1, with all eight permutations
+        bad[tuple((0, 0, 0, 1)[p] for p in perm)] = 0.3
     with pytest.raises(h5.H5Error, match="point group"):
         h5.save_hamiltonian(path, h, bad, 1, 1, 0.0, 2, irreps)
     bad_h = h.copy()
@@ -245,6 +246,14 @@ def test_save_refuses_what_the_packed_format_cannot_hold(tmp_path):
         h5.save_hamiltonian(path, bad_h, V, 1, 1, 0.0, 2, irreps)
     with pytest.raises(h5.H5Error, match="out of range"):
         h5.save_hamiltonian(path, h, V, 1, 1, 0.0, 2, [0, 2, 0, 1])  # C2 has 2 irreps
+    asym = V.copy()
+    asym[0, 1, 0, 1] += 0.5  # breaks <ij|kl> = <kj|il> and the other permutations
+    with pytest.raises(h5.H5Error, match="eight-fold"):
+        h5.save_hamiltonian(path, h, asym, 1, 1, 0.0, 2, irreps)
+    hs = h.copy()
+    hs[0, 2] += 0.1
+    with pytest.raises(h5.H5Error, match="not symmetric"):
+        h5.save_hamiltonian(path, hs, V, 1, 1, 0.0, 2, irreps)
     with pytest.raises(h5.H5Error):
         h5.read_hamiltonian(os.path.join(tmp_path, "missing.h5"))
 
