@@ -75,6 +75,19 @@ def hdf5_prefix() -> str | None:
     return None
 
 
+def hdf5_link_args(prefix: str) -> list[str]:
+    """Linker arguments for the HDF5 C library under `prefix`: the library by
+    its path (no -L, so the compiler's own C++ runtime is the one linked) and a
+    run path that finds the system C++ runtime before anything else in the
+    HDF5 prefix (an HDF5 distribution may ship an older libstdc++)."""
+    libdir = os.path.join(prefix, "lib" if os.path.exists(os.path.join(prefix, "lib", "libhdf5.so")) else "lib64")
+    gxx = shutil.which("g++") or "g++"
+    r = subprocess.run([gxx, "-print-file-name=libstdc++.so"], capture_output=True, text=True)
+    cxx_dir = os.path.dirname(os.path.realpath(r.stdout.strip())) if r.returncode == 0 else ""
+    rpath = ":".join(d for d in (cxx_dir, libdir) if d)
+    return [os.path.join(libdir, "libhdf5.so"), f"-Wl,-rpath,{rpath}"]
+
+
 def build_h5(verbose: bool = True) -> str | None:
     """Build the host-only HDF5 file library libciflow_h5.so (no GPU code).
     Returns None, and builds nothing, where no HDF5 C library is installed."""
@@ -83,11 +96,9 @@ def build_h5(verbose: bool = True) -> str | None:
         if verbose:
             print("HDF5 C library not found: libciflow_h5.so not built")
         return None
-    libdir = os.path.join(prefix, "lib" if os.path.exists(os.path.join(prefix, "lib", "libhdf5.so")) else "lib64")
     tmp = H5_LIB + ".tmp"
     cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
-           "-I", os.path.join(prefix, "include"), os.path.join(CSRC, "h5io.cpp"), "-o", tmp,
-           f"-L{libdir}", "-lhdf5", f"-Wl,-rpath,{libdir}"]
+           "-I", os.path.join(prefix, "include"), os.path.join(CSRC, "h5io.cpp"), "-o", tmp] + hdf5_link_args(prefix)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"HDF5 library build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
