@@ -62,6 +62,11 @@ def parse():
     p.add_argument("--sigma-L", default="8,9,10,11,12,16", help="sigma: orbital counts (FCI, half filling)")
     p.add_argument("--batch-class", default="FCI11", help="batch: request class (FCI8..FCI11, DOCI17..DOCI20)")
     p.add_argument("--lanes", type=int, default=32, help="batch: requests per lock-step batch")
+    p.add_argument("--served-precision", choices=["auto", "f32", "f64"], default="auto",
+                   help="stream: per-request precision (auto: FP32 values + FP64 polish only where the "
+                        "operator stores matrix values, FP64 elsewhere)")
+    p.add_argument("--batch-precision", choices=["f32", "f64"], default="f32",
+                   help="batch: FP32 operator + FP64 polish (f32) or FP64 throughout (f64)")
     p.add_argument("--nel", type=int, default=6)
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--precision", choices=["f64", "f32"], default="f64")
@@ -342,7 +347,8 @@ def bench_stream(a, CA):
     slice_sz = NREQ / K
     nwarm = int(round(W * slice_sz))
     method = {"auto": CA.CF_SOLVER_AUTO, "lanczos": CA.CF_LANCZOS, "davidson": CA.CF_DAVIDSON}[a.solver]
-    served = dict(precision=CA.CF_F32, check_every=2, tol_energy=1e-9, direct_fci=True, method=method)
+    sprec = {"auto": CA.CF_PREC_AUTO, "f32": CA.CF_F32, "f64": CA.CF_F64}[a.served_precision]
+    served = dict(precision=sprec, check_every=2, tol_energy=1e-9, direct_fci=True, method=method)
     reqs = S.make_stream(NREQ, rate=0.0)
     warm = S.make_stream(nwarm, rate=0.0, first_seed=500000, mix_seed=99) if nwarm > 0 else []
     t0 = time.perf_counter()
@@ -408,6 +414,12 @@ def bench_stream(a, CA):
         cpu = cpu_baseline_stream(reqs, nmv_by_class)
         log(f"cpu baseline: {cpu['value']:.3f} solves/s (sample {cpu['sample_s']:.1f} s)")
     kinds = np.array([r.kind for r in reqs])
+    n_f32 = sum(1 for d in res if d["n_matvec_f32"] > 0)
+    dtype = ("f64" if n_f32 == 0 else "f32-values/f64-accumulate+f64-polish" if n_f32 == len(res)
+             else f"f64, f32-values/f64-accumulate+f64-polish on {n_f32} stored-CRS requests")
+    prec_note = {"auto": "per-request precision CF_PREC_AUTO (FP32 values + FP64 polish where the operator "
+                         "stores matrix values; none of this mix does, so FP64 throughout)",
+                 "f32": "FP32 operator + FP64 polish", "f64": "FP64"}[a.served_precision]
     return {
         "metric": METRIC,
         "value": rate,
@@ -419,11 +431,11 @@ def bench_stream(a, CA):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-values/f64-accumulate+f64-polish",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {"workload": ("served stream of rotated-integral ground-state solves: 70% DOCI L~U[12,20], "
                                 "30% FCI L~U[8,11] (on-the-fly sigma), half filling, seeds 1000+k, U=exp(K) "
-                                "K~N(0,0.05^2), FP32 values + FP64 polish, Kato-Temple energy stop 1e-9 Eh"),
+                                f"K~N(0,0.05^2), {prec_note}, Kato-Temple energy stop 1e-9 Eh"),
                    "solver": {"auto": ("DOCI <= 13,312 determinants: batched FP64 Lanczos, one workgroup per "
                                        "request (cf_solve_doci_batch); FCI and larger DOCI: Davidson"),
                               "lanczos": "thick-restart Lanczos", "davidson": "Davidson"}[a.solver],
@@ -568,11 +580,13 @@ def bench_fci12(a, CA):
 
 def bench_direct16(a, CA):
     """FCI L=16 8a8b seed 3 (165,636,900 determinants) on the on-the-fly sigma:
-    FP64 solve (Lanczos, ARPACK semantics) and FP32-sigma + FP64 refinement,
-    at most 40 Krylov vectors, then the 1-RDM of the ground state.  One step =
-    one FP32+refinement solve, by Davidson (--solver auto/davidson: residual
+    one step = one solve in --precision (f64, the default: FP64 sigma
+    throughout, measured faster here than the FP32 phase, whose sigma saves
+    only 12% and whose FP64 refinement restarts the basis; f32: FP32 sigma +
+    FP64 refinement), by Davidson (--solver auto/davidson: residual
     corrections preconditioned by the diagonal, at most 24 basis vectors) or
-    by Lanczos (--solver lanczos)."""
+    by Lanczos (--solver lanczos), then the 1-RDM of the ground state and,
+    unless --no-fp64, the plain FP64 Lanczos solve (ARPACK semantics)."""
     L, nel, seed = 16, 8, 3
     ctx = CA.Context(0)
     h, V = CA.gen_integrals(L, seed)
@@ -584,16 +598,18 @@ def bench_direct16(a, CA):
     n = d.n
     ctx.enable_timing(True)
     ctx.reset_timing()
-    log(f"direct FCI L={L}: n={n}, setup {t_setup:.2f} s; FP32+polish solve ...")
+    prec = CA.CF_F32 if a.precision == "f32" else CA.CF_F64
+    log(f"direct FCI L={L}: n={n}, setup {t_setup:.2f} s; {'FP32+polish' if prec == CA.CF_F32 else 'FP64'} "
+        f"solve ...")
     def prog(nmv, th, r):
         log(f"  {nmv} products, theta {th:.10f}, residual {r:.2e}, {time.perf_counter() - t0:.1f} s")
 
     method = CA.CF_LANCZOS if a.solver == "lanczos" else CA.CF_DAVIDSON
     t0 = time.perf_counter()
-    i32 = d.solve(precision=CA.CF_F32, ncv=39, progress=prog, method=method)
+    i32 = d.solve(precision=prec, ncv=39, progress=prog, method=method)
     t32 = time.perf_counter() - t0
     e32 = d.get_ci_energy()
-    log(f"  E={e32:.12f} matvec {i32.n_matvec}+{i32.n_matvec_f32} {t32:.1f} s")
+    log(f"  E={e32:.12f} matvec {i32.n_matvec}+{i32.n_matvec_f32} (FP64+FP32) {t32:.1f} s")
     n64, ms64 = ctx.kernel_time("sigma_f64")
     n32, ms32 = ctx.kernel_time("sigma_f32")
     t0 = time.perf_counter()
@@ -610,13 +626,13 @@ def bench_direct16(a, CA):
         e64 = d.get_ci_energy()
         n64b, ms64b = ctx.kernel_time("sigma_f64")
         out64 = {"energy": e64, "n_matvec": i64.n_matvec, "seconds": t64, "sigma_ms": ms64b / max(n64b, 1),
-                 "dE_f32_polish_vs_f64": abs(e32 - e64)}
+                 "dE_vs_fp64_lanczos": abs(e32 - e64)}
         log(f"  E={e64:.12f} matvec {i64.n_matvec} {t64:.1f} s")
     ctx.enable_timing(False)
     return {
         "metric": METRIC, "value": 1.0 / t32, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": t32 * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32-sigma/f64-accumulate+f64-polish", "data": "synthetic",
+        "dtype": "f32-sigma/f64-accumulate+f64-polish" if prec == CA.CF_F32 else "f64", "data": "synthetic",
         "config": {"workload": f"direct FCI L={L} {nel}a{nel}b seed {seed}, on-the-fly sigma, ncv 39",
                    "determinants": n, "solver": "lanczos" if method == CA.CF_LANCZOS else "davidson"},
         "energy": e32, "n_matvec": i32.n_matvec, "n_matvec_f32": i32.n_matvec_f32,
@@ -735,6 +751,7 @@ def bench_batch(a, CA):
     L = int(cls[4:] if kind == CA.CF_DOCI else cls[3:])
     K = a.steps if a.steps is not None else 3
     B = a.lanes
+    prec = CA.CF_F64 if a.batch_precision == "f64" else CA.CF_F32
     ctx = CA.Context(0)
 
     def batch(first):
@@ -742,7 +759,7 @@ def bench_batch(a, CA):
         for k in range(B):
             r = S.StreamRequest(k, kind, L, L // 2, first + k, 0.0)
             h, V, U = S.request_arrays(r)
-            reqs.append(dict(kind=kind, h=h, V=V, nup=L // 2, ndown=L // 2, U=U, precision=CA.CF_F32,
+            reqs.append(dict(kind=kind, h=h, V=V, nup=L // 2, ndown=L // 2, U=U, precision=prec,
                              tol_energy=1e-9, tag=first + k))
         return reqs
 
@@ -759,7 +776,8 @@ def bench_batch(a, CA):
     n64 = float(np.mean([r["n_matvec"] for r in flat]))
     log(f"batch {cls} x {B}: {dt / K * 1e3:.1f} ms per batch, {rate:.1f} solves/s, products {n32:.1f} + {n64:.1f}")
     return {"metric": "batched solves/s", "value": rate, "unit": "solves/s", "higher_is_better": True,
-            "workload": f"{B} {cls} requests per lock-step batch (cf_solve_batch), FP32 + FP64 polish, "
+            "workload": f"{B} {cls} requests per lock-step batch (cf_solve_batch), "
+                        f"{'FP32 + FP64 polish' if prec == CA.CF_F32 else 'FP64'}, "
                         f"energy stop 1e-9 Eh", "steps": K, "ms_per_batch": dt / K * 1e3,
             "mean_products": {"f32": n32, "f64": n64}, "n": flat[0]["n"]}
 

@@ -28,6 +28,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "ciflow_amd.h")
 CF_OK, CF_ERR_ARG, CF_ERR_HIP, CF_ERR_NOMEM, CF_ERR_NOCONV, CF_ERR_UNSUPPORTED, CF_ERR_STATE = range(7)
 CF_FCI, CF_DOCI, CF_DETS = 0, 1, 2
 CF_F64, CF_F32 = 0, 1
+CF_PREC_AUTO = 2  # served requests: FP32 values only where the operator stores them
 CF_LANCZOS, CF_DAVIDSON, CF_SOLVER_AUTO = 0, 1, 2
 
 _lib = None

@@ -638,6 +638,11 @@ extern "C" int cf_solve_batch(cf_ctx* ctx, int count, const cf_request* reqs, cf
     try {
         CF_REQUIRE(ctx && (count == 0 || (reqs && results)), "NULL argument");
         CF_REQUIRE(count >= 0, "negative count");
+        // CF_PREC_AUTO: FP64 (no operator of this batch stores matrix values)
+        std::vector<cf_request> resolved(reqs, reqs + count);
+        for (cf_request& q : resolved)
+            if (q.precision == CF_PREC_AUTO) q.precision = CF_F64;
+        reqs = resolved.data();
         // requests by shape, in request order
         std::map<uint64_t, std::vector<int>> groups;
         std::vector<uint64_t> order;

@@ -36,7 +36,13 @@ enum { CF_FCI = 0, CF_DOCI = 1, CF_DETS = 2 };
 /* Value storage of the CRS matrix and arithmetic of a product. */
 enum {
     CF_F64 = 0,     /* FP64 values (the reference's SparseMatrix_CRS data) */
-    CF_F32 = 1      /* FP32-stored values, FP64 accumulation */
+    CF_F32 = 1,     /* FP32-stored values, FP64 accumulation */
+    CF_PREC_AUTO = 2 /* served requests (cf_server_submit, cf_solve_batch, cf_request_footprint)
+                        only: CF_F32 where the request's operator reads stored matrix values (a
+                        stored CRS: the FP32 copy halves the product's bytes), CF_F64 where it
+                        does not (the on-the-fly sigma, the block-staged and one-launch DOCI
+                        products: measured equal or faster in FP64, the FP32 phase adding the
+                        products of the FP64 polish) */
 };
 
 typedef struct cf_ctx cf_ctx;
@@ -367,7 +373,7 @@ typedef struct {
     const double* V;    /* L^4 (copied at submit) */
     const double* U;    /* L*L rotation applied on the GPU first, or NULL */
     double econst;
-    int precision;      /* CF_F64 or CF_F32 (FP32 Lanczos + FP64 polish) */
+    int precision;      /* CF_F64, CF_F32 (FP32 operator + FP64 polish) or CF_PREC_AUTO */
     int ncv;            /* <= 0: 42 */
     double tol;         /* <= 0: 1e-12 */
     int check_every;    /* cf_solve_opts.check_every */
@@ -413,7 +419,7 @@ int cf_solve_doci_batch(cf_ctx* ctx, int count, const cf_request* reqs, cf_resul
  * requests per launch.  Each request runs the solver of cf_solve with
  * CF_DAVIDSON: precision CF_F32 an FP32-operand phase (tol_f32 of
  * cf_solve_opts_default, the tol_energy estimate, or a stall) then an FP64
- * polish; stops on tol (<= 0: 1e-12) or the tol_energy estimate; integrals
+ * polish (CF_PREC_AUTO: FP64 throughout); stops on tol (<= 0: 1e-12) or the tol_energy estimate; integrals
  * rotated by U (NULL: as given).  n_matvec counts the FP64 products,
  * n_matvec_f32 the FP32 ones.  results[i] belongs to reqs[i]; a request the
  * batch does not take gets an error status and leaves the others unaffected.

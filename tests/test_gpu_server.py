@@ -109,3 +109,45 @@ def test_failed_request_reports_error_and_others_finish():
     assert set(res) == {r.k for r in reqs} | {99}
     assert res[99]["status"] != 0 and res[99]["error"]
     assert all(res[r.k]["status"] == 0 for r in reqs)
+
+
+def test_precision_auto_resolves_per_route():
+    """CF_PREC_AUTO: FP32 values + FP64 polish for a stored-CRS request, FP64
+    for the on-the-fly sigma (lock-step batch and single solve), the one-launch
+    DOCI batch and the block-staged lock-step batch; every energy within 1e-8
+    Eh of the oracle's FP64 solve, and the admission footprint that of the
+    resolved precision."""
+    import time
+
+    cases = [  # kind, L, direct, method, FP32 phase expected
+        (CA.CF_FCI, 8, False, CA.CF_LANCZOS, True),
+        (CA.CF_FCI, 8, True, CA.CF_SOLVER_AUTO, False),
+        (CA.CF_FCI, 9, True, CA.CF_LANCZOS, False),
+        (CA.CF_DOCI, 12, False, CA.CF_SOLVER_AUTO, False),
+        (CA.CF_DOCI, 17, False, CA.CF_SOLVER_AUTO, False),
+    ]
+    reqs, arrs = [], []
+    for k, (kind, L, _, _, _) in enumerate(cases):
+        reqs.append(S.StreamRequest(k, kind, L, L // 2, 7000 + k, 0.0))
+        arrs.append(S.request_arrays(reqs[-1]))
+    results = {}
+    with CA.SolveServer(slots=2) as srv:
+        for r, (h, V, U), (_, _, direct, method, _) in zip(reqs, arrs, cases):
+            srv.submit(r.k, r.kind, h, V, r.n, r.n, U=U, precision=CA.CF_PREC_AUTO, tol_energy=1e-9, check_every=2,
+                       direct=direct, method=method)
+        deadline = time.perf_counter() + 240
+        while len(results) < len(cases):
+            for d in srv.poll():
+                results[d["tag"]] = d
+            assert time.perf_counter() < deadline, "served requests did not finish"
+            time.sleep(0.01)
+    for r, a, (kind, L, direct, method, f32) in zip(reqs, arrs, cases):
+        d = results[r.k]
+        say("auto", kind, L, direct, method, d["n_matvec"], d["n_matvec_f32"])
+        assert d["status"] == 0, d["error"]
+        assert (d["n_matvec_f32"] > 0) == f32, (kind, L, direct, method)
+        assert abs(d["energy"] - oracle_energy(r, a)) <= 1e-8
+    fp = CA.SolveServer.footprint
+    assert fp(CA.CF_FCI, 10, 5, 5, precision=CA.CF_PREC_AUTO) == fp(CA.CF_FCI, 10, 5, 5, precision=CA.CF_F32)
+    assert fp(CA.CF_FCI, 10, 5, 5, precision=CA.CF_PREC_AUTO, direct=True) == \
+        fp(CA.CF_FCI, 10, 5, 5, precision=CA.CF_F64, direct=True)
