@@ -3,9 +3,11 @@
 Default workload (--workload stream): the served stream of rotated-integral
 ground-state solves -- 70% DOCI with L uniform in [12, 20] and 30% FCI with L
 uniform in [8, 11], half filling, integrals from seed 1000+k rotated by
-U = exp(K), K antisymmetric with N(0, 0.05^2) entries.  Each request is
-served with FP32 matrix values (FP32 sigma for FCI) and an FP64 polish, and
-stops at a Kato-Temple energy bound of 1e-9 Eh.  The timed stream holds
+U = exp(K), K antisymmetric with N(0, 0.05^2) entries.  Each request's
+precision is CF_PREC_AUTO (--served-precision): FP32 values + FP64 polish
+where its operator stores matrix values, FP64 where it does not (every route
+of this mix: the on-the-fly sigma, the block-staged and one-launch DOCI
+products), and it stops at a Kato-Temple energy estimate of 1e-9 Eh.  The timed stream holds
 --requests (4,096) requests; one step = one slice of requests/steps
 consecutive requests, and the K slices are served back to back.
   * value: solves/s of the whole stream submitted at once (closed batch) to
@@ -743,8 +745,9 @@ def bench_rotate(a, CA):
 def bench_batch(a, CA):
     """Lock-step batched Davidson solves of one stream class (cf_solve_batch):
     --lanes requests of class --batch-class (the stream's rotated seeded
-    integrals, FP32 operator + FP64 polish, energy stop 1e-9 Eh) per batch, one
-    warm-up batch, then --steps timed batches of fresh requests."""
+    integrals, --batch-precision: FP32 operator + FP64 polish or FP64, energy
+    stop 1e-9 Eh) per batch, one warm-up batch, then --steps timed batches of
+    fresh requests."""
     from ciflow_amd import stream as S
     cls = a.batch_class.upper()
     kind = CA.CF_DOCI if cls.startswith("DOCI") else CA.CF_FCI
