@@ -1,4 +1,4 @@
-// The bindings INTEGRATION.md gives a CIFlow maintainer, compiled with g++
+// The bindings a CIFlow maintainer would add, compiled with g++
 // against include/ciflow_amd.h and linked to libciflow_amd.so:
 //   Route 1: a matrix class overriding the virtual mvprod(const double*, double*)
 //            seam (SparseMatrix_CRS.h:92,126) that forwards to cf_mvprod;

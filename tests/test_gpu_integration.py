@@ -1,4 +1,4 @@
-"""GPU test of the C++ bindings INTEGRATION.md gives (Routes 1, 2 and 3),
+"""GPU test of the C++ bindings a CIFlow maintainer would add (Routes 1, 2 and 3),
 compiled with g++ against include/ciflow_amd.h by __graft_entry__.build()
 (tests/integration/Makefile) and run as its own process: the RCI loop
 driving a GPU mvprod through a virtual mvprod seam, cf_solve, cf_rdm and the
