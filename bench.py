@@ -22,6 +22,8 @@ consecutive requests, and the K slices are served back to back.
   * roofline: the stored-CRS SpMV of FCI L=12 6a6b (1,553,018,544 nonzeros),
     timed with HIP events on the library stream during one FP64 solve.
 --workload fci12: FCI L=12 6a6b seed 1 build + solve per step.
+--workload spmv_dets: the stored SpMV of a CAS-like determinant list (--L 12:
+101,640 determinants; --L 16: 1,029,600).
 
 Prints ONE JSON line.  cpu_baseline times the oracle's restatement of the
 reference CPU path on a bounded sample and extrapolates (see cpu_baseline_*).
@@ -46,8 +48,8 @@ METRIC = "CI solves/s + p50/p99 latency at |dE|<=1e-8 Eh; SpMV HBM GB/s (% of pe
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "spmv_doci", "rotate",
-                                          "sigma", "batch"],
+    p.add_argument("--workload", choices=["stream", "fci12", "doci24", "direct16", "spmv", "spmv_doci", "spmv_dets",
+                                          "rotate", "sigma", "batch"],
                    default="stream")
     p.add_argument("--steps", type=int, default=None,
                    help="stream: slices of the --requests stream (16); fci12: solves (4)")
@@ -289,7 +291,8 @@ def spmv_roofline(CA, ctx, L=12, nel=6, seed=1):
     names = {1: "k_spmv (CRS row gather)", 3: "k_spmv_v4 (CRS row gather, vectorised streams)",
              4: "k_spmv_ell2 (FCI position-major complete rows, double-buffered LDS x blocks, FP64 values)",
              5: "k_spmv_doci (DOCI position-major pair-hop columns, pair elements in LDS)",
-             6: "k_spmv_doci_blk (DOCI block-staged, x blocks and pair elements in LDS)"}
+             6: "k_spmv_doci_blk (DOCI block-staged, x blocks and pair elements in LDS)",
+             7: "k_spmv_sell (sliced rows: 64-row chunks, interleaved entries)"}
     crs_bytes = nnz_full * 12 + (n + 1) * 8 + 16 * n
     out = {"bound": "hbm", "kernel": names.get(kern, str(kern)), "crs_equivalent_bytes": crs_bytes,
            "crs_equivalent_GBps": crs_bytes / (avg_ms * 1e-3) / 1e9,
@@ -680,6 +683,60 @@ def bench_spmv(a, CA, doci=False):
     return {"metric": METRIC, "workload": wl, "n": n, "nnz": nnz, **out}
 
 
+def bench_spmv_dets(a, CA):
+    """Stored-matrix SpMV over a determinant list (selected CI, FCI_File): a
+    CAS-like list of --L orbitals at half filling, the first N alpha strings
+    times every beta string in (up, down) order -- N = 110 at L=12 (101,640
+    determinants), 80 at L=16 (1,029,600), else about 10^5 determinants --
+    seeded integrals (seed 1).  Per value precision, --steps products (after
+    one warm-up) on the automatic kernel (the sliced rows) and on the
+    vectorised CRS row gather (16 lanes per row, unroll 2), HIP events on the
+    library stream; then three 2 GiB device-to-device copies (the reference
+    kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes)."""
+    from math import comb
+    L = a.L
+    ne = L // 2
+    d = comb(L, ne)
+    nalpha = {12: 110, 16: 80}.get(L, max(1, min(d, round(1e5 / d))))
+    ctx = CA.Context(0)
+    s = ctx.unrank(L, ne, 0, d)
+    up = np.repeat(s[:nalpha], d)
+    down = np.tile(s, nalpha)
+    h, V = CA.gen_integrals(L, 1)
+    t0 = time.perf_counter()
+    m = CA.FCI_File(CA.Hamiltonian(h, V, ne, ne, 0.0, ctx), (up, down), value_prec=CA.CF_F32).get_mat()
+    ctx.synchronize()
+    build_s = time.perf_counter() - t0
+    n, nnz, _ = m.info()
+    x = np.random.default_rng(0).standard_normal(n)
+    steps = a.steps or 5
+    names = {3: "k_spmv_v4", 7: "k_spmv_sell"}
+    out = {}
+    for prec, fam in ((CA.CF_F64, "spmv_f64"), (CA.CF_F32, "spmv_f32")):
+        for label, shape in (("auto", None), ("row_gather", (3, 16, 2))):
+            if shape:
+                m.set_spmv_shape(prec, *shape)
+            m.mvprod(x, prec)
+            ctx.enable_timing(True)
+            ctx.reset_timing()
+            for _ in range(steps):
+                m.mvprod(x, prec)
+            nl, ms = ctx.kernel_time(fam)
+            ctx.enable_timing(False)
+            kern, algo = m.spmv_bytes(prec)
+            gbs = algo / (ms / nl * 1e-3) / 1e9
+            out[f"{fam}_{label}"] = {"kernel": names.get(kern, str(kern)), "avg_launch_ms": ms / nl,
+                                     "algorithmic_bytes": algo, "GBps": gbs, "frac": gbs / HBM_PEAK_GBS}
+            log(f"{fam} {label} ({names.get(kern, kern)}): {ms / nl:.4f} ms {gbs:.0f} GB/s "
+                f"({gbs / HBM_PEAK_GBS * 100:.1f}%)")
+    copy_bytes = 2 << 30  # reference copies for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes
+    CA._check(CA.lib().cf_ctx_stream_copy(ctx.ptr, copy_bytes, 3))
+    out["copy_bytes_each_way"] = copy_bytes
+    return {"metric": METRIC, "workload": (f"determinant list: first {nalpha} alpha strings x all {d} beta "
+                                           f"strings of L={L}, half filling, seed 1"),
+            "n": n, "nnz": nnz, "build_s": build_s, "steps": steps, **out}
+
+
 def bench_sigma(a, CA):
     """On-the-fly FCI sigma alone (CI_Big::mvprod's role): for each L in
     --sigma-L, FCI floor(L/2) a + b electrons with seeded integrals (seed 3),
@@ -795,6 +852,7 @@ def main():
 
     run = {"stream": bench_stream, "fci12": bench_fci12, "doci24": bench_fci12, "direct16": bench_direct16,
            "spmv": bench_spmv, "spmv_doci": lambda a, CA: bench_spmv(a, CA, doci=True),
+           "spmv_dets": bench_spmv_dets,
            "rotate": bench_rotate, "sigma": bench_sigma, "batch": bench_batch}[a.workload]
     out = run(a, CA)
     print(json.dumps(out))

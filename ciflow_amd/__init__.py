@@ -483,7 +483,9 @@ class SparseMatrix_CRS_Sym:
         """0 automatic, 1 plain row gather, 2 FCI alpha-block staged, 3 vectorised
         row gather, 4 FCI position-major rows, 5 DOCI pair-hop columns
         (tile_rows = workgroup size for 4 and 5), 6 DOCI block-staged
-        (tile_rows = low orbitals)."""
+        (tile_rows = low orbitals), 7 sliced rows: 64-row chunks with
+        interleaved entries, a second copy of the matrix (tile_rows = waves per
+        chunk, 1, 2, 4, 8 or 16)."""
         _check(lib().cf_crs_set_spmv_kernel(self.ptr, kind, tile_rows))
 
     def set_spmv_shape(self, precision: int, kernel: int, p1: int, p2: int = 0):

@@ -135,11 +135,27 @@ void launch_spmv_f64(uint64_t n, const int64_t* rowptr, const uint32_t* col, con
                      double* y, int lanes_per_row, hipStream_t stream);
 void launch_spmv_f32(uint64_t n, const int64_t* rowptr, const uint32_t* col, const float* val, const double* x,
                      double* y, int lanes_per_row, hipStream_t stream);
-// Vectorised-stream row gather: (lanes_per_row, unroll) in {(16,2), (8,2), (16,1)}.
+// Vectorised-stream row gather: lanes_per_row 4, 8, 16 or 32, unroll 1 or 2.
 void launch_spmv_v4_f64(uint64_t n, const int64_t* rowptr, const uint32_t* col, const double* val, const double* x,
                         double* y, int lanes_per_row, int unroll, hipStream_t stream);
 void launch_spmv_v4_f32(uint64_t n, const int64_t* rowptr, const uint32_t* col, const float* val, const double* x,
                         double* y, int lanes_per_row, int unroll, hipStream_t stream);
+// Sliced row gather over the chunked copy (64 rows per chunk, quad-interleaved
+// entries; qoff = nchunk + 1 exclusive quad offsets): waves per chunk 1, 2, 4,
+// 8 or 16, unroll 1 or 2 (4 with 4, 8 or 16 waves).  A launch with few
+// chunks splits each into up to kSellMaxSplit quad ranges whose sums go
+// through part (part_cap doubles; splits needing more than part_cap are not
+// used) and a second pass.
+constexpr uint32_t kSellMaxSplit = 4;
+void launch_spmv_sell_f64(uint64_t n, const int64_t* qoff, const uint32_t* col, const double* val, const double* x,
+                          double* y, double* part, uint64_t part_cap, int waves, int unroll, hipStream_t stream);
+void launch_spmv_sell_f32(uint64_t n, const int64_t* qoff, const uint32_t* col, const float* val, const double* x,
+                          double* y, double* part, uint64_t part_cap, int waves, int unroll, hipStream_t stream);
+// Chunked copy of CRS rows: quads of each chunk's longest row, then the fill
+// (val32 / sval32 may be NULL).
+void launch_sell_quads(uint64_t n, const int64_t* rowptr, uint32_t* quads, hipStream_t stream);
+void launch_sell_fill(uint64_t n, const int64_t* rowptr, const uint32_t* col, const double* val, const float* val32,
+                      const int64_t* qoff, uint32_t* scol, double* sval, float* sval32, hipStream_t stream);
 
 // Krylov vector kernels (vectors are columns of a column-major n x k block).
 void launch_multi_dot(uint64_t n, int k, const double* Vb, const double* w, double* partial, int nblocks,

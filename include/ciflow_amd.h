@@ -203,12 +203,16 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * elements in LDS (DOCI matrices; 4 bytes per element; tile_rows > 0 =
  * workgroup size), 6 = DOCI block-staged (no stored columns; x blocks of the
  * rows sharing their high orbitals and the pair elements in LDS; tile_rows >
- * 0 = low orbitals w, 1..12, with 1..20 high orbitals).  The DOCI copies are
- * built with matrices of >= 40,000 rows, else on request. */
+ * 0 = low orbitals w, 1..12, with 1..20 high orbitals), 7 = sliced rows: a
+ * copy of the CRS rows in chunks of 64 rows with interleaved entries, so one
+ * load reads the same position of 64 neighbouring rows (any matrix; builds a
+ * second copy of the columns and values; tile_rows > 0 = waves per chunk,
+ * 1, 2, 4, 8 or 16).  The DOCI copies are built with matrices of >= 40,000
+ * rows, else on request. */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
-/* Kernel the next product with value_prec would run (1-6) and its algorithmic
- * HBM bytes (matrix stream + x + y; kernel 6: diagonal + x + y), the roofline
- * numerator. */
+/* Kernel the next product with value_prec would run (1-7) and its algorithmic
+ * HBM bytes (matrix stream + x + y; kernel 6: diagonal + x + y; kernel 7: the
+ * stored entries without the chunk padding), the roofline numerator. */
 int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* kernel);
 /* Launch shape of one SpMV kernel for one value precision (no reference
  * counterpart; the reference's mvprod has no launch shape).  kernel 1: p1 =
@@ -217,7 +221,10 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
  * <= 1024, 0 = default), p2 = beta strings per thread (1, 2 or 4, dividing
  * d_beta; 0 = the single-buffered kernel with p1 in 256/512/1024); kernel 5:
  * p1 = workgroup size (multiple of 64, <= 1024, 0 = 256), p2 = lanes per row
- * (1, 2, 4 or 8); kernel 6: p1 = low orbitals w (0 = min(L/2, 12)).  The shape
+ * (1, 2, 4 or 8); kernel 6: p1 = low orbitals w (0 = min(L/2, 12)); kernel 7:
+ * p1 = waves per 64-row chunk (1, 2, 4, 8, 16; 0 = 4), p2 = quads in flight
+ * per lane (1, 2, or 4 with 4-16 waves; 0 = 2), building the sliced copy if
+ * it does not exist.  The shape
  * becomes the automatic choice for value_prec and the matrix returns to
  * automatic selection (kernel 0). */
 int cf_crs_set_spmv_shape(cf_crs* m, int value_prec, int kernel, int p1, int p2);
@@ -228,7 +235,9 @@ typedef struct {
 } cf_tune_entry;
 /* Autotuner (no reference counterpart): times every launch shape the matrix
  * supports for value_prec (row gather lanes per row; vectorised lanes x
- * unroll; ELL workgroup size x strings per thread when the ELL copy exists)
+ * unroll; ELL workgroup size x strings per thread when the ELL copy exists;
+ * DOCI hop and block shapes; sliced rows, waves x unroll, for determinant
+ * lists -- building the sliced copy -- or when that copy exists)
  * with HIP events on the context stream, one warm-up and reps timed launches
  * each, installs the fastest as the automatic choice for value_prec and
  * returns the table (at most cap entries written, *count = number timed). */

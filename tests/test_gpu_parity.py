@@ -258,6 +258,110 @@ def test_det_list_beyond_32_orbitals_bit_exact():
     check_det_crs(cim, O.build_crs_dets(h, V, na, nb, up, down))
 
 
+
+def _cas_block(L, n_alpha, seed, repeats=()):
+    """The first n_alpha alpha strings times every beta string (half filling),
+    shuffled, with the listed determinants appended again."""
+    na = L // 2
+    S = O.enumerate_strings(L, na)
+    ia = np.repeat(np.arange(n_alpha), len(S))
+    ib = np.tile(np.arange(len(S)), n_alpha)
+    perm = np.random.default_rng(seed).permutation(len(ia))
+    up, down = S[ia[perm]], S[ib[perm]]
+    rep = list(repeats)
+    return np.concatenate([up, up[rep]]), np.concatenate([down, down[rep]])
+
+
+@pytest.mark.parametrize("case", ["beh2_full_list", "be_gkci100", "cas_block_L10", "fci_L8_4a4b_config1",
+                                  "doci_L10_5", "cas_block_L12"])
+def test_sliced_spmv_matches_oracle(case):
+    """Kernel 7 (rows in chunks of 64 with interleaved entries, each chunk
+    padded to its longest row; launches with few chunks split each into quad
+    ranges added by a second pass): every waves x unroll shape against the
+    oracle's product (FP64 values, 1e-12 relative; for the 31,416-determinant
+    block, whose all-pairs oracle build is slow, the vectorised row gather on
+    the same values, 1e-13) and against the scalar row gather on the same FP32
+    values; ragged rows (a shuffled CAS-like block with repeated determinants,
+    D2h zeros dropped), last chunks with fewer than 64 rows (100 and 1,225
+    determinants), repeated products bitwise equal."""
+    say("sliced spmv", case)
+    if case in DET_FILES:
+        npz, dets = DET_FILES[case]
+        h, V, na, nb, ec = fixture(npz)
+        up, down = O.read_dets(os.path.join(GOLD, dets))
+        cim = CA.FCI_File(CA.Hamiltonian(h, V, na, nb, ec), (up, down), value_prec=CA.CF_F32)
+        ref = O.build_crs_dets(h, V, na, nb, up, down)
+    elif case == "cas_block_L10":
+        h, V = CA.gen_integrals(10, 23)
+        up, down = _cas_block(10, 37, 23, repeats=(3, 4000, 9000))
+        cim = CA.FCI_File(CA.Hamiltonian(h, V, 5, 5), (up, down), value_prec=CA.CF_F32)
+        ref = O.build_crs_dets(h, V, 5, 5, up, down)
+    elif case == "cas_block_L12":
+        h, V = CA.gen_integrals(12, 24)
+        up, down = _cas_block(12, 34, 24)
+        cim = CA.FCI_File(CA.Hamiltonian(h, V, 6, 6), (up, down), value_prec=CA.CF_F32)
+        ref = None
+    else:
+        kind, make = CASES[case]
+        h, V, na, nb = make()
+        ham, cim, ref = build_pair(kind, h, V, na, nb, prec=CA.CF_F32)
+    m = cim.get_mat()
+    n = cim.get_dim()
+    x = np.random.default_rng(11).standard_normal(n)
+    m.set_spmv_kernel(3)
+    y_ref = O.mvprod_sym(ref, x) if ref is not None else m.mvprod(x)
+    tol = 1e-12 if ref is not None else 1e-13
+    m.set_spmv_kernel(1)
+    y1_32 = m.mvprod(x, CA.CF_F32)
+    m.set_spmv_kernel(7)  # builds the sliced copy
+    assert m.spmv_bytes(CA.CF_F64)[0] == 7
+    shapes = [(w, u) for w in (1, 2, 4, 8, 16) for u in (1, 2)] + [(4, 4), (8, 4), (16, 4)]
+    for w, u in shapes:
+        m.set_spmv_shape(CA.CF_F64, 7, w, u)
+        y = m.mvprod(x, CA.CF_F64)
+        assert np.abs(y - y_ref).max() <= tol * np.abs(y_ref).max(), (case, w, u)
+        assert np.array_equal(m.mvprod(x, CA.CF_F64), y)
+        m.set_spmv_shape(CA.CF_F32, 7, w, u)
+        y32 = m.mvprod(x, CA.CF_F32)
+        assert np.abs(y32 - y1_32).max() <= 1e-13 * np.abs(y1_32).max(), (case, w, u)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 7, 3, 1)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 7, 4, 3)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_shape(CA.CF_F64, 7, 2, 4)
+    with pytest.raises(CA.CIFlowError):
+        m.set_spmv_kernel(7, 32)
+
+
+def test_sliced_spmv_follows_set_ham_and_autotune():
+    """After new integrals on a resident determinant-list matrix the sliced
+    copy is rebuilt from the new CRS (a fresh build's product, 1e-13); the
+    autotuner times the sliced shapes for determinant lists and installs the
+    fastest, whose product agrees."""
+    L, na = 8, 4
+    h, V = CA.gen_integrals(L, 12)
+    h2, V2 = CA.gen_integrals(L, 13)
+    up, down = _cas_block(L, 30, 12)
+    sel = CA.FCI_File(CA.Hamiltonian(h, V, na, na), (up, down))
+    m = sel.get_mat()
+    x = np.random.default_rng(12).standard_normal(len(up))
+    m.set_spmv_kernel(7, 2)
+    y_old = m.mvprod(x)
+    sel.set_ham(CA.Hamiltonian(h2, V2, na, na))
+    ref = CA.FCI_File(CA.Hamiltonian(h2, V2, na, na), (up, down)).get_mat()
+    y_new = ref.mvprod(x)
+    y = sel.get_mat().mvprod(x)
+    assert sel.get_mat().spmv_bytes(CA.CF_F64)[0] == 7
+    assert np.abs(y - y_new).max() <= 1e-13 * np.abs(y_new).max()
+    assert np.abs(y_old - y_new).max() > 1e-3
+    tab = ref.autotune(CA.CF_F64, reps=2)
+    assert 7 in {k for k, _, _, _ in tab} and 3 in {k for k, _, _, _ in tab}
+    best = min(tab, key=lambda t: t[3])
+    assert ref.spmv_bytes(CA.CF_F64)[0] == best[0]
+    assert np.abs(ref.mvprod(x) - y_new).max() <= 1e-13 * np.abs(y_new).max()
+
+
 # ---- CI_Big: determinant lists with the product on the fly --------------------
 @pytest.mark.parametrize("case,stored", [("beh2_full_list", 0), ("beh2_full_list", 300),
                                          ("beh2_full_list", 1225), ("be_gkci100", 40)])
