@@ -1198,13 +1198,18 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
             m->auto64 = k5;
         }
     }
-    // Determinant lists: the sliced copy (kernel 7) when it fits in half of
-    // the free device memory.  Measured on MI355X (FP64 / FP32 values), CAS-like
-    // lists sorted by (up, down): 101,640 determinants of L=12 0.247 / 0.153 ms
-    // against 0.361 / 0.244 ms (vectorised row gather), 1,029,600 of L=16
-    // 4.76 / 2.93 ms against 6.81 / 4.57 ms; a random 10^5 subset of L=14
-    // 0.0157 / 0.0111 ms against 0.0171 / 0.0126 ms.
-    if (sp->kind == CF_DETS && make_sell(m.get(), true))
+    // Determinant lists, and FCI spaces left on the row gather (too few alpha
+    // strings for the position-major copy) with >= 8,192 rows, get the sliced
+    // copy (kernel 7) when it fits in half of the free device memory.  Measured
+    // on MI355X (FP64 / FP32 values): CAS-like lists sorted by (up, down),
+    // 101,640 determinants of L=12 0.247 / 0.153 ms against 0.361 / 0.244 ms
+    // (vectorised row gather), 1,029,600 of L=16 4.76 / 2.93 ms against
+    // 6.81 / 4.57 ms, a random 10^5 subset of L=14 0.0157 / 0.0111 ms against
+    // 0.0171 / 0.0126 ms; FCI L=10 (63,504 rows) 0.120 / 0.077 ms against
+    // 0.165 / 0.122 ms, FCI L=9 (15,876) 26 / 16 us against 27 / 24 us.  DOCI
+    // rows (65-82 entries at L=16-18) are too short to gain.
+    const bool gather = m->auto64.kernel == 1 || m->auto64.kernel == 3;
+    if ((sp->kind == CF_DETS || (sp->kind == CF_FCI && gather && m->n >= 8192)) && make_sell(m.get(), true))
         m->auto64 = m->auto32 = SpmvChoice{7, 16, 2, {}, {}};
     *out = m.release();
     API_END

@@ -916,7 +916,9 @@ def test_autotune_shapes_agree_and_fastest_is_installed(L, na, nb):
         m.set_spmv_kernel(1)
         y0 = m.mvprod(x, prec)
         tab = m.autotune(prec, reps=2)
-        assert {k for k, _, _, _ in tab} == {1, 3, 4}
+        kinds = {k for k, _, _, _ in tab}
+        # the sliced copy (kernel 7) is made at build for FCI spaces of >= 8,192 rows
+        assert kinds == ({1, 3, 4, 7} if cim.get_dim() >= 8192 else {1, 3, 4})
         best = min(tab, key=lambda t: t[3])
         assert m.spmv_bytes(prec)[0] == best[0]
         yb = m.mvprod(x, prec)
