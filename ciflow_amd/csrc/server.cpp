@@ -25,6 +25,8 @@
 #include "../../include/ciflow_amd.h"
 #include "common.h"
 #include "davbatch.h"
+#include "kernels.h"
+#include "spmv_ell.h"
 
 namespace {
 double now_s() {
@@ -112,6 +114,11 @@ uint64_t estimate_footprint(const cf_request& r) {
     const uint64_t bytes_nnz = r.precision == CF_F32 ? 16 : 12;
     uint64_t crs = n * row * bytes_nnz + (n + 1) * 8 + n * 4;
     if (r.kind == CF_DOCI) crs += n * (row - 1) * 4;  // hop columns of the DOCI SpMV
+    // FCI spaces without the position-major copy keep a sliced copy of the
+    // rows (cf_crs_build, when it fits): the entries again (padding to 4 per
+    // row), chunk offsets and the split launches' range sums
+    if (r.kind == CF_FCI && !(da >= 400 && db <= cf::kEllMaxDb) && n >= 8192)
+        crs += n * (row + 3) * bytes_nnz + (n / 64 + 2) * 8 + (uint64_t)cf::kSellMaxSplit * n * 8;
     const uint64_t krylov = (uint64_t)(ncv + 3) * n * 8;
     return crs + krylov + ints + (64ull << 20);
 }

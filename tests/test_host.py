@@ -129,6 +129,9 @@ def test_request_footprint_and_argument_checks():
     # FCI L=11 5a5b: 213,444 rows of 1,261 entries at 16 B (f64 + f32 value, u32 column)
     assert 4.2e9 < big < 4.6e9
     assert fp(CA.CF_FCI, 11, 5, 5, CA.CF_F64) < big
+    # FCI L=10 5a5b (252 alpha strings, no position-major copy): 63,504 rows of
+    # 876 entries, stored twice (CRS and the sliced copy)
+    assert fp(CA.CF_FCI, 10, 5, 5, CA.CF_F64) > 2 * 63504 * 876 * 12
     with pytest.raises(CA.CIFlowError):
         fp(CA.CF_FCI, 8, 9, 4)
 
