@@ -358,25 +358,28 @@ __global__ void __launch_bounds__(256) k_bd_same_spin(BdView b, uint64_t da, uin
     const uint64_t nb1 = (da + kSsK - 1) / kSsK, nb2 = (db + kSsK - 1) / kSsK, nbt = nb1 + nb2;
     // thread t stages four A and four B operand elements of a block:
     //   H_alpha C: A[i][k] = Ha[k][i] (Ha symmetric), B[k][j] = C[k][j];
-    //   C H_beta:  A[i][k] = C[i][k], B[k][j] = Hb[k][j]
-    const int ka = t >> 4, c4 = (t & 15) * 4;  // row-major staging: k row ka, columns c4..c4+3
-    const int i2 = t >> 2, k4 = (t & 3) * 4;   // C H_beta A operand: row i2, k k4..k4+3
+    //   C H_beta:  A[i][k] = C[i][k], B[k][j] = Hb[k][j].
+    // Row-major staging: k row ka, columns cl + 16 e, so 16 lanes read and
+    // store 16 consecutive doubles; the C H_beta A operand: row it of C, k
+    // rows kr + e, so a wavefront stores 64 consecutive doubles of one k row.
+    const int ka = t >> 4, cl = t & 15;
+    const int it = t & 63, kr = 4 * (t >> 6);
     double ra[4], rb[4];
     auto fetch = [&](uint64_t kb) {
         if (kb < nb1) {
             const uint64_t kk = kb * kSsK + ka;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const uint64_t i = Ia0 + c4 + e, j = Ib0 + c4 + e;
+                const uint64_t i = Ia0 + cl + 16 * e, j = Ib0 + cl + 16 * e;
                 ra[e] = kk < da && i < da ? Ha[kk * da + i] : 0.0;
                 rb[e] = kk < da && j < db ? C[kk * db + j] : 0.0;
             }
         } else {
             const uint64_t k0 = (kb - nb1) * kSsK;
-            const uint64_t i = Ia0 + i2, kk = k0 + ka;
+            const uint64_t i = Ia0 + it, kk = k0 + ka;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const uint64_t ke = k0 + k4 + e, j = Ib0 + c4 + e;
+                const uint64_t ke = k0 + kr + e, j = Ib0 + cl + 16 * e;
                 ra[e] = i < da && ke < db ? C[i * db + ke] : 0.0;
                 rb[e] = kk < db && j < db ? Hb[kk * db + j] : 0.0;
             }
@@ -385,13 +388,13 @@ __global__ void __launch_bounds__(256) k_bd_same_spin(BdView b, uint64_t da, uin
     auto stash = [&](uint64_t kb, int buf) {
         if (kb < nb1) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) sa[buf][ka][c4 + e] = ra[e];
+            for (int e = 0; e < 4; ++e) sa[buf][ka][cl + 16 * e] = ra[e];
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) sa[buf][k4 + e][i2] = ra[e];
+            for (int e = 0; e < 4; ++e) sa[buf][kr + e][it] = ra[e];
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) sb[buf][ka][c4 + e] = rb[e];
+        for (int e = 0; e < 4; ++e) sb[buf][ka][cl + 16 * e] = rb[e];
     };
     v4d acc00 = {0.0, 0.0, 0.0, 0.0}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
     // a quadrant wholly past da or db computes nothing (it still stages operands)
