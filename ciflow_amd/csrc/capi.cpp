@@ -1434,10 +1434,11 @@ int cf_crs_autotune(cf_crs* m, int value_prec, int reps, cf_tune_entry* table, i
         for (int t : {128, 256, 512})
             for (int u : {1, 2, 4, 8}) cand.push_back(make_choice(5, t, u, db, value_prec));
     if (m->blk_ready) cand.push_back(make_choice(6, 0, 0, db, value_prec));
-    // the sliced copy doubles the matrix memory: timed for determinant lists
-    // (whose only other kernels are the row gathers) or when it already exists
-    if (m->sp->kind == CF_DETS || m->sell_ready) {
-        make_sell(m);
+    // the sliced copy doubles the matrix memory: timed when it exists (refilled
+    // if new integrals made it stale), or for determinant lists (whose only
+    // other kernels are the row gathers) when it fits in half of the free
+    // device memory
+    if ((m->scol.p || m->sp->kind == CF_DETS) && make_sell(m, true)) {
         for (int w : {1, 2, 4, 8, 16})
             for (int u : {1, 2, 4})
                 if (u < 4 || w >= 4) cand.push_back(make_choice(7, w, u, db, value_prec));

@@ -335,10 +335,11 @@ def test_sliced_spmv_matches_oracle(case):
 
 
 def test_sliced_spmv_follows_set_ham_and_autotune():
-    """After new integrals on a resident determinant-list matrix the sliced
-    copy is rebuilt from the new CRS (a fresh build's product, 1e-13); the
-    autotuner times the sliced shapes for determinant lists and installs the
-    fastest, whose product agrees."""
+    """After new integrals on a resident determinant-list or small-alpha FCI
+    matrix the sliced copy is rebuilt from the new CRS (a fresh build's
+    product, 1e-13), also when the autotuner is the first to read it; the
+    autotuner times the sliced shapes and installs the fastest, whose product
+    agrees."""
     L, na = 8, 4
     h, V = CA.gen_integrals(L, 12)
     h2, V2 = CA.gen_integrals(L, 13)
@@ -360,6 +361,24 @@ def test_sliced_spmv_follows_set_ham_and_autotune():
     best = min(tab, key=lambda t: t[3])
     assert ref.spmv_bytes(CA.CF_F64)[0] == best[0]
     assert np.abs(ref.mvprod(x) - y_new).max() <= 1e-13 * np.abs(y_new).max()
+    # the autotuner on the resident matrix refills its stale sliced copy
+    tab = sel.get_mat().autotune(CA.CF_F64, reps=2)
+    assert 7 in {k for k, _, _, _ in tab}
+    assert np.abs(sel.get_mat().mvprod(x) - y_new).max() <= 1e-13 * np.abs(y_new).max()
+    # FCI with too few alpha strings for the position-major copy (126) gets the
+    # sliced copy at build; new integrals refill it
+    a9, b9 = CA.gen_integrals(9, 5), CA.gen_integrals(9, 6)
+    f = CA.FCI(CA.Hamiltonian(*a9, 4, 4))
+    assert f.get_mat().spmv_bytes(CA.CF_F64)[0] == 7
+    f.set_ham(CA.Hamiltonian(*b9, 4, 4))
+    x9 = np.random.default_rng(13).standard_normal(f.get_dim())
+    y9 = CA.FCI(CA.Hamiltonian(*b9, 4, 4)).get_mat().mvprod(x9)
+    assert np.abs(f.get_mat().mvprod(x9) - y9).max() <= 1e-13 * np.abs(y9).max()
+    f.set_ham(CA.Hamiltonian(*a9, 4, 4))
+    f.set_ham(CA.Hamiltonian(*b9, 4, 4))
+    tab = f.get_mat().autotune(CA.CF_F64, reps=2)
+    assert 7 in {k for k, _, _, _ in tab}
+    assert np.abs(f.get_mat().mvprod(x9) - y9).max() <= 1e-13 * np.abs(y9).max()
 
 
 # ---- CI_Big: determinant lists with the product on the fly --------------------
