@@ -777,7 +777,9 @@ def bench_sigma(a, CA):
 def bench_rotate(a, CA):
     """Batched FP64-MFMA integral rotation (OrbitalTransform::fillHamCI): 64
     Hamiltonians at L=20 per call, 8 L^5 + 4 L^3 flop each; kernel time from HIP
-    events on the library stream, 1 warm-up + 5 timed calls."""
+    events on the library stream, 1 warm-up + 5 timed calls.  At L/8 flop per
+    byte the transform is bound by HBM: the roofline is the bytes of its six
+    passes (each reads and writes its array once)."""
     ctx = CA.Context(0)
     L, B = 20, 64
     h1, V1 = CA.gen_integrals(L, 5)
@@ -792,11 +794,15 @@ def bench_rotate(a, CA):
     nl, ms = ctx.kernel_time("rotate")
     ctx.enable_timing(False)
     flop = B * (8.0 * L ** 5 + 4.0 * L ** 3)
-    tf = flop / (ms / nl * 1e-3) / 1e12
-    log(f"rotate L={L} batch {B}: {ms / nl:.3f} ms per call, {tf:.1f} TFLOP/s")
+    # four two-body and two one-body passes, each reading and writing its array once
+    nbytes = B * (4 * 2 * L ** 4 + 2 * 2 * L ** 2) * 8.0
+    sec = ms / nl * 1e-3
+    tf, gbs = flop / sec / 1e12, nbytes / sec / 1e9
+    log(f"rotate L={L} batch {B}: {ms / nl:.3f} ms per call, {gbs:.0f} GB/s, {tf:.1f} TFLOP/s")
     return {"metric": METRIC, "workload": f"batched integral rotation, L={L}, {B} Hamiltonians per call",
-            "avg_call_ms": ms / nl, "flop_per_call": flop, "TFLOPs": tf,
-            "roofline": {"bound": "mfma", "achieved": tf, "peak": 78.6, "unit": "TFLOP/s", "frac": tf / 78.6}}
+            "avg_call_ms": ms / nl, "flop_per_call": flop, "bytes_per_call": nbytes, "TFLOPs": tf,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "mfma_frac": tf / 78.6}}
 
 
 def bench_batch(a, CA):

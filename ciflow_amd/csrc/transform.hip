@@ -11,9 +11,8 @@
 // the end, out[(jkl), a] = sum_i in[i, (jkl)] U[a, i], so four identical
 // passes return V' in its natural (a,b,c,d) order and every pass is one
 // GEMM (L^3 x L) = (L x L^3)^T (L x L)^T, batched over independent
-// Hamiltonians.  8 L^5 flop per Hamiltonian (C1 symmetry).  Each wavefront
-// computes one 16 x 16 output tile with v_mfma_f64_16x16x4f64, K = L in
-// steps of 4.
+// Hamiltonians.  8 L^5 flop per Hamiltonian (C1 symmetry), on
+// v_mfma_f64_16x16x4f64 with K = L in steps of 4.
 #include "transform.h"
 
 namespace cf {
@@ -23,37 +22,74 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // in:  batch x [K][M]   (leading index contracted), K = L
 // U:   batch x [L][L]   U[a*L + i]
 // out: batch x [M][L]
+// One wavefront per MT x 16 columns m computes the 16 x 16 output tiles of all
+// NA = ceil(L / 16) blocks of the new index: its L x (16 MT) input block is
+// read once (every k-step's loads issued before the MFMAs) and serves every
+// tile.  Each tile accumulates its k-steps in order 0, 4, 8, ... (KS k-steps
+// of registers, KS >= ceil(L / 4)).
+template <int NA, int KS, int MT>
 __global__ void __launch_bounds__(64) k_quarter_mfma(int L, uint64_t M, const double* __restrict__ in,
                                                     const double* __restrict__ U, double* __restrict__ out) {
     const uint64_t b = blockIdx.z;
-    const uint64_t m0 = (uint64_t)blockIdx.x * 16;
-    const int a0 = blockIdx.y * 16;
+    const uint64_t m0 = (uint64_t)blockIdx.x * 16 * MT;
     const int lane = threadIdx.x;
     const int r = lane & 15, kk = lane >> 4;
     const double* inb = in + b * (uint64_t)L * M;
     const double* Ub = U + b * (uint64_t)L * L;
     double* ob = out + b * M * (uint64_t)L;
-    v4d acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < L; k0 += 4) {
-        const int k = k0 + kk;
-        const double av = (k < L && m0 + r < M) ? inb[(uint64_t)k * M + m0 + r] : 0.0;   // A[m][k]
-        const double bv = (k < L && a0 + r < L) ? Ub[(uint64_t)(a0 + r) * L + k] : 0.0;  // B[k][a]
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    const int ks = (L + 3) / 4;
+    double av[MT][KS];  // A[m][k] for m = m0 + 16 u + r, k = 4 s + kk
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const int k = 4 * s + kk;
+#pragma unroll
+        for (int u = 0; u < MT; ++u) {
+            const uint64_t m = m0 + 16 * u + r;
+            av[u][s] = (s < ks && k < L && m < M) ? inb[(uint64_t)k * M + m] : 0.0;
+        }
+    }
+    v4d acc[MT][NA];
+#pragma unroll
+    for (int u = 0; u < MT; ++u)
+#pragma unroll
+        for (int t = 0; t < NA; ++t) acc[u][t] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        if (s < ks) {
+            const int k = 4 * s + kk;
+#pragma unroll
+            for (int t = 0; t < NA; ++t) {
+                const int a = 16 * t + r;
+                const double bv = (k < L && a < L) ? Ub[(uint64_t)a * L + k] : 0.0;  // B[k][a]
+#pragma unroll
+                for (int u = 0; u < MT; ++u)
+                    acc[u][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][s], bv, acc[u][t], 0, 0, 0);
+            }
+        }
     }
     // f64 16x16x4 accumulator layout: D[i][j] sits in lane j + 16*(i%4),
     // element i/4 (row = 4*element + lane/16, column = lane%16)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const uint64_t m = m0 + 4 * v + kk;
-        const int a = a0 + r;
-        if (m < M && a < L) ob[m * L + a] = acc[v];
-    }
+    for (int u = 0; u < MT; ++u)
+#pragma unroll
+        for (int t = 0; t < NA; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const uint64_t m = m0 + 16 * u + 4 * v + kk;
+                const int a = 16 * t + r;
+                if (m < M && a < L) ob[m * L + a] = acc[u][t][v];
+            }
 }
 
 void launch_quarter(int L, uint64_t M, int batch, const double* in, const double* U, double* out, hipStream_t s) {
     if (batch <= 0) return;
-    dim3 grid((unsigned)((M + 15) / 16), (unsigned)((L + 15) / 16), (unsigned)batch);
-    k_quarter_mfma<<<grid, 64, 0, s>>>(L, M, in, U, out);
+    if (L < 1 || L > kMaxL) throw Error(ERR_ARG, "rotation: L out of range");
+    constexpr int MT = 4;
+    const dim3 grid((unsigned)((M + 16 * MT - 1) / (16 * MT)), 1, (unsigned)batch);
+    if (L <= 16) k_quarter_mfma<1, 4, MT><<<grid, 64, 0, s>>>(L, M, in, U, out);
+    else if (L <= 32) k_quarter_mfma<2, 8, MT><<<grid, 64, 0, s>>>(L, M, in, U, out);
+    else if (L <= 48) k_quarter_mfma<3, 12, MT><<<grid, 64, 0, s>>>(L, M, in, U, out);
+    else k_quarter_mfma<4, 16, MT><<<grid, 64, 0, s>>>(L, M, in, U, out);
     CF_HIP(hipGetLastError());
 }
 
