@@ -334,6 +334,30 @@ def test_sliced_spmv_matches_oracle(case):
         m.set_spmv_kernel(7, 32)
 
 
+def test_sliced_spmv_ragged_random_list():
+    """Kernel 7 on 100,000 random determinants of FCI L=14 7a7b in random order
+    (rows of 1 to ~100 entries, so the chunks' padding is large and the
+    launches split their chunks): every shape against the vectorised CRS row
+    gather on the same values (1e-13 relative), FP64 and FP32, and the
+    automatic choice is the sliced kernel."""
+    L, na = 14, 7
+    h, V = CA.gen_integrals(L, 31)
+    S = O.enumerate_strings(L, na)
+    pick = np.random.default_rng(31).choice(len(S) * len(S), 100000, replace=False)
+    up, down = S[pick // len(S)], S[pick % len(S)]
+    m = CA.FCI_File(CA.Hamiltonian(h, V, na, na), (up, down), value_prec=CA.CF_F32).get_mat()
+    assert m.spmv_bytes(CA.CF_F64)[0] == 7 and m.spmv_bytes(CA.CF_F32)[0] == 7
+    x = np.random.default_rng(32).standard_normal(len(up))
+    for prec in (CA.CF_F64, CA.CF_F32):
+        m.set_spmv_shape(prec, 3, 16, 2)
+        y3 = m.mvprod(x, prec)
+        for w, u in ((16, 2), (16, 4), (8, 1), (4, 2), (1, 1)):
+            m.set_spmv_shape(prec, 7, w, u)
+            y7 = m.mvprod(x, prec)
+            assert np.abs(y7 - y3).max() <= 1e-13 * np.abs(y3).max(), (prec, w, u)
+            assert np.array_equal(m.mvprod(x, prec), y7)
+
+
 def test_sliced_spmv_follows_set_ham_and_autotune():
     """After new integrals on a resident determinant-list or small-alpha FCI
     matrix the sliced copy is rebuilt from the new CRS (a fresh build's
