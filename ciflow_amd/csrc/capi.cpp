@@ -1042,11 +1042,13 @@ int cf_space_create_dets(cf_ham* ham, uint64_t n, const uint64_t* up, const uint
     CF_REQUIRE(n >= 1 && n < (1ull << 32), "determinant count must be in [1, 2^32)");
     const int L = ham->L;
     const uint64_t mask = L >= 64 ? ~0ull : ((1ull << L) - 1);
+    uint64_t ascending = 0;
     for (uint64_t i = 0; i < n; ++i) {
         CF_REQUIRE((up[i] & ~mask) == 0 && (down[i] & ~mask) == 0,
                    "determinant " + std::to_string(i) + " occupies an orbital >= L");
         CF_REQUIRE(__builtin_popcountll(up[i]) == ham->nup && __builtin_popcountll(down[i]) == ham->ndown,
                    "determinant " + std::to_string(i) + " does not hold nup alpha and ndown beta electrons");
+        if (i && (up[i] > up[i - 1] || (up[i] == up[i - 1] && down[i] > down[i - 1]))) ++ascending;
     }
     cf_ctx* c = ham->ctx;
     c->use();
@@ -1056,6 +1058,7 @@ int cf_space_create_dets(cf_ham* ham, uint64_t n, const uint64_t* up, const uint
     sp->n = n;
     sp->da = n;
     sp->db = 1;
+    sp->ascending = ascending;
     sp->dup.alloc(n);
     sp->ddown.alloc(n);
     copy_sync(sp->dup.p, up, n * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream);
@@ -1198,18 +1201,22 @@ int cf_crs_build(cf_space* sp, int value_prec, double zero, cf_crs** out) {
             m->auto64 = k5;
         }
     }
-    // Determinant lists, and FCI spaces left on the row gather (too few alpha
+    // Determinant lists in (up, down) order (at least 90% of neighbouring
+    // pairs ascending), and FCI spaces left on the row gather (too few alpha
     // strings for the position-major copy) with >= 8,192 rows, get the sliced
-    // copy (kernel 7) when it fits in half of the free device memory.  Measured
-    // on MI355X (FP64 / FP32 values): CAS-like lists sorted by (up, down),
-    // 101,640 determinants of L=12 0.247 / 0.153 ms against 0.361 / 0.244 ms
-    // (vectorised row gather), 1,029,600 of L=16 4.76 / 2.93 ms against
-    // 6.81 / 4.57 ms, a random 10^5 subset of L=14 0.0157 / 0.0111 ms against
-    // 0.0171 / 0.0126 ms; FCI L=10 (63,504 rows) 0.120 / 0.077 ms against
-    // 0.165 / 0.122 ms, FCI L=9 (15,876) 26 / 16 us against 27 / 24 us.  DOCI
-    // rows (65-82 entries at L=16-18) are too short to gain.
+    // copy (kernel 7) when it fits in half of the free device memory: their
+    // neighbouring rows have neighbouring columns.  Measured on MI355X (FP64 /
+    // FP32 values): CAS-like lists sorted by (up, down), 101,640 determinants
+    // of L=12 0.242 / 0.146 ms against 0.361 / 0.253 ms (vectorised row
+    // gather), 1,029,600 of L=16 4.76 / 2.93 ms against 6.81 / 4.57 ms; FCI
+    // L=10 (63,504 rows) 0.120 / 0.077 ms against 0.165 / 0.122 ms, FCI L=9
+    // (15,876) 26 / 16 us against 27 / 24 us.  In random order the same
+    // CAS-like list runs 8-11% slower on the sliced kernel, a random 10^5 subset
+    // of L=14 15-30%, so unsorted lists keep the row gather.  DOCI rows (65-82
+    // entries at L=16-18) are too short to gain.
     const bool gather = m->auto64.kernel == 1 || m->auto64.kernel == 3;
-    if ((sp->kind == CF_DETS || (sp->kind == CF_FCI && gather && m->n >= 8192)) && make_sell(m.get(), true))
+    const bool sorted_list = sp->kind == CF_DETS && (sp->n <= 1 || 10 * sp->ascending >= 9 * (sp->n - 1));
+    if ((sorted_list || (sp->kind == CF_FCI && gather && m->n >= 8192)) && make_sell(m.get(), true))
         m->auto64 = m->auto32 = SpmvChoice{7, 16, 2, {}, {}};
     *out = m.release();
     API_END

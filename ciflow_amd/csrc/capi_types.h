@@ -163,6 +163,7 @@ struct cf_space {
     bool vec_sign_pending = false;  // sign convention applied when the vector is read
     uint64_t version = 0;  // incremented by cf_space_set_ham
     DBuf<str_t> dup, ddown;  // CF_DETS: the determinant list, list order
+    uint64_t ascending = 0;  // CF_DETS: neighbours (i - 1, i) in ascending (up, down) order
 };
 
 struct cf_sigma {

@@ -207,8 +207,9 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * copy of the CRS rows in chunks of 64 rows with interleaved entries, so one
  * load reads the same position of 64 neighbouring rows (any matrix; builds a
  * second copy of the columns and values; tile_rows > 0 = waves per chunk,
- * 1, 2, 4, 8 or 16).  The DOCI copies are built with matrices of >= 40,000
- * rows, else on request. */
+ * 1, 2, 4, 8 or 16; built with determinant lists in (up, down) order and FCI
+ * spaces of >= 8,192 rows with < 400 alpha strings, else on request).  The
+ * DOCI copies are built with matrices of >= 40,000 rows, else on request. */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
 /* Kernel the next product with value_prec would run (1-7) and its algorithmic
  * HBM bytes (matrix stream + x + y; kernel 6: diagonal + x + y; kernel 7: the

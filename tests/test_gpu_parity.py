@@ -338,15 +338,19 @@ def test_sliced_spmv_ragged_random_list():
     """Kernel 7 on 100,000 random determinants of FCI L=14 7a7b in random order
     (rows of 1 to ~100 entries, so the chunks' padding is large and the
     launches split their chunks): every shape against the vectorised CRS row
-    gather on the same values (1e-13 relative), FP64 and FP32, and the
-    automatic choice is the sliced kernel."""
+    gather on the same values (1e-13 relative), FP64 and FP32.  The list is
+    not in (up, down) order, so the automatic choice stays the row gather."""
     L, na = 14, 7
     h, V = CA.gen_integrals(L, 31)
     S = O.enumerate_strings(L, na)
     pick = np.random.default_rng(31).choice(len(S) * len(S), 100000, replace=False)
     up, down = S[pick // len(S)], S[pick % len(S)]
     m = CA.FCI_File(CA.Hamiltonian(h, V, na, na), (up, down), value_prec=CA.CF_F32).get_mat()
-    assert m.spmv_bytes(CA.CF_F64)[0] == 7 and m.spmv_bytes(CA.CF_F32)[0] == 7
+    assert m.spmv_bytes(CA.CF_F64)[0] != 7 and m.spmv_bytes(CA.CF_F32)[0] != 7
+    order = np.lexsort((down, up))
+    sorted_m = CA.FCI_File(CA.Hamiltonian(h, V, na, na), (up[order], down[order])).get_mat()
+    assert sorted_m.spmv_bytes(CA.CF_F64)[0] == 7
+    del sorted_m
     x = np.random.default_rng(32).standard_normal(len(up))
     for prec in (CA.CF_F64, CA.CF_F32):
         m.set_spmv_shape(prec, 3, 16, 2)
