@@ -155,7 +155,7 @@ struct cf_crs {
     DBuf<float> sval32;
     DBuf<double> spart;  // range sums of a split launch (kSellMaxSplit x n, with fewer than 2^21 rows)
     bool sell_ready = false;
-    int sell_waves = 4;  // waves per chunk when kernel 7 is set explicitly
+    int sell_waves = 16;  // waves per chunk when kernel 7 is set explicitly
     std::unique_ptr<FciSpmvPlan> plan;
 };
 
@@ -851,7 +851,7 @@ SpmvChoice make_choice(int kernel, int p1, int p2, uint64_t db, int prec) {
         ch.doci.threads = p1 > 0 ? p1 : DociShape{}.threads;
         ch.doci.sub = p2;
     } else if (kernel == 7) {
-        ch.lanes = p1 > 0 ? p1 : 4;
+        ch.lanes = p1 > 0 ? p1 : 16;
         ch.unroll = p2 > 0 ? p2 : 2;
     } else {
         ch.lanes = p1;
@@ -1548,7 +1548,7 @@ int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows) {
             CF_REQUIRE(tile_rows == 1 || tile_rows == 2 || tile_rows == 4 || tile_rows == 8 || tile_rows == 16,
                        "kernel 7: waves per chunk must be 1, 2, 4, 8 or 16");
         make_sell(m);
-        m->sell_waves = tile_rows > 0 ? tile_rows : 4;
+        m->sell_waves = tile_rows > 0 ? tile_rows : 16;
     }
     if (kind == 2) {
         CF_REQUIRE(m->sp->kind == CF_FCI, "the staged SpMV needs an FCI matrix");

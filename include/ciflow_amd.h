@@ -206,8 +206,8 @@ int cf_mvprod_prec(cf_crs* m, int value_prec, const double* x, double* y);   /* 
  * 0 = low orbitals w, 1..12, with 1..20 high orbitals), 7 = sliced rows: a
  * copy of the CRS rows in chunks of 64 rows with interleaved entries, so one
  * load reads the same position of 64 neighbouring rows (any matrix; builds a
- * second copy of the columns and values; tile_rows > 0 = waves per chunk,
- * 1, 2, 4, 8 or 16; built with determinant lists in (up, down) order and FCI
+ * second copy of the columns and values; tile_rows = waves per chunk,
+ * 1, 2, 4, 8 or 16, 0 = 16; built with determinant lists in (up, down) order and FCI
  * spaces of >= 8,192 rows with < 400 alpha strings, else on request).  The
  * DOCI copies are built with matrices of >= 40,000 rows, else on request. */
 int cf_crs_set_spmv_kernel(cf_crs* m, int kind, int tile_rows);
@@ -223,7 +223,7 @@ int cf_crs_spmv_bytes(const cf_crs* m, int value_prec, uint64_t* bytes, int* ker
  * d_beta; 0 = the single-buffered kernel with p1 in 256/512/1024); kernel 5:
  * p1 = workgroup size (multiple of 64, <= 1024, 0 = 256), p2 = lanes per row
  * (1, 2, 4 or 8); kernel 6: p1 = low orbitals w (0 = min(L/2, 12)); kernel 7:
- * p1 = waves per 64-row chunk (1, 2, 4, 8, 16; 0 = 4), p2 = quads in flight
+ * p1 = waves per 64-row chunk (1, 2, 4, 8, 16; 0 = 16), p2 = quads in flight
  * per lane (1, 2, or 4 with 4-16 waves; 0 = 2), building the sliced copy if
  * it does not exist.  The shape
  * becomes the automatic choice for value_prec and the matrix returns to
